@@ -1,0 +1,106 @@
+/*
+ * ps_capi.h -- C ABI of the MI355X solve path of the Poisson solver.
+ *
+ * Plain C: opaque handle, plain pointers and sizes, integer status codes.
+ * No exception crosses this boundary.  One context owns one HIP stream and is
+ * not thread-safe (like the reference object); distinct contexts may be used
+ * from distinct threads.  Each entry point names the reference interface it
+ * replaces (paths relative to the reference checkout, c++/source/).
+ *
+ * Single-GPU semantics: nproc = 1, so the partition list has one entry that
+ * covers the whole grid.  Host arrays are real double, x fastest,
+ * ijk = (k*ny + j)*nx + i, as in push.cpp:82-90.
+ */
+#ifndef PS_CAPI_H
+#define PS_CAPI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ps_ctx ps_ctx;
+
+/* status codes */
+enum {
+    PS_OK = 0,
+    PS_EINVAL = 1,        /* bad argument (NULL handle, non-positive size, ...) */
+    PS_EBC = 2,           /* unsupported boundary-condition tuple (greens3d.cpp:522-541) */
+    PS_ESTATE = 3,        /* call out of order, or rhs missing for the selected operator
+                             (solve3d.cpp:57-60, :71-74, :87-90) */
+    PS_ENOMEM = 4,        /* device allocation failed */
+    PS_EHIP = 5,          /* HIP runtime error, or no device */
+    PS_EUNSUPPORTED = 6   /* grid length with a prime factor above 7, 2D solver */
+};
+
+/* precision of the solve (extension; the reference is FP64 only) */
+enum {
+    PS_FP64 = 0,          /* FP64 storage and arithmetic (the reference's precision) */
+    PS_FP32 = 1,          /* FP32 storage and arithmetic, FP64-built kernel */
+    PS_MIXED = 2          /* FP32 storage, MFMA DFT stages with split operands */
+};
+
+/* replaces poisson_solver::poisson_solver() (poisson_solver.hpp:107) */
+int ps_create(ps_ctx** out);
+/* replaces poisson_solver::~poisson_solver() (poisson_solver.hpp:108, destructor.cpp:8-90) */
+int ps_destroy(ps_ctx* ctx);
+
+/* extension: solve precision, set before ps_setup3d (default PS_FP64) */
+int ps_set_precision(ps_ctx* ctx, int precision);
+
+/* replaces poisson_solver::setup3d(int[3], int[3], double[3], int)
+   (poisson_solver.hpp:111, setup3d.cpp:8-71): ncell, bc (0 unbounded,
+   1 periodic), cell size, regularisation order */
+int ps_setup3d(ps_ctx* ctx, const int ncell[3], const int bc[3], const double dx[3], int reg_order);
+/* replaces poisson_solver::setup2d (poisson_solver.hpp:110); not built: PS_EUNSUPPORTED */
+int ps_setup2d(ps_ctx* ctx, const int ncell[2], const int bc[2], const double dx[2], int reg_order);
+
+/* replace set_return_grad/div/curl and set_reprojection (settings.cpp:9-30) */
+int ps_set_return_grad(ps_ctx* ctx, int toggle);
+int ps_set_return_div(ps_ctx* ctx, int toggle);
+int ps_set_return_curl(ps_ctx* ctx, int toggle);
+int ps_set_reprojection(ps_ctx* ctx, int toggle);
+
+/* replaces poisson_solver::push(double*, double*, double*) (push.cpp:9-397);
+   NULL = component absent */
+int ps_push(ps_ctx* ctx, const double* bx, const double* by, const double* bz);
+/* replaces poisson_solver::solve3d() (solve3d.cpp:8-635) */
+int ps_solve3d(ps_ctx* ctx);
+/* replaces poisson_solver::solve2d() (poisson_solver.hpp:118); not built: PS_EUNSUPPORTED */
+int ps_solve2d(ps_ctx* ctx);
+/* replaces poisson_solver::pull(double* x6) (pull.cpp:9-469): lhs X,Y,Z then
+   rhs X,Y,Z; a component is written only when the pointer is non-NULL and the
+   solver holds that component */
+int ps_pull(ps_ctx* ctx, double* ax, double* ay, double* az, double* bx, double* by, double* bz);
+
+/* replaces the public member std::vector<class_partition> partition
+   (poisson_solver.hpp:102, partition_setup.cpp:9-199) */
+int ps_partition_count(const ps_ctx* ctx);
+int ps_partition(const ps_ctx* ctx, int rank, int ncell[3], int icell[3], double dx[3]);
+
+/* status text; the last error message recorded on a context */
+const char* ps_strerror(int status);
+const char* ps_last_error(const ps_ctx* ctx);
+
+/* --- extensions: device-resident data, streams, timing (no reference counterpart) --- */
+/* real element type of device arrays: double for PS_FP64, float otherwise */
+int ps_real_bytes(const ps_ctx* ctx);
+/* device pointers of that element type, same layout as the host arrays */
+int ps_push_device(ps_ctx* ctx, const void* bx, const void* by, const void* bz);
+int ps_pull_device(ps_ctx* ctx, void* ax, void* ay, void* az, void* bx, void* by, void* bz);
+/* the context's HIP stream (hipStream_t) */
+void* ps_stream(ps_ctx* ctx);
+int ps_synchronize(ps_ctx* ctx);
+/* run `iters` solves and time each pass with HIP events on the context's
+   stream: ms[0..4] = x, y, z, y^-1, x^-1 pass averages, ms[5] = whole solve */
+int ps_profile_solve(ps_ctx* ctx, int iters, double ms[6]);
+/* describe the plan chosen for transform length L: E, threads per pencil,
+   number of stages and radices (up to 14); returns PS_EUNSUPPORTED if none */
+int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int* nstage, int radix[14]);
+/* evaluate the device Si (which = 0) or Ji0 (which = 1) at n host arguments */
+int ps_eval_special(const double* x, double* y, long n, int which);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PS_CAPI_H */

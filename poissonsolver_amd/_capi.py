@@ -1,0 +1,69 @@
+"""ctypes binding of the C ABI (include/ps_capi.h) of libpoissonsolver_amd.so.
+
+The library is the in-tree build (python -m poissonsolver_amd.build).  There is
+no fallback: if the shared library is missing, importing raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpoissonsolver_amd.so")
+
+PS_OK, PS_EINVAL, PS_EBC, PS_ESTATE, PS_ENOMEM, PS_EHIP, PS_EUNSUPPORTED = range(7)
+PS_FP64, PS_FP32, PS_MIXED = 0, 1, 2
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "ps_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "ps_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_set_precision": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_setup3d": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_dbl_p, ctypes.c_int]),
+    "ps_setup2d": (ctypes.c_int, [ctypes.c_void_p, _c_int_p, _c_int_p, _c_dbl_p, ctypes.c_int]),
+    "ps_set_return_grad": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_set_return_div": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_set_return_curl": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_set_reprojection": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_push": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 3),
+    "ps_solve3d": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_solve2d": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_pull": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 6),
+    "ps_partition_count": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_partition": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p]),
+    "ps_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "ps_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "ps_real_bytes": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_push_device": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 3),
+    "ps_pull_device": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 6),
+    "ps_stream": (ctypes.c_void_p, [ctypes.c_void_p]),
+    "ps_synchronize": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_profile_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p]),
+    "ps_plan_describe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
+                                        _c_int_p, _c_int_p]),
+    "ps_eval_special": (ctypes.c_int, [_c_dbl_p, _c_dbl_p, ctypes.c_long, ctypes.c_int]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load the shared library once; raise if it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libpoissonsolver_amd.so is not built: run python -m poissonsolver_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def strerror(status: int) -> str:
+    return lib().ps_strerror(status).decode()

@@ -1,0 +1,723 @@
+// Solver context and C ABI (include/ps_capi.h).
+//
+// One context = one reference `poisson_solver` object on one GPU.  The
+// per-solve pipeline is five kernel launches on the context's stream:
+//   x pass   real rows -> half spectra, R2C two rows per pencil   (solve3d.cpp:118-173)
+//   y pass   zero-padded My pencils                               (solve3d.cpp:183-237)
+//   z pass   pad -> FFT -> Ghat*zeta*operator -> IFFT -> truncate (solve3d.cpp:243-441)
+//   y^-1     inverse, keep ny                                     (solve3d.cpp:447-536)
+//   x^-1     C2R, keep nx, real part                              (solve3d.cpp:541-629, pull.cpp:101-131)
+// The two MPI transposes of the reference (map.cpp) are the strides of the
+// strided passes: nothing is moved between passes except by the passes.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/ps_capi.h"
+#include "ps_greens.hpp"
+#include "ps_launch.hpp"
+#include "ps_plan.hpp"
+
+using namespace psamd;
+
+namespace {
+
+struct DevPlan {
+    Plan1D p;
+    void* tw = nullptr;      // complex T table of length L
+    int tile = 1;            // B (strided) or P (x pass)
+};
+
+}  // namespace
+
+struct ps_ctx {
+    int prec = PS_FP64;
+    bool grad = false, div = false, curl = false, reproj = false;
+    bool ready = false;
+    int ncell[3] = {0, 0, 0}, bc[3] = {0, 0, 0};
+    double dx[3] = {0, 0, 0};
+    int reg = 0;
+    int M[3] = {0, 0, 0}, O[3] = {0, 0, 0};
+    int H = 0, Hp = 0;
+    long n3 = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    void* rhs = nullptr;    // 3 x n3 real, pushed sources
+    void* lhs = nullptr;    // 3 x n3 real, solutions
+    void* rhsf = nullptr;   // 3 x n3 real, filtered sources (computed on demand)
+    void* A1 = nullptr;     // ncomp_alloc x nz x ny x Hp complex
+    void* A2 = nullptr;     // ncomp_alloc x nz x My x Hp complex
+    long A1c = 0, A2c = 0;  // complex elements per component
+    int ncomp_alloc = 0;
+    void* G = nullptr;      // octant of Ghat*zeta/N, real
+
+    DevPlan px, py, pz1, pz3;
+
+    bool rhs_present[3] = {false, false, false};
+    bool lhs_present[3] = {false, false, false};
+    bool solved_after_push = false;
+    bool rhsf_valid = false;
+    bool solve_filter_active = false;   // reg > 0 or reprojection, at the last solve
+    bool solve_reproj = false;
+
+    hipEvent_t ev[6] = {};
+    bool ev_made = false;
+
+    std::string err;
+};
+
+namespace {
+
+int real_bytes(const ps_ctx* c) { return c->prec == PS_FP64 ? 8 : 4; }
+
+int fail(ps_ctx* c, int status, const std::string& msg)
+{
+    if (c) c->err = msg;
+    return status;
+}
+
+#define CK(call)                                                                                 \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail(c, e_ == hipErrorOutOfMemory ? PS_ENOMEM : PS_EHIP,                      \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                      \
+    } while (0)
+
+void free_ptr(void*& p)
+{
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void release(ps_ctx* c)
+{
+    free_ptr(c->rhs);
+    free_ptr(c->lhs);
+    free_ptr(c->rhsf);
+    free_ptr(c->A1);
+    free_ptr(c->A2);
+    free_ptr(c->G);
+    for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) free_ptr(d->tw);
+    c->ncomp_alloc = 0;
+    c->ready = false;
+}
+
+template <typename T> int upload_twiddles(ps_ctx* c, DevPlan* d)
+{
+    std::vector<double> re, im;
+    twiddles(d->p.L, re, im);
+    std::vector<T> h(2 * (size_t)d->p.L);
+    for (int j = 0; j < d->p.L; ++j) {
+        h[2 * j] = (T)re[j];
+        h[2 * j + 1] = (T)im[j];
+    }
+    CK(hipMalloc(&d->tw, h.size() * sizeof(T)));
+    CK(hipMemcpy(d->tw, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PS_OK;
+}
+
+int plan_dim(ps_ctx* c, DevPlan* d, int L, int ncomp, const char* what)
+{
+    std::string why;
+    if (!make_plan(L, max_E_for(real_bytes(c), ncomp), &d->p, &why))
+        return fail(c, PS_EUNSUPPORTED, std::string(what) + ": " + why);
+    return PS_OK;
+}
+
+int ensure_comp_buffers(ps_ctx* c, int ncomp)
+{
+    if (ncomp <= c->ncomp_alloc) return PS_OK;
+    free_ptr(c->A1);
+    free_ptr(c->A2);
+    const size_t cb = 2 * (size_t)real_bytes(c);
+    CK(hipMalloc(&c->A1, (size_t)ncomp * c->A1c * cb));
+    CK(hipMalloc(&c->A2, (size_t)ncomp * c->A2c * cb));
+    c->ncomp_alloc = ncomp;
+    return PS_OK;
+}
+
+int ensure_real(ps_ctx* c, void** p)
+{
+    if (*p) return PS_OK;
+    CK(hipMalloc(p, 3 * (size_t)c->n3 * real_bytes(c)));
+    return PS_OK;
+}
+
+// Operator selection of solve3d.cpp:43-107 (grad > div > curl > plain).
+struct OpSel {
+    int op = OP_PLAIN;
+    int nc = 1;                 // 1: component-wise kernel, 3: mixes components
+    int in_first = 0, in_step = 1, nin = 0;
+    int out_first = 0, out_step = 1, nout = 0;
+};
+
+bool arith(const bool present[3], int* first, int* step, int* n)
+{
+    int idx[3], k = 0;
+    for (int d = 0; d < 3; ++d)
+        if (present[d]) idx[k++] = d;
+    *n = k;
+    if (k == 0) return false;
+    *first = idx[0];
+    *step = (k >= 2) ? idx[1] - idx[0] : 1;
+    return true;
+}
+
+int select_op(ps_ctx* c, OpSel* s)
+{
+    const bool rX = c->rhs_present[0], rY = c->rhs_present[1], rZ = c->rhs_present[2];
+    if (!arith(c->rhs_present, &s->in_first, &s->in_step, &s->nin))
+        return fail(c, PS_ESTATE, "[poisson_solver.solve3d]: no rhs field has been pushed");
+    if (c->grad) {
+        if (!rX)
+            return fail(c, PS_ESTATE,
+                        "[poisson_solver.solve3d]: Error rhs fields have not been pushed correctly for grad option");
+        if (!(rY && rZ))
+            return fail(c, PS_ESTATE,
+                        "[poisson_solver.solve3d]: grad is component-wise and needs rhs X, Y and Z");
+        s->op = OP_GRAD;
+        s->nc = 1;
+        s->out_first = 0; s->out_step = 1; s->nout = 3;
+    } else if (c->div) {
+        if (!(rX && rY && rZ))
+            return fail(c, PS_ESTATE,
+                        "[poisson_solver.solve3d]: Error rhs fields have not been pushed correctly for div option");
+        s->op = OP_DIV;
+        s->nc = 3;
+        s->out_first = 0; s->out_step = 1; s->nout = 1;
+    } else if (c->curl) {
+        if (!(rX && rY && rZ))
+            return fail(c, PS_ESTATE,
+                        "[poisson_solver.solve3d]: Error rhs fields have not been pushed correctly for curl option");
+        s->op = OP_CURL;
+        s->nc = 3;
+        s->out_first = 0; s->out_step = 1; s->nout = 3;
+    } else {
+        s->op = OP_PLAIN;
+        s->nc = 1;
+        s->out_first = s->in_first; s->out_step = s->in_step; s->nout = s->nin;
+    }
+    return PS_OK;
+}
+
+template <typename T>
+void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
+{
+    a.plan = device_plan<T>(d.p, reinterpret_cast<const cplx<T>*>(d.tw));
+    a.in = a.out = c->A2;
+    a.in_comp = a.out_comp = c->A2c;
+    a.in_slab = a.out_slab = c->Hp;                  // slab = ky
+    a.in_pos = a.out_pos = (long)c->M[1] * c->Hp;    // pos = kz
+    a.in_col = a.out_col = 1;                         // col = kx
+    a.n_in = a.n_out = c->ncell[2];
+    a.ncol = c->H;
+    a.B = d.tile;
+    a.ncoltile = (c->H + a.B - 1) / a.B;
+    a.nslab = c->M[1];
+    a.load_mode = LOAD_COMPLEX;
+    a.store_mode = STORE_COMPLEX;
+    a.do_fwd = a.do_inv = 1;
+    a.G = reinterpret_cast<const T*>(c->G);
+    a.g_ky = c->O[0];
+    a.g_kz = (long)c->O[0] * c->O[1];
+    a.Mx = c->M[0];
+    a.My = c->M[1];
+    a.Mz = c->M[2];
+    for (int k = 0; k < 3; ++k) a.dk[k] = 1.0 / ((double)c->M[k] * c->dx[k]);
+    const double h = std::max(std::max(c->dx[0], c->dx[1]), c->dx[2]);
+    a.zarg = 0.5 * (2.0 * h) * (2.0 * h);
+    int regm = c->reg;
+    if (regm > 0 && regm % 2) regm += 1;
+    a.reg = regm > 0 ? regm : 0;
+    a.invN = 1.0 / ((double)c->M[0] * c->M[1] * c->M[2]);
+}
+
+// The five passes.  `src` holds nin real components starting at in_first
+// with step in_step; `dst` receives nout components at out_first/out_step.
+template <typename T>
+int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin, int op, int nc, void* dst,
+                 int out_first, int out_step, int nout, hipEvent_t* ev)
+{
+    hipStream_t s = c->stream;
+    const int nx = c->ncell[0], ny = c->ncell[1], nz = c->ncell[2];
+    int r = ensure_comp_buffers(c, std::max(nin, nc));
+    if (r) return r;
+    if (ev) CK(hipEventRecord(ev[0], s));
+
+    XArgs<T> xa{};
+    xa.plan = device_plan<T>(c->px.p, reinterpret_cast<const cplx<T>*>(c->px.tw));
+    xa.in = reinterpret_cast<const T*>(src) + (long)in_first * c->n3;
+    xa.out = c->A1;
+    xa.r_comp = (long)in_step * c->n3;
+    xa.r_row = nx;
+    xa.c_comp = c->A1c;
+    xa.c_row = c->Hp;
+    xa.nrows = ny * nz;
+    xa.nvalid = nx;
+    xa.H = c->H;
+    xa.P = c->px.tile;
+    xa.npairs = (ny * nz + 1) / 2;
+    xa.ncomp = nin;
+    CK(launch_x_r2c<T>(c->px.p.E, xa, s));
+    if (ev) CK(hipEventRecord(ev[1], s));
+
+    PencilArgs<T> ya{};
+    ya.plan = device_plan<T>(c->py.p, reinterpret_cast<const cplx<T>*>(c->py.tw));
+    ya.in = c->A1;
+    ya.out = c->A2;
+    ya.in_comp = c->A1c;
+    ya.in_slab = (long)ny * c->Hp;
+    ya.in_pos = c->Hp;
+    ya.in_col = 1;
+    ya.out_comp = c->A2c;
+    ya.out_slab = (long)c->M[1] * c->Hp;
+    ya.out_pos = c->Hp;
+    ya.out_col = 1;
+    ya.n_in = ny;
+    ya.n_out = c->M[1];
+    ya.ncol = c->H;
+    ya.B = c->py.tile;
+    ya.ncoltile = (c->H + ya.B - 1) / ya.B;
+    ya.nslab = nz;
+    ya.ncomp = nin;
+    ya.load_mode = LOAD_COMPLEX;
+    ya.store_mode = STORE_COMPLEX;
+    ya.do_fwd = 1;
+    ya.do_inv = 0;
+    ya.op = OP_NONE;
+    CK(launch_pencil<T>(c->py.p.E, 1, ya, s));
+    if (ev) CK(hipEventRecord(ev[2], s));
+
+    const DevPlan& zp = (nc == 3) ? c->pz3 : c->pz1;
+    PencilArgs<T> za{};
+    fill_zargs<T>(c, za, zp);
+    za.op = op;
+    za.ncomp = (nc == 3) ? 1 : nin;
+    CK(launch_pencil<T>(zp.p.E, nc, za, s));
+    if (ev) CK(hipEventRecord(ev[3], s));
+
+    PencilArgs<T> yi = ya;
+    yi.in = c->A2;
+    yi.out = c->A1;
+    yi.in_comp = c->A2c;
+    yi.in_slab = (long)c->M[1] * c->Hp;
+    yi.out_comp = c->A1c;
+    yi.out_slab = (long)ny * c->Hp;
+    yi.n_in = c->M[1];
+    yi.n_out = ny;
+    yi.ncomp = nout;
+    yi.do_fwd = 0;
+    yi.do_inv = 1;
+    CK(launch_pencil<T>(c->py.p.E, 1, yi, s));
+    if (ev) CK(hipEventRecord(ev[4], s));
+
+    XArgs<T> xi = xa;
+    xi.in = c->A1;
+    xi.out = reinterpret_cast<T*>(dst) + (long)out_first * c->n3;
+    xi.r_comp = (long)out_step * c->n3;
+    xi.ncomp = nout;
+    CK(launch_x_c2r<T>(c->px.p.E, xi, s));
+    if (ev) CK(hipEventRecord(ev[5], s));
+    return PS_OK;
+}
+
+template <typename T> int setup_impl(ps_ctx* c)
+{
+    const int nx = c->ncell[0], ny = c->ncell[1], nz = c->ncell[2];
+    for (int d = 0; d < 3; ++d) {
+        c->M[d] = (c->bc[d] == 0) ? 2 * c->ncell[d] : c->ncell[d];     // partition_setup.cpp:71-96
+        c->O[d] = c->M[d] / 2 + 1;
+    }
+    c->H = c->M[0] / 2 + 1;
+    const int align = (real_bytes(c) == 4) ? 16 : 8;                    // 128-byte rows
+    c->Hp = (c->H + align - 1) / align * align;
+    c->n3 = (long)nx * ny * nz;
+    c->A1c = (long)nz * ny * c->Hp;
+    c->A2c = (long)nz * c->M[1] * c->Hp;
+
+    int r;
+    if ((r = plan_dim(c, &c->px, c->M[0], 1, "x transform"))) return r;
+    if ((r = plan_dim(c, &c->py, c->M[1], 1, "y transform"))) return r;
+    if ((r = plan_dim(c, &c->pz1, c->M[2], 1, "z transform"))) return r;
+    if ((r = plan_dim(c, &c->pz3, c->M[2], 3, "z transform (3 components)"))) return r;
+    const int eb = real_bytes(c);
+    c->px.tile = choose_P(c->px.p, eb, (ny * nz + 1) / 2);
+    c->py.tile = choose_B(c->py.p, eb, c->H);
+    c->pz1.tile = choose_B(c->pz1.p, eb, c->H);
+    c->pz3.tile = choose_B(c->pz3.p, eb, c->H);
+    for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3})
+        if ((r = upload_twiddles<T>(c, d))) return r;
+
+    CK(hipMalloc(&c->G, (size_t)c->O[0] * c->O[1] * c->O[2] * sizeof(T)));
+    r = build_greens<T>(c->ncell, c->bc, c->dx, c->reg, reinterpret_cast<T*>(c->G), c->stream, &c->err);
+    if (r) return r;
+    return PS_OK;
+}
+
+template <typename T> int solve_impl(ps_ctx* c, hipEvent_t* ev)
+{
+    OpSel s;
+    int r = select_op(c, &s);
+    if (r) return r;
+    if ((r = ensure_real(c, &c->lhs))) return r;
+    r = run_pipeline<T>(c, c->rhs, s.in_first, s.in_step, s.nin, s.op, s.nc, c->lhs, s.out_first, s.out_step,
+                        s.nout, ev);
+    if (r) return r;
+    for (int k = 0; k < s.nout; ++k) c->lhs_present[s.out_first + k * s.out_step] = true;
+    c->solved_after_push = true;
+    c->rhsf_valid = false;
+    const bool all3 = c->rhs_present[0] && c->rhs_present[1] && c->rhs_present[2];
+    c->solve_reproj = c->reproj && all3;                          // solve3d.cpp:388
+    c->solve_filter_active = (c->reg > 0) || c->solve_reproj;
+    return PS_OK;
+}
+
+// The filtered (and reprojected) rhs the reference leaves in rhs* after a
+// solve (solve3d.cpp:329-343, :388-396, :403-424), computed when pulled.
+template <typename T> int rhs_filter_impl(ps_ctx* c)
+{
+    int first, step, n;
+    if (!arith(c->rhs_present, &first, &step, &n)) return PS_OK;
+    int r;
+    if ((r = ensure_real(c, &c->rhsf))) return r;
+    const int op = c->solve_reproj ? OP_RHSF_PROJ : OP_RHSF;
+    const int nc = c->solve_reproj ? 3 : 1;
+    r = run_pipeline<T>(c, c->rhs, first, step, n, op, nc, c->rhsf, first, step, n, nullptr);
+    if (r) return r;
+    CK(hipStreamSynchronize(c->stream));
+    c->rhsf_valid = true;
+    return PS_OK;
+}
+
+template <typename T> int push_host(ps_ctx* c, const double* const b[3])
+{
+    std::vector<T> tmp;
+    for (int d = 0; d < 3; ++d) {
+        if (!b[d]) continue;
+        T* dst = reinterpret_cast<T*>(c->rhs) + (long)d * c->n3;
+        if (sizeof(T) == sizeof(double)) {
+            CK(hipMemcpy(dst, b[d], c->n3 * sizeof(double), hipMemcpyHostToDevice));
+        } else {
+            tmp.resize(c->n3);
+            for (long i = 0; i < c->n3; ++i) tmp[i] = (T)b[d][i];
+            CK(hipMemcpy(dst, tmp.data(), c->n3 * sizeof(T), hipMemcpyHostToDevice));
+        }
+    }
+    return PS_OK;
+}
+
+template <typename T> int pull_host(ps_ctx* c, double* dst, const void* src_base, int d)
+{
+    const T* src = reinterpret_cast<const T*>(src_base) + (long)d * c->n3;
+    if (sizeof(T) == sizeof(double)) {
+        CK(hipMemcpy(dst, src, c->n3 * sizeof(double), hipMemcpyDeviceToHost));
+    } else {
+        std::vector<T> tmp(c->n3);
+        CK(hipMemcpy(tmp.data(), src, c->n3 * sizeof(T), hipMemcpyDeviceToHost));
+        for (long i = 0; i < c->n3; ++i) dst[i] = (double)tmp[i];
+    }
+    return PS_OK;
+}
+
+int check_ready(ps_ctx* c)
+{
+    if (!c) return PS_EINVAL;
+    if (!c->ready) return fail(c, PS_ESTATE, "setup3d has not been called");
+    return PS_OK;
+}
+
+// Where rhs component d is read from by pull: the pushed field unless a solve
+// since the last push applied a filter (regularisation or reprojection).
+int rhs_source(ps_ctx* c, const void** src)
+{
+    *src = c->rhs;
+    if (!c->solved_after_push || !c->solve_filter_active) return PS_OK;
+    if (!c->rhsf_valid) {
+        int r = (c->prec == PS_FP64) ? rhs_filter_impl<double>(c) : rhs_filter_impl<float>(c);
+        if (r) return r;
+    }
+    *src = c->rhsf;
+    return PS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ps_create(ps_ctx** out)
+{
+    if (!out) return PS_EINVAL;
+    *out = new (std::nothrow) ps_ctx();
+    return *out ? PS_OK : PS_ENOMEM;
+}
+
+int ps_destroy(ps_ctx* c)
+{
+    if (!c) return PS_EINVAL;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    if (c->ev_made)
+        for (hipEvent_t& e : c->ev) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return PS_OK;
+}
+
+int ps_set_precision(ps_ctx* c, int precision)
+{
+    if (!c) return PS_EINVAL;
+    if (precision != PS_FP64 && precision != PS_FP32 && precision != PS_MIXED)
+        return fail(c, PS_EINVAL, "unknown precision");
+    if (c->ready) return fail(c, PS_ESTATE, "set the precision before setup3d");
+    c->prec = precision;
+    return PS_OK;
+}
+
+int ps_setup3d(ps_ctx* c, const int ncell[3], const int bc[3], const double dx[3], int reg_order)
+{
+    if (!c) return PS_EINVAL;
+    if (!ncell || !bc || !dx) return fail(c, PS_EINVAL, "setup3d: NULL argument");
+    for (int d = 0; d < 3; ++d) {
+        if (ncell[d] < 1) return fail(c, PS_EINVAL, "setup3d: ncell must be positive");
+        if (!(dx[d] > 0.0)) return fail(c, PS_EINVAL, "setup3d: dx must be positive");
+        if (bc[d] != 0 && bc[d] != 1) return fail(c, PS_EBC, "setup3d: bc must be 0 (unbounded) or 1 (periodic)");
+    }
+    const bool uuu = !bc[0] && !bc[1] && !bc[2], ppu = bc[0] && bc[1] && !bc[2];
+    const bool uup = !bc[0] && !bc[1] && bc[2], ppp = bc[0] && bc[1] && bc[2];
+    if (!(uuu || ppu || uup || ppp)) {
+        const int sum = bc[0] + bc[1] + bc[2];
+        if (sum == 1)
+            return fail(c, PS_EBC, "[poisson_solver.greens3d]: For unbounded-unbounded-periodic domains the "
+                                   "z-direction must be the periodic direction");
+        return fail(c, PS_EBC, "[poisson_solver.greens3d]: For unbounded-periodic-periodic domains the "
+                               "z-direction must be the unbounded direction");
+    }
+    if (reg_order < 0) return fail(c, PS_EINVAL, "setup3d: reg_order must be >= 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(c, PS_EHIP, "no HIP device available");
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    CK(hipGetDevice(&c->device));
+    if (!c->stream) CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int d = 0; d < 3; ++d) {
+        c->ncell[d] = ncell[d];
+        c->bc[d] = bc[d];
+        c->dx[d] = dx[d];
+        c->rhs_present[d] = c->lhs_present[d] = false;
+    }
+    c->reg = reg_order;
+    c->solved_after_push = c->rhsf_valid = false;
+    int r = (c->prec == PS_FP64) ? setup_impl<double>(c) : setup_impl<float>(c);
+    if (r) {
+        release(c);
+        return r;
+    }
+    c->ready = true;
+    return PS_OK;
+}
+
+int ps_setup2d(ps_ctx* c, const int*, const int*, const double*, int)
+{
+    return fail(c, PS_EUNSUPPORTED, "setup2d: the 2D solver is not part of this library");
+}
+
+int ps_set_return_grad(ps_ctx* c, int t) { if (!c) return PS_EINVAL; c->grad = t != 0; return PS_OK; }
+int ps_set_return_div(ps_ctx* c, int t) { if (!c) return PS_EINVAL; c->div = t != 0; return PS_OK; }
+int ps_set_return_curl(ps_ctx* c, int t) { if (!c) return PS_EINVAL; c->curl = t != 0; return PS_OK; }
+int ps_set_reprojection(ps_ctx* c, int t) { if (!c) return PS_EINVAL; c->reproj = t != 0; return PS_OK; }
+
+int ps_push(ps_ctx* c, const double* bx, const double* by, const double* bz)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if ((r = ensure_real(c, &c->rhs))) return r;
+    const double* b[3] = {bx, by, bz};
+    r = (c->prec == PS_FP64) ? push_host<double>(c, b) : push_host<float>(c, b);
+    if (r) return r;
+    for (int d = 0; d < 3; ++d)
+        if (b[d]) c->rhs_present[d] = true;
+    c->solved_after_push = false;
+    c->rhsf_valid = false;
+    return PS_OK;
+}
+
+int ps_push_device(ps_ctx* c, const void* bx, const void* by, const void* bz)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if ((r = ensure_real(c, &c->rhs))) return r;
+    const void* b[3] = {bx, by, bz};
+    const size_t bytes = c->n3 * real_bytes(c);
+    for (int d = 0; d < 3; ++d) {
+        if (!b[d]) continue;
+        CK(hipMemcpyAsync(static_cast<char*>(c->rhs) + d * bytes, b[d], bytes, hipMemcpyDeviceToDevice, c->stream));
+        c->rhs_present[d] = true;
+    }
+    CK(hipStreamSynchronize(c->stream));
+    c->solved_after_push = false;
+    c->rhsf_valid = false;
+    return PS_OK;
+}
+
+int ps_solve3d(ps_ctx* c)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->rhs) return fail(c, PS_ESTATE, "[poisson_solver.solve3d]: no rhs field has been pushed");
+    r = (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
+    if (r) return r;
+    CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+int ps_solve2d(ps_ctx* c) { return fail(c, PS_EUNSUPPORTED, "solve2d: the 2D solver is not part of this library"); }
+
+int ps_pull(ps_ctx* c, double* ax, double* ay, double* az, double* bx, double* by, double* bz)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    double* a[3] = {ax, ay, az};
+    double* b[3] = {bx, by, bz};
+    for (int d = 0; d < 3; ++d) {
+        if (!a[d] || !c->lhs_present[d]) continue;
+        r = (c->prec == PS_FP64) ? pull_host<double>(c, a[d], c->lhs, d) : pull_host<float>(c, a[d], c->lhs, d);
+        if (r) return r;
+    }
+    bool want = false;
+    for (int d = 0; d < 3; ++d) want = want || (b[d] && c->rhs_present[d]);
+    if (!want) return PS_OK;
+    const void* src = nullptr;
+    if ((r = rhs_source(c, &src))) return r;
+    for (int d = 0; d < 3; ++d) {
+        if (!b[d] || !c->rhs_present[d]) continue;
+        r = (c->prec == PS_FP64) ? pull_host<double>(c, b[d], src, d) : pull_host<float>(c, b[d], src, d);
+        if (r) return r;
+    }
+    return PS_OK;
+}
+
+int ps_pull_device(ps_ctx* c, void* ax, void* ay, void* az, void* bx, void* by, void* bz)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    const size_t bytes = c->n3 * real_bytes(c);
+    void* a[3] = {ax, ay, az};
+    void* b[3] = {bx, by, bz};
+    for (int d = 0; d < 3; ++d)
+        if (a[d] && c->lhs_present[d])
+            CK(hipMemcpyAsync(a[d], static_cast<const char*>(c->lhs) + d * bytes, bytes, hipMemcpyDeviceToDevice,
+                              c->stream));
+    bool want = false;
+    for (int d = 0; d < 3; ++d) want = want || (b[d] && c->rhs_present[d]);
+    if (want) {
+        const void* src = nullptr;
+        if ((r = rhs_source(c, &src))) return r;
+        for (int d = 0; d < 3; ++d)
+            if (b[d] && c->rhs_present[d])
+                CK(hipMemcpyAsync(b[d], static_cast<const char*>(src) + d * bytes, bytes, hipMemcpyDeviceToDevice,
+                                  c->stream));
+    }
+    CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+int ps_partition_count(const ps_ctx* c) { return (c && c->ready) ? 1 : 0; }
+
+int ps_partition(const ps_ctx* c, int rank, int ncell[3], int icell[3], double dx[3])
+{
+    if (!c || !ncell || !icell || !dx) return PS_EINVAL;
+    if (!c->ready) return PS_ESTATE;
+    if (rank != 0) return PS_EINVAL;
+    for (int d = 0; d < 3; ++d) {
+        ncell[d] = c->ncell[d];
+        icell[d] = 0;
+        dx[d] = c->dx[d];
+    }
+    return PS_OK;
+}
+
+const char* ps_strerror(int s)
+{
+    switch (s) {
+    case PS_OK: return "ok";
+    case PS_EINVAL: return "invalid argument";
+    case PS_EBC: return "unsupported boundary conditions";
+    case PS_ESTATE: return "invalid solver state";
+    case PS_ENOMEM: return "out of device memory";
+    case PS_EHIP: return "HIP runtime error";
+    case PS_EUNSUPPORTED: return "unsupported configuration";
+    default: return "unknown status";
+    }
+}
+
+const char* ps_last_error(const ps_ctx* c) { return c ? c->err.c_str() : "NULL context"; }
+
+int ps_real_bytes(const ps_ctx* c) { return c ? real_bytes(c) : 0; }
+
+void* ps_stream(ps_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int ps_synchronize(ps_ctx* c)
+{
+    if (!c) return PS_EINVAL;
+    if (c->stream) CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+int ps_profile_solve(ps_ctx* c, int iters, double ms[6])
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (iters < 1 || !ms) return fail(c, PS_EINVAL, "profile: iters must be >= 1");
+    if (!c->ev_made) {
+        for (hipEvent_t& e : c->ev) CK(hipEventCreate(&e));
+        c->ev_made = true;
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int it = 0; it < iters; ++it) {
+        r = (c->prec == PS_FP64) ? solve_impl<double>(c, c->ev) : solve_impl<float>(c, c->ev);
+        if (r) return r;
+        CK(hipEventSynchronize(c->ev[5]));
+        for (int k = 0; k < 5; ++k) {
+            float t = 0.f;
+            CK(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
+            acc[k] += t;
+        }
+        float tt = 0.f;
+        CK(hipEventElapsedTime(&tt, c->ev[0], c->ev[5]));
+        acc[5] += tt;
+    }
+    for (int k = 0; k < 6; ++k) ms[k] = acc[k] / iters;
+    return PS_OK;
+}
+
+int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int* nstage, int radix[14])
+{
+    Plan1D p;
+    std::string why;
+    if (!make_plan(L, max_E_for(precision_bytes, ncomp), &p, &why)) return PS_EUNSUPPORTED;
+    if (E) *E = p.E;
+    if (tp) *tp = p.TP;
+    if (nstage) *nstage = p.nstage;
+    if (radix)
+        for (int i = 0; i < kMaxStage; ++i) radix[i] = p.radix[i];
+    return PS_OK;
+}
+
+int ps_eval_special(const double* x, double* y, long n, int which)
+{
+    if (!x || !y || n < 0 || (which != 0 && which != 1)) return PS_EINVAL;
+    if (n == 0) return PS_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PS_EHIP;
+    std::string err;
+    return eval_special_device(x, y, n, which, &err);
+}
+
+}  // extern "C"

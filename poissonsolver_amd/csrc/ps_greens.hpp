@@ -1,0 +1,25 @@
+// Green's kernel construction (FP64 on the device).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/ps_capi.h"
+
+namespace psamd {
+
+constexpr int kGreensEBC = PS_EBC;
+constexpr int kGreensENOMEM = PS_ENOMEM;
+constexpr int kGreensEHIP = PS_EHIP;
+constexpr int kGreensEUNSUPPORTED = PS_EUNSUPPORTED;
+
+// Ghat * zeta / (Mx My Mz) on the octant (Mz/2+1, My/2+1, Mx/2+1), x fastest,
+// written to `out` (device memory of that many T).  Returns a PS_* status.
+template <typename T>
+int build_greens(const int ncell[3], const int bc[3], const double dx[3], int reg, T* out, hipStream_t s,
+                 std::string* err);
+
+// Device Si (which = 0) or Ji0 (which = 1) at n host arguments.
+int eval_special_device(const double* x, double* y, long n, int which, std::string* err);
+
+}  // namespace psamd

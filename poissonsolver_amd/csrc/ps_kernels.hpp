@@ -1,0 +1,68 @@
+// Pass kernels of the 3D solve (declarations shared by host and device code).
+//
+// Data layout in HBM (x fastest, as the reference's ijk = (k ny + j) nx + i):
+//   real fields        [c][nz][ny][nx]           T
+//   A1 (after x pass)  [c][nz][ny][Hp]           complex T, kx in [0, H)
+//   A2 (after y pass)  [c][nz][My][Hp]           complex T
+//   Gk octant          [Mz/2+1][My/2+1][H]       real T  = Ghat * zeta / (Mx My Mz)
+// The z pass reads nz planes of A2, transforms a zero-padded Mz pencil in
+// registers, applies the k-space operator and writes nz planes back, so the
+// z-doubled spectrum never touches HBM.
+#pragma once
+#include "ps_fft.hpp"
+
+namespace psamd {
+
+enum PencilOp : int {
+    OP_NONE = 0,      // copy (pure transform pass)
+    OP_PLAIN = 1,     // l_c = Gk r_c
+    OP_GRAD = 2,      // l_d = i k_d Gk r_d           (component-wise)
+    OP_DIV = 3,       // l_0 = Gk i k . r
+    OP_CURL = 4,      // l = Gk i k x r
+    OP_RHSF = 5,      // r_c * zeta / N               (filtered rhs, no reprojection)
+    OP_RHSF_PROJ = 6, // r_d*zeta/N - Gk (kh_d kh.r + kn_d kn.r)  (filtered + reprojected)
+};
+
+enum LoadMode : int { LOAD_COMPLEX = 0, LOAD_REAL_EVEN = 1 };
+enum StoreMode : int { STORE_COMPLEX = 0, STORE_REAL = 1 };
+
+// Strided pencil pass: pencils along `pos`, tiles of B columns.
+// element (c, slab, pos, col) at c*comp + slab*slab + pos*pos + col*col
+template <typename T> struct PencilArgs {
+    FftPlan<T> plan;
+    const void* in;
+    void* out;
+    long in_comp, in_slab, in_pos, in_col;
+    long out_comp, out_slab, out_pos, out_col;
+    int n_in;       // input positions present (the rest are zero padding)
+    int n_out;      // output positions stored
+    int ncol, B, ncoltile, nslab, ncomp;
+    int load_mode, store_mode;
+    int do_fwd, do_inv;
+    int op;
+    // k-space operator (z pass): slab = ky, col = kx, pos = kz
+    const T* G;
+    long g_ky, g_kz;
+    int Mx, My, Mz;
+    double dk[3];    // 1/(M_d dx_d)
+    double zarg;     // 0.5 (2h)^2
+    int reg;         // regularisation order (even), 0 = none
+    double invN;     // 1/(Mx My Mz)
+};
+
+// x pass: two real rows (a, b) per complex pencil.
+template <typename T> struct XArgs {
+    FftPlan<T> plan;
+    const void* in;
+    void* out;
+    long r_comp, r_row;   // real rows
+    long c_comp, c_row;   // complex rows
+    int nrows;            // rows per component (ny * nz)
+    int nvalid;           // nx
+    int H;                // L/2 + 1
+    int P;                // pairs per workgroup
+    int npairs;
+    int ncomp;
+};
+
+}  // namespace psamd

@@ -1,0 +1,177 @@
+"""Python mirror of the reference's `poisson_solver` class (poisson_solver.hpp:39-124).
+
+Same method names and argument meaning: ``setup3d(ncell, bc, dx, reg_order=0)``,
+``set_return_grad/div/curl(bool)``, ``set_reprojection(bool)``, ``push(bx, by,
+bz)`` with None for an absent component, ``solve3d()``, ``pull(ax, ay, az, bx,
+by, bz)`` filling caller-owned float64 arrays, and the public ``partition``
+list.  Arrays hold nx*ny*nz doubles with x fastest (numpy shape (nz, ny, nx)).
+Like the reference, a failing call prints a message to stderr and returns;
+``last_status`` holds the status code.  The solve runs on the GPU through the
+C ABI; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import sys
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _capi
+
+
+@dataclass
+class class_partition:
+    """class_partition.hpp:29-43."""
+    ncell: list
+    icell: list
+    dx: list
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class poisson_solver:
+    def __init__(self, precision: int = _capi.PS_FP64):
+        self._lib = _capi.lib()
+        self._ctx = ctypes.c_void_p()
+        self.last_status = self._lib.ps_create(ctypes.byref(self._ctx))
+        self.partition = []
+        self._n = None
+        if self.last_status == _capi.PS_OK and precision != _capi.PS_FP64:
+            self.set_precision(precision)
+
+    # -- lifecycle ---------------------------------------------------------
+    def close(self):
+        if getattr(self, "_ctx", None) and self._ctx.value:
+            self._lib.ps_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _report(self, status, where):
+        self.last_status = status
+        if status != _capi.PS_OK:
+            msg = self._lib.ps_last_error(self._ctx).decode() if self._ctx.value else ""
+            print(" [poisson_solver.%s]: %s: %s" % (where, _capi.strerror(status), msg), file=sys.stderr)
+            return False
+        return True
+
+    # -- setup (setup3d.cpp:8-71) ------------------------------------------
+    def set_precision(self, precision: int):
+        return self._report(self._lib.ps_set_precision(self._ctx, int(precision)), "set_precision")
+
+    def setup3d(self, ncell, bound_cond, dx, reg_order: int = 0):
+        n = (ctypes.c_int * 3)(*[int(v) for v in ncell])
+        b = (ctypes.c_int * 3)(*[int(v) for v in bound_cond])
+        d = (ctypes.c_double * 3)(*[float(v) for v in dx])
+        if not self._report(self._lib.ps_setup3d(self._ctx, n, b, d, int(reg_order)), "setup3d"):
+            return
+        self._n = tuple(int(v) for v in ncell)
+        self.partition = []
+        for r in range(self._lib.ps_partition_count(self._ctx)):
+            nc = (ctypes.c_int * 3)()
+            ic = (ctypes.c_int * 3)()
+            dd = (ctypes.c_double * 3)()
+            self._lib.ps_partition(self._ctx, r, nc, ic, dd)
+            self.partition.append(class_partition(list(nc), list(ic), list(dd)))
+
+    def setup2d(self, ncell, bound_cond, dx, reg_order: int = 0):
+        n = (ctypes.c_int * 2)(*[int(v) for v in ncell])
+        b = (ctypes.c_int * 2)(*[int(v) for v in bound_cond])
+        d = (ctypes.c_double * 2)(*[float(v) for v in dx])
+        self._report(self._lib.ps_setup2d(self._ctx, n, b, d, int(reg_order)), "setup2d")
+
+    # -- options (settings.cpp:9-30) ---------------------------------------
+    def set_return_grad(self, toggle: bool):
+        self._report(self._lib.ps_set_return_grad(self._ctx, int(bool(toggle))), "set_return_grad")
+
+    def set_return_div(self, toggle: bool):
+        self._report(self._lib.ps_set_return_div(self._ctx, int(bool(toggle))), "set_return_div")
+
+    def set_return_curl(self, toggle: bool):
+        self._report(self._lib.ps_set_return_curl(self._ctx, int(bool(toggle))), "set_return_curl")
+
+    def set_reprojection(self, toggle: bool):
+        self._report(self._lib.ps_set_reprojection(self._ctx, int(bool(toggle))), "set_reprojection")
+
+    # -- data --------------------------------------------------------------
+    def _host(self, a, writable):
+        if a is None:
+            return None
+        if not isinstance(a, np.ndarray) or a.dtype != np.float64 or not a.flags.c_contiguous:
+            raise TypeError("arrays must be C-contiguous float64 numpy arrays")
+        if writable and not a.flags.writeable:
+            raise TypeError("output arrays must be writable")
+        if self._n is not None and a.size != self._n[0] * self._n[1] * self._n[2]:
+            raise ValueError("array size %d does not match the grid %s" % (a.size, self._n))
+        return a
+
+    def push(self, bx=None, by=None, bz=None):
+        """push.cpp:9-397."""
+        arrs = [self._host(a, False) for a in (bx, by, bz)]
+        self._report(self._lib.ps_push(self._ctx, *[_ptr(a) for a in arrs]), "push")
+
+    def solve3d(self):
+        """solve3d.cpp:8-635."""
+        self._report(self._lib.ps_solve3d(self._ctx), "solve3d")
+
+    def solve2d(self):
+        self._report(self._lib.ps_solve2d(self._ctx), "solve2d")
+
+    def pull(self, ax=None, ay=None, az=None, bx=None, by=None, bz=None):
+        """pull.cpp:9-469."""
+        arrs = [self._host(a, True) for a in (ax, ay, az, bx, by, bz)]
+        self._report(self._lib.ps_pull(self._ctx, *[_ptr(a) for a in arrs]), "pull")
+
+    # -- extensions --------------------------------------------------------
+    def push_device(self, bx=None, by=None, bz=None):
+        """Device pointers (int addresses, e.g. torch tensor.data_ptr()) of the
+        solve precision's real type."""
+        p = [None if v is None else ctypes.c_void_p(int(v)) for v in (bx, by, bz)]
+        self._report(self._lib.ps_push_device(self._ctx, *p), "push_device")
+
+    def pull_device(self, ax=None, ay=None, az=None, bx=None, by=None, bz=None):
+        p = [None if v is None else ctypes.c_void_p(int(v)) for v in (ax, ay, az, bx, by, bz)]
+        self._report(self._lib.ps_pull_device(self._ctx, *p), "pull_device")
+
+    def stream(self) -> int:
+        return self._lib.ps_stream(self._ctx) or 0
+
+    def synchronize(self):
+        self._report(self._lib.ps_synchronize(self._ctx), "synchronize")
+
+    def profile_solve(self, iters: int = 1):
+        """Per-pass average times (ms) measured with HIP events on the
+        context's stream: dict x, y, z, y_inv, x_inv, solve."""
+        ms = (ctypes.c_double * 6)()
+        if not self._report(self._lib.ps_profile_solve(self._ctx, int(iters), ms), "profile_solve"):
+            return None
+        return dict(zip(("x", "y", "z", "y_inv", "x_inv", "solve"), list(ms)))
+
+
+def plan_describe(L: int, ncomp: int = 1, precision_bytes: int = 8):
+    """(E, threads per pencil, radices) the library picks for length L, or None."""
+    E, tp, ns = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    rad = (ctypes.c_int * 14)()
+    st = _capi.lib().ps_plan_describe(int(L), int(ncomp), int(precision_bytes), ctypes.byref(E),
+                                      ctypes.byref(tp), ctypes.byref(ns), rad)
+    if st != _capi.PS_OK:
+        return None
+    return E.value, tp.value, [rad[i] for i in range(ns.value)]
+
+
+def eval_special(x, which: str):
+    """Device Si ('si') or Ji0 ('ji0') at float64 arguments x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    st = _capi.lib().ps_eval_special(x.ctypes.data_as(_capi._c_dbl_p), y.ctypes.data_as(_capi._c_dbl_p),
+                                     x.size, 0 if which == "si" else 1)
+    if st != _capi.PS_OK:
+        raise RuntimeError("ps_eval_special: " + _capi.strerror(st))
+    return y

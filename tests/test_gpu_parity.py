@@ -1,0 +1,84 @@
+"""HIP path vs the oracle (FP64 restatement of the reference) on the same
+seeded inputs.  FP64: rel-L2 <= 1e-12 (the north-star bar for the FP64 GPU
+path).  FP32: rel-L2 <= 2e-6 against the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import announce, gpu_solve, random_rhs
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 2e-6
+
+CASES = [
+    # (ncell, bc, dx)
+    ((16, 16, 16), (0, 0, 0), (0.125, 0.125, 0.125)),
+    ((12, 10, 9), (0, 0, 0), (0.1, 0.12, 0.09)),
+    ((16, 12, 20), (1, 1, 0), (1 / 16, 1 / 12, 0.1)),
+    ((15, 9, 8), (1, 1, 0), (1 / 15, 1 / 9, 0.125)),
+    ((16, 20, 12), (0, 0, 1), (0.1, 0.1, 1 / 12)),
+    ((14, 10, 7), (0, 0, 1), (0.11, 0.09, 1 / 7)),
+    ((16, 12, 10), (1, 1, 1), (1 / 16, 1 / 12, 0.1)),
+    ((9, 7, 5), (1, 1, 1), (1 / 9, 1 / 7, 0.2)),
+]
+
+
+def _compare(n, bc, dx, reg, op, reproject, prec, tol, seed):
+    announce(n, bc, reg, op, reproject, prec)
+    b = random_rhs(n, seed)
+    if op == "plain":
+        b = [b[0], None, b[2]]
+    lo, ro = O.solve(n, bc, dx, reg, b, op=op, reproject=reproject)
+    lg, rg = gpu_solve(n, bc, dx, reg, b, op=op, reproject=reproject, prec=prec, want_rhs=True)
+    got = [lg[d] for d in range(3) if lo[d] is not None]
+    ref = [lo[d] for d in range(3) if lo[d] is not None]
+    err = O.rel_l2(got, ref)
+    assert err <= tol, ("lhs", err)
+    for d in range(3):
+        if lo[d] is None:
+            assert np.isnan(lg[d]).all()   # not written
+    gr = [rg[d] for d in range(3) if ro[d] is not None]
+    rr = [ro[d] for d in range(3) if ro[d] is not None]
+    err_r = O.rel_l2(gr, rr)
+    assert err_r <= tol, ("rhs", err_r)
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("op", ["plain", "grad", "div", "curl"])
+def test_fp64_parity_reg0(case, op):
+    n, bc, dx = case
+    _compare(n, bc, dx, 0, op, False, "fp64", FP64_TOL, 7)
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("reg,reproject", [(2, False), (5, True), (6, True), (10, False)])
+def test_fp64_parity_regularised(case, reg, reproject):
+    n, bc, dx = case
+    _compare(n, bc, dx, reg, "curl", reproject, "fp64", FP64_TOL, 11)
+
+
+@pytest.mark.parametrize("case", CASES[:4], ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("op", ["plain", "curl"])
+def test_fp32_parity(case, op):
+    n, bc, dx = case
+    _compare(n, bc, dx, 2, op, False, "fp32", FP32_TOL, 13)
+
+
+def test_fp64_parity_larger_uuu():
+    n = (64, 48, 40)
+    dx = (2 / 64, 1.5 / 48, 1.25 / 40)
+    _compare(n, (0, 0, 0), dx, 2, "plain", False, "fp64", FP64_TOL, 17)
+
+
+def test_device_special_functions_match_reference_golden():
+    import os
+    import poissonsolver_amd as P
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "special_ref.npz"))
+    announce("special functions")
+    si = P.eval_special(g["si_x"], "si")
+    ji = P.eval_special(g["ji0_x"], "ji0")
+    # same published algorithms; device libm may differ from glibc by an ulp
+    assert np.max(np.abs(si - g["si_y"]) / np.maximum(np.abs(g["si_y"]), 1e-300)) < 1e-13
+    assert np.max(np.abs(ji - g["ji0_y"]) / np.maximum(np.abs(g["ji0_y"]), 1e-300)) < 1e-13
