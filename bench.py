@@ -1,0 +1,184 @@
+"""Benchmark of the 3D solve path on one MI355X.
+
+Default workload: 512^3 fully unbounded scalar Poisson solve (bc 0,0,0; domain
+doubled to 1024^3 through the R2C x pass and the pruned z pass), regularisation
+order 2, Gaussian source (s = 0.1 on [-1,1]^3), FP32 storage.  One step = one
+`solve3d` with the source already resident in HBM (pushed before the timed
+region).  Prints one JSON line.
+
+  python bench.py [--steps K] [--warmup W] [--prec fp32|mixed|fp64]
+                  [--n 512] [--op plain|curl] [--no-cpu] [--no-check]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "solves/s + HBM GB/s, 512^3 unbounded: MFMA mixed-prec vs FP32 FFT @relL2≤1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def gaussian_source(n, s=0.1):
+    h = 2.0 / n
+    x = -1.0 + (np.arange(n) + 0.5) * h
+    r2 = (x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2)
+    return np.exp(-r2 / (2 * s * s)) / ((2 * np.pi) ** 1.5 * s ** 3), h
+
+
+def alg_bytes(n, cin, cout, eb):
+    """Algorithmic HBM bytes of the pruned 5-pass pipeline (R2C on x, z never
+    stored doubled, real octant of the kernel): per pass P1..P5."""
+    nx = ny = nz = n
+    Mx, My, Mz = 2 * n, 2 * n, 2 * n
+    H = Mx // 2 + 1
+    er, ec, eg = eb, 2 * eb, eb
+    P1 = cin * (nx * ny * nz * er + H * ny * nz * ec)
+    P2 = cin * (H * ny * nz + H * My * nz) * ec
+    P3 = cin * H * My * nz * ec + (Mx // 2 + 1) * (My // 2 + 1) * (Mz // 2 + 1) * eg + cout * H * My * nz * ec
+    P4 = cout * (H * My * nz + H * ny * nz) * ec
+    P5 = cout * (H * ny * nz * ec + nx * ny * nz * er)
+    return dict(x=P1, y=P2, z=P3, y_inv=P4, x_inv=P5, solve=P1 + P2 + P3 + P4 + P5)
+
+
+def cpu_baseline(n_target, cin, op):
+    """The oracle (numpy FP64 restatement of the reference, one core) timed on
+    a bounded sample: push+solve+pull at 128^3, scaled to n_target by
+    n^3 log n.  The reference itself cannot be built here (it needs MPI)."""
+    from oracle import oracle as O
+    ns = 128
+    b, h = gaussian_source(ns)
+    s = O.OracleSolver()
+    s.setup3d((ns,) * 3, (0, 0, 0), (h,) * 3, 2)
+    s.set_return_curl(op == "curl")
+    reps, t = 0, 0.0
+    while reps < 3 and t < 20.0:
+        t0 = time.perf_counter()
+        s.push(*([b] * cin))
+        s.solve3d()
+        out = [np.empty_like(b) for _ in range(3)]
+        s.pull(*out)
+        t += time.perf_counter() - t0
+        reps += 1
+    per = t / reps
+    scale = (n_target / ns) ** 3 * math.log2(2 * n_target) / math.log2(2 * ns)
+    return {"value": 1.0 / (per * scale), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": "oracle (numpy FP64 C2C restatement) push+solve3d+pull at %d^3 UUU reg2, "
+                      "%d reps, %.2f s/solve, scaled to %d^3 by n^3 log n (x%.0f)" % (ns, reps, per, n_target, scale)}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--prec", default="fp32", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--op", default="plain", choices=["plain", "curl"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
+        return 2
+
+    import poissonsolver_amd as P
+
+    n = a.n
+    b, h = gaussian_source(n)
+    cin = 3 if a.op == "curl" else 1
+    srcs = [b, 0.5 * b, -0.25 * b][:cin] if a.op == "curl" else [b]
+    prec = {"fp32": P.PS_FP32, "mixed": P.PS_MIXED, "fp64": P.PS_FP64}[a.prec]
+    s = P.poisson_solver(prec)
+    s.setup3d((n,) * 3, (0, 0, 0), (h,) * 3, 2)
+    s.set_return_curl(a.op == "curl")
+    s.push(*srcs)
+    if s.last_status:
+        return 1
+    for _ in range(a.warmup):
+        s.solve3d()
+    s.synchronize()
+    t0 = time.perf_counter()
+    prof = s.profile_solve(a.steps)          # K solves, HIP events on the solver stream
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    if prof is None:
+        return 1
+    ms = prof["solve"]
+    eb = 8 if a.prec == "fp64" else 4
+    cout = 3 if a.op == "curl" else 1
+    B = alg_bytes(n, cin, cout, eb)
+    dom = "z"
+    achieved = B[dom] / (prof[dom] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": 1000.0 / ms,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.prec == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "%d^3 fully unbounded %s Poisson solve, FFT domain doubled to %d^3, reg order 2, "
+                        "Gaussian source, %s" % (n, "vector curl" if a.op == "curl" else "scalar", 2 * n, a.prec),
+            "ncell": [n, n, n], "bc": [0, 0, 0], "reg_order": 2, "op": a.op, "precision": a.prec,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "z pass (k_pencil NC=%d: pad, FFT, Ghat*zeta*op, IFFT, truncate)" % (3 if a.op == "curl" else 1),
+            "alg_bytes_per_launch": B[dom], "avg_ms": prof[dom],
+        },
+        "solve_hbm_gbs": B["solve"] / (ms * 1e-3) / 1e9,
+        "solve_roofline_frac": B["solve"] / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "alg_bytes_per_solve": B["solve"],
+        "pass_ms": {k: prof[k] for k in ("x", "y", "z", "y_inv", "x_inv")},
+        "pass_gbs": {k: B[k] / (prof[k] * 1e-3) / 1e9 for k in ("x", "y", "z", "y_inv", "x_inv")},
+        "wall_s_timed": wall,
+    }
+    if not a.no_check:
+        from scipy.special import erf
+        lhs = [np.empty_like(b) for _ in range(cout)]
+        s.pull(*lhs)
+        if a.prec != "fp64":
+            r = P.poisson_solver(P.PS_FP64)
+            r.setup3d((n,) * 3, (0, 0, 0), (h,) * 3, 2)
+            r.set_return_curl(a.op == "curl")
+            r.push(*srcs)
+            r.solve3d()
+            ref = [np.empty_like(b) for _ in range(cout)]
+            r.pull(*ref)
+            r.close()
+            num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(lhs, ref))
+            den = sum(float(np.sum(y ** 2)) for y in ref)
+            out["rel_l2_vs_fp64_gpu"] = math.sqrt(num / den)
+        if a.op == "plain":
+            x = -1.0 + (np.arange(n) + 0.5) * h
+            r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+            rr = np.sqrt(r2)
+            se = math.sqrt(0.1 ** 2 + (2 * h) ** 2)
+            A = erf(rr / (math.sqrt(2) * se)) / (4 * np.pi * rr)
+            out["rel_l2_vs_analytic"] = float(np.sqrt(np.sum((lhs[0] - A) ** 2) / np.sum(A ** 2)))
+    s.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n, cin, a.op)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -79,6 +79,8 @@ def test_device_special_functions_match_reference_golden():
     announce("special functions")
     si = P.eval_special(g["si_x"], "si")
     ji = P.eval_special(g["ji0_x"], "ji0")
-    # same published algorithms; device libm may differ from glibc by an ulp
-    assert np.max(np.abs(si - g["si_y"]) / np.maximum(np.abs(g["si_y"]), 1e-300)) < 1e-13
-    assert np.max(np.abs(ji - g["ji0_y"]) / np.maximum(np.abs(g["ji0_y"]), 1e-300)) < 1e-13
+    # same published algorithms; device libm may differ from glibc by an ulp.
+    # Ji0(x) ~ x^2/8 near 0, where the series' absolute round-off (~1e-16)
+    # dominates, so the bound is absolute + relative.
+    assert np.all(np.abs(si - g["si_y"]) <= 1e-13 * np.abs(g["si_y"]))
+    assert np.all(np.abs(ji - g["ji0_y"]) <= 1e-15 + 1e-13 * np.abs(g["ji0_y"]))
