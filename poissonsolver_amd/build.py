@@ -24,6 +24,8 @@ COMMON = [
     "-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH,
     # no packed FP32 instructions: hipcc -O3 would otherwise form v_pk_* ops
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # lets x + 0.0 fold away where zero padding feeds the first butterflies
+    "-fno-signed-zeros",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
 
@@ -41,10 +43,18 @@ def e_values():
     return [int(v) for v in re.findall(r"X\((\d+)\)", m.group(1))]
 
 
+def sl_values():
+    txt = open(os.path.join(CSRC, "ps_static.hpp")).read()
+    m = re.search(r"#define PS_FOR_EACH_SL\(X\)(.*)", txt)
+    return [int(v) for v in re.findall(r"X\((\d+)\)", m.group(1))]
+
+
 def sources():
     units = []
     for e in e_values():
         units.append(("kern_e.hip", ["-DPS_E=%d" % e], "kern_e%d.o" % e))
+    for L in sl_values():
+        units.append(("skern.hip", ["-DPS_SL=%d" % L], "skern_%d.o" % L))
     for f in ("ps_greens.hip", "ps_dispatch.cpp", "ps_plan.cpp", "ps_capi.cpp", "poisson_solver.cpp"):
         units.append((f, [], os.path.splitext(f)[0] + ".o"))
     return units

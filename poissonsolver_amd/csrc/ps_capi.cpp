@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -22,6 +23,7 @@
 #include "ps_greens.hpp"
 #include "ps_launch.hpp"
 #include "ps_plan.hpp"
+#include "ps_static.hpp"
 
 using namespace psamd;
 
@@ -31,7 +33,30 @@ struct DevPlan {
     Plan1D p;
     void* tw = nullptr;      // complex T table of length L
     int tile = 1;            // B (strided) or P (x pass)
+    bool stat = false;       // compile-time specialised kernels for this length
 };
+
+// Tile shapes and the z blocking of A2.  Defaults come from the planner;
+// PS_TUNE_* environment variables override them (read at setup3d).
+struct Tuning {
+    int By = 0, Bz = 0, Px = 0;   // 0: planner's choice
+    int zblk = -2;                // log2 z block of A2; -1: [ky][z][kx]; 0: [z][ky][kx]; -2: planner
+    int lds_kb = 80;              // LDS budget per workgroup
+    int xcd = 1;                  // XCD-contiguous work ids in strided passes
+    int stat = 1;                 // use the specialised kernels where they exist
+    int stat3 = 0;                // ... also for the 3-component z pass
+    int twreg_z = 1;              // z pass: stage twiddles in registers (1) or loaded per stage (0)
+    int twreg_y = 0;              // y passes: same choice
+    int occ4_z = 0;               // z pass: 4 waves/SIMD target (1) or 2 (0)
+    int halfb_z = 1, halfb_y = 0; // specialised strided passes: half-width tiles (measured: z faster, y slower)
+};
+
+int env_int(const char* name, int dflt)
+{
+    const char* v = getenv(name);
+    if (!v || !*v) return dflt;
+    return atoi(v);
+}
 
 }  // namespace
 
@@ -58,6 +83,10 @@ struct ps_ctx {
     void* G = nullptr;      // octant of Ghat*zeta/N, real
 
     DevPlan px, py, pz1, pz3;
+    Tuning tune;
+    int zb = 0;             // A2 layout: log2 z block (-1 = [ky][z][kx])
+    Addr a2_ypass;          // A2 addressing seen by the y passes (slab = z, pos = ky)
+    Addr a2_zpass;          // A2 addressing seen by the z pass (slab = ky, pos = z)
 
     bool rhs_present[3] = {false, false, false};
     bool lhs_present[3] = {false, false, false};
@@ -212,10 +241,11 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
 {
     a.plan = device_plan<T>(d.p, reinterpret_cast<const cplx<T>*>(d.tw));
     a.in = a.out = c->A2;
-    a.in_comp = a.out_comp = c->A2c;
-    a.in_slab = a.out_slab = c->Hp;                  // slab = ky
-    a.in_pos = a.out_pos = (long)c->M[1] * c->Hp;    // pos = kz
-    a.in_col = a.out_col = 1;                         // col = kx
+    a.ia = a.oa = c->a2_zpass;                        // slab = ky, pos = kz, col = kx
+    a.xcd_remap = c->tune.xcd;
+    a.slab_pair = 1;
+    a.twreg = c->tune.twreg_z;
+    a.occ4 = c->tune.occ4_z;
     a.n_in = a.n_out = c->ncell[2];
     a.ncol = c->H;
     a.B = d.tile;
@@ -225,8 +255,8 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
     a.store_mode = STORE_COMPLEX;
     a.do_fwd = a.do_inv = 1;
     a.G = reinterpret_cast<const T*>(c->G);
-    a.g_ky = c->O[0];
-    a.g_kz = (long)c->O[0] * c->O[1];
+    a.g_ky = (long)c->O[0] * c->O[2];                 // octant stored [ky][kz][kx]
+    a.g_kz = c->O[0];
     a.Mx = c->M[0];
     a.My = c->M[1];
     a.Mz = c->M[2];
@@ -265,21 +295,18 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     xa.P = c->px.tile;
     xa.npairs = (ny * nz + 1) / 2;
     xa.ncomp = nin;
-    CK(launch_x_r2c<T>(c->px.p.E, xa, s));
+    if (c->px.stat) CK(static_launch_x<T>(c->px.p.L, xa, 0, c->bc[0] == 0, s));
+    else CK(launch_x_r2c<T>(c->px.p.E, xa, s));
     if (ev) CK(hipEventRecord(ev[1], s));
 
     PencilArgs<T> ya{};
     ya.plan = device_plan<T>(c->py.p, reinterpret_cast<const cplx<T>*>(c->py.tw));
     ya.in = c->A1;
     ya.out = c->A2;
-    ya.in_comp = c->A1c;
-    ya.in_slab = (long)ny * c->Hp;
-    ya.in_pos = c->Hp;
-    ya.in_col = 1;
-    ya.out_comp = c->A2c;
-    ya.out_slab = (long)c->M[1] * c->Hp;
-    ya.out_pos = c->Hp;
-    ya.out_col = 1;
+    ya.ia = addr_plain(c->A1c, (long)ny * c->Hp, c->Hp, 1);
+    ya.oa = c->a2_ypass;
+    ya.xcd_remap = c->tune.xcd;
+    ya.twreg = c->tune.twreg_y;
     ya.n_in = ny;
     ya.n_out = c->M[1];
     ya.ncol = c->H;
@@ -292,7 +319,8 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     ya.do_fwd = 1;
     ya.do_inv = 0;
     ya.op = OP_NONE;
-    CK(launch_pencil<T>(c->py.p.E, 1, ya, s));
+    if (c->py.stat) CK(static_launch_pencil<T>(c->py.p.L, ya, 1, SM_FWD, c->bc[1] == 0, s));
+    else CK(launch_pencil<T>(c->py.p.E, 1, ya, s));
     if (ev) CK(hipEventRecord(ev[2], s));
 
     const DevPlan& zp = (nc == 3) ? c->pz3 : c->pz1;
@@ -300,22 +328,22 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     fill_zargs<T>(c, za, zp);
     za.op = op;
     za.ncomp = (nc == 3) ? 1 : nin;
-    CK(launch_pencil<T>(zp.p.E, nc, za, s));
+    if (zp.stat) CK(static_launch_pencil<T>(zp.p.L, za, nc, SM_FUSED, c->bc[2] == 0, s));
+    else CK(launch_pencil<T>(zp.p.E, nc, za, s));
     if (ev) CK(hipEventRecord(ev[3], s));
 
     PencilArgs<T> yi = ya;
     yi.in = c->A2;
     yi.out = c->A1;
-    yi.in_comp = c->A2c;
-    yi.in_slab = (long)c->M[1] * c->Hp;
-    yi.out_comp = c->A1c;
-    yi.out_slab = (long)ny * c->Hp;
+    yi.ia = c->a2_ypass;
+    yi.oa = addr_plain(c->A1c, (long)ny * c->Hp, c->Hp, 1);
     yi.n_in = c->M[1];
     yi.n_out = ny;
     yi.ncomp = nout;
     yi.do_fwd = 0;
     yi.do_inv = 1;
-    CK(launch_pencil<T>(c->py.p.E, 1, yi, s));
+    if (c->py.stat) CK(static_launch_pencil<T>(c->py.p.L, yi, 1, SM_INV, c->bc[1] == 0, s));
+    else CK(launch_pencil<T>(c->py.p.E, 1, yi, s));
     if (ev) CK(hipEventRecord(ev[4], s));
 
     XArgs<T> xi = xa;
@@ -323,7 +351,8 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     xi.out = reinterpret_cast<T*>(dst) + (long)out_first * c->n3;
     xi.r_comp = (long)out_step * c->n3;
     xi.ncomp = nout;
-    CK(launch_x_c2r<T>(c->px.p.E, xi, s));
+    if (c->px.stat) CK(static_launch_x<T>(c->px.p.L, xi, 1, c->bc[0] == 0, s));
+    else CK(launch_x_c2r<T>(c->px.p.E, xi, s));
     if (ev) CK(hipEventRecord(ev[5], s));
     return PS_OK;
 }
@@ -340,7 +369,48 @@ template <typename T> int setup_impl(ps_ctx* c)
     c->Hp = (c->H + align - 1) / align * align;
     c->n3 = (long)nx * ny * nz;
     c->A1c = (long)nz * ny * c->Hp;
-    c->A2c = (long)nz * c->M[1] * c->Hp;
+
+    Tuning& tn = c->tune;
+    tn.By = env_int("PS_TUNE_BY", 0);
+    tn.Bz = env_int("PS_TUNE_BZ", 0);
+    tn.Px = env_int("PS_TUNE_PX", 0);
+    tn.zblk = env_int("PS_TUNE_ZBLK", -2);
+    tn.lds_kb = env_int("PS_TUNE_LDS_KB", 80);
+    tn.xcd = env_int("PS_TUNE_XCD", 1);
+    tn.stat = env_int("PS_TUNE_STATIC", 1);
+    tn.stat3 = env_int("PS_TUNE_STATIC3", 0);
+    tn.twreg_z = env_int("PS_TUNE_TWREG_Z", 1);
+    tn.twreg_y = env_int("PS_TUNE_TWREG_Y", 0);
+    tn.occ4_z = env_int("PS_TUNE_OCC4_Z", 0);
+    tn.halfb_z = env_int("PS_TUNE_HALFB_Z", 1);
+    tn.halfb_y = env_int("PS_TUNE_HALFB_Y", 0);
+    set_lds_budget(tn.lds_kb * 1024);
+    set_preferred_E(env_int("PS_TUNE_E", 16));
+
+    // A2 layout: z blocked by Zb = 2^zb planes, [z/Zb][ky][z%Zb][kx]; zb = 0 is
+    // [z][ky][kx] (y pass writes contiguous planes, z pencils strided by a
+    // plane), zb = -1 is [ky][z][kx] (z pencils contiguous, y pass strided).
+    int zb = tn.zblk;
+    if (zb == -2) zb = (nz >= 32) ? 4 : 0;    // 16-plane blocks: measured best at 512^3
+    const long My = c->M[1];
+    if (zb > 0 && (1 << zb) >= nz) zb = -1;
+    c->zb = zb;
+    if (zb == 0) {
+        c->A2c = (long)nz * My * c->Hp;
+        c->a2_ypass = addr_plain(c->A2c, My * c->Hp, c->Hp, 1);
+        c->a2_zpass = addr_plain(c->A2c, c->Hp, My * c->Hp, 1);
+    } else if (zb < 0) {
+        c->A2c = (long)nz * My * c->Hp;
+        c->a2_ypass = addr_plain(c->A2c, c->Hp, (long)nz * c->Hp, 1);
+        c->a2_zpass = addr_plain(c->A2c, (long)nz * c->Hp, c->Hp, 1);
+    } else {
+        const long Zb = 1L << zb, nzb = (nz + Zb - 1) / Zb;
+        c->A2c = nzb * My * Zb * c->Hp;
+        // y pass: slab = z (blocked), pos = ky
+        c->a2_ypass = Addr{c->A2c, 1, c->Hp, My * Zb * c->Hp, Zb * c->Hp, 0, zb, 30};
+        // z pass: slab = ky, pos = z (blocked)
+        c->a2_zpass = Addr{c->A2c, 1, Zb * c->Hp, 0, c->Hp, My * Zb * c->Hp, 30, zb};
+    }
 
     int r;
     if ((r = plan_dim(c, &c->px, c->M[0], 1, "x transform"))) return r;
@@ -348,10 +418,30 @@ template <typename T> int setup_impl(ps_ctx* c)
     if ((r = plan_dim(c, &c->pz1, c->M[2], 1, "z transform"))) return r;
     if ((r = plan_dim(c, &c->pz3, c->M[2], 3, "z transform (3 components)"))) return r;
     const int eb = real_bytes(c);
-    c->px.tile = choose_P(c->px.p, eb, (ny * nz + 1) / 2);
-    c->py.tile = choose_B(c->py.p, eb, c->H);
-    c->pz1.tile = choose_B(c->pz1.p, eb, c->H);
+    c->px.tile = tn.Px > 0 ? tn.Px : choose_P(c->px.p, eb, (ny * nz + 1) / 2);
+    c->py.tile = tn.By > 0 ? tn.By : choose_B(c->py.p, eb, c->H);
+    c->pz1.tile = tn.Bz > 0 ? tn.Bz : choose_B(c->pz1.p, eb, c->H);
     c->pz3.tile = choose_B(c->pz3.p, eb, c->H);
+    // specialised kernels: fixed E and tile shape per length
+    for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) {
+        int E = 0, B = 0, P = 0;
+        const bool allowed = tn.stat && (d != &c->pz3 || tn.stat3);
+        if (allowed && static_policy(d->p.L, eb, &E, &B, &P)) {
+            d->stat = true;
+            d->p.E = E;
+            d->p.TP = d->p.L / E;
+            d->tile = (d == &c->px) ? P : B;
+            if (B >= 2 && ((d == &c->py && tn.halfb_y) || ((d == &c->pz1 || d == &c->pz3) && tn.halfb_z)))
+                d->tile = B / 2;
+        }
+    }
+    if (c->zb > 0) {
+        for (DevPlan* d : {&c->pz1, &c->pz3})
+            if (d->stat && (d->p.TP % (1 << c->zb)) != 0) d->stat = false;
+    }
+    for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3})
+        if (d->p.TP * d->tile > kMaxThreads)
+            return fail(c, PS_EINVAL, "tile shape exceeds the workgroup size bound");
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3})
         if ((r = upload_twiddles<T>(c, d))) return r;
 

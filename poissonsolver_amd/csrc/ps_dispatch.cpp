@@ -1,6 +1,7 @@
 // Runtime dispatch from a plan's elements-per-thread value to the kernels
 // instantiated for it in kern_e.hip.
 #include "ps_launch.hpp"
+#include "ps_static.hpp"
 
 namespace psamd {
 
@@ -41,6 +42,49 @@ template <typename T> hipError_t launch_pencil(int E, int NC, const PencilArgs<T
     default: return hipErrorInvalidValue;
     }
 }
+
+bool static_policy(int L, int elem_bytes, int* E, int* B, int* P)
+{
+    switch (L) {
+#define PS_CASE(l)                                                                            \
+    case l:                                                                                   \
+        if (elem_bytes == 4) {                                                                \
+            *E = SPolicy<float, l>::E; *B = SPolicy<float, l>::B; *P = SPolicy<float, l>::P;   \
+        } else {                                                                              \
+            *E = SPolicy<double, l>::E; *B = SPolicy<double, l>::B; *P = SPolicy<double, l>::P; \
+        }                                                                                     \
+        return true;
+        PS_FOR_EACH_SL(PS_CASE)
+#undef PS_CASE
+    default: return false;
+    }
+}
+
+template <typename T> hipError_t static_launch_x(int L, const XArgs<T>& a, int inverse, int half, hipStream_t s)
+{
+    switch (L) {
+#define PS_CASE(l) case l: return s_launch_x<T, l>(a, inverse, half, s);
+        PS_FOR_EACH_SL(PS_CASE)
+#undef PS_CASE
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename T>
+hipError_t static_launch_pencil(int L, const PencilArgs<T>& a, int nc, int mode, int half, hipStream_t s)
+{
+    switch (L) {
+#define PS_CASE(l) case l: return s_launch_pencil<T, l>(a, nc, mode, half, s);
+        PS_FOR_EACH_SL(PS_CASE)
+#undef PS_CASE
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template hipError_t static_launch_x<float>(int, const XArgs<float>&, int, int, hipStream_t);
+template hipError_t static_launch_x<double>(int, const XArgs<double>&, int, int, hipStream_t);
+template hipError_t static_launch_pencil<float>(int, const PencilArgs<float>&, int, int, int, hipStream_t);
+template hipError_t static_launch_pencil<double>(int, const PencilArgs<double>&, int, int, int, hipStream_t);
 
 template hipError_t launch_x_r2c<float>(int, const XArgs<float>&, hipStream_t);
 template hipError_t launch_x_r2c<double>(int, const XArgs<double>&, hipStream_t);

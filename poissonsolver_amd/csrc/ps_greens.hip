@@ -108,7 +108,8 @@ __global__ void k_finalize(T* out, const double* gsp, int mode, const double* g2
             }
             z = sum * exp(-a);
         }
-        out[id] = (T)(g * z * invN);
+        // stored as [ky][kz][kx]: a z pencil tile (fixed ky) reads one contiguous slab
+        out[((long)j * Oz + k) * Ox + i] = (T)(g * z * invN);
     }
 }
 
@@ -164,10 +165,8 @@ static int dct_pass(double* g, const int O[3], int d, int M, hipStream_t s, std:
     } else {
         pos_s = sz; col_s = sx; slab_s = sy; ncol = O[0]; nslab = O[1];
     }
-    a.in_pos = a.out_pos = pos_s;
-    a.in_col = a.out_col = col_s;
-    a.in_slab = a.out_slab = slab_s;
-    a.in_comp = a.out_comp = 0;
+    a.ia = a.oa = addr_plain(0, slab_s, pos_s, col_s);
+    a.xcd_remap = 1;
     a.n_in = M;
     a.n_out = M / 2 + 1;
     a.ncol = ncol;

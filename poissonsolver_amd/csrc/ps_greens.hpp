@@ -13,8 +13,9 @@ constexpr int kGreensENOMEM = PS_ENOMEM;
 constexpr int kGreensEHIP = PS_EHIP;
 constexpr int kGreensEUNSUPPORTED = PS_EUNSUPPORTED;
 
-// Ghat * zeta / (Mx My Mz) on the octant (Mz/2+1, My/2+1, Mx/2+1), x fastest,
-// written to `out` (device memory of that many T).  Returns a PS_* status.
+// Ghat * zeta / (Mx My Mz) on the octant kx <= Mx/2, ky <= My/2, kz <= Mz/2,
+// stored [ky][kz][kx] (kx fastest) in `out` (device memory of that many T).
+// Returns a PS_* status.
 template <typename T>
 int build_greens(const int ncell[3], const int bc[3], const double dx[3], int reg, T* out, hipStream_t s,
                  std::string* err);

@@ -26,20 +26,36 @@ enum PencilOp : int {
 enum LoadMode : int { LOAD_COMPLEX = 0, LOAD_REAL_EVEN = 1 };
 enum StoreMode : int { STORE_COMPLEX = 0, STORE_REAL = 1 };
 
+// Element addressing of a strided pass: element (c, slab, pos, col) sits at
+//   c*comp + col*col + off(slab) + off(pos),
+//   off(i) = (i >> sh)*hi + (i & (2^sh - 1))*lo,
+// i.e. a plain stride (sh = 30, lo = stride) or a layout blocked by 2^sh.
+struct Addr {
+    long comp, col, slab_lo, slab_hi, pos_lo, pos_hi;
+    int slab_sh, pos_sh;
+};
+
+inline Addr addr_plain(long comp, long slab, long pos, long col)
+{
+    return Addr{comp, col, slab, 0, pos, 0, 30, 30};
+}
+
 // Strided pencil pass: pencils along `pos`, tiles of B columns.
-// element (c, slab, pos, col) at c*comp + slab*slab + pos*pos + col*col
 template <typename T> struct PencilArgs {
     FftPlan<T> plan;
     const void* in;
     void* out;
-    long in_comp, in_slab, in_pos, in_col;
-    long out_comp, out_slab, out_pos, out_col;
+    Addr ia, oa;
     int n_in;       // input positions present (the rest are zero padding)
     int n_out;      // output positions stored
     int ncol, B, ncoltile, nslab, ncomp;
     int load_mode, store_mode;
     int do_fwd, do_inv;
     int op;
+    int xcd_remap;   // 1: contiguous work-id ranges per XCD
+    int twreg;       // specialised kernels: 1 = stage twiddles held in registers
+    int occ4;        // specialised fused kernel: target 4 waves/SIMD (else 2)
+    int slab_pair;   // 1: visit slabs as 0, M/2, 1, M-1, 2, M-2, ... (z pass: ky and My-ky share a row of Gk)
     // k-space operator (z pass): slab = ky, col = kx, pos = kz
     const T* G;
     long g_ky, g_kz;

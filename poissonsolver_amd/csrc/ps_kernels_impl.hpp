@@ -20,10 +20,169 @@ __device__ __forceinline__ long xcd_remap(long lin, long nblk)
 
 // Wavenumber of index m of a length-M transform (greens3d.cpp:75-118): the
 // index maps to m - M when 2m >= M, so the Nyquist wavenumber is negative.
+// slab order 0, M/2, 1, M-1, 2, M-2, ... : the two slabs that share a row of
+// the kernel octant are processed back to back (same XCD, L2-resident row)
+__device__ __forceinline__ int paired_slab(int s, int M)
+{
+    if (s == 0) return 0;
+    if ((M & 1) == 0) {
+        if (s == 1) return M / 2;
+        s -= 2;
+    } else {
+        s -= 1;
+    }
+    const int j = s / 2 + 1;
+    return (s & 1) ? M - j : j;
+}
+
+__device__ __forceinline__ long boff(int i, int sh, long lo, long hi)
+{
+    return (long)(i >> sh) * hi + (long)(i & ((1 << sh) - 1)) * lo;
+}
+
 __device__ __forceinline__ double kappa_full(int m, int M, double dk)
 {
     const int mm = (2 * m < M) ? m : m - M;
     return 2.0 * kPi * (double)mm * dk;
+}
+
+
+// Per-tile wavenumber data of the z pass: x from the column, y from the slab;
+// kh is the Hermitian part of i*k (Nyquist zeroed), kn the Nyquist-only part.
+template <typename T> struct TileK {
+    T kxh, kyh, kxn, kyn;
+    double kxf, kyf;
+    const T* grow;
+};
+
+template <typename T>
+__device__ __forceinline__ TileK<T> tile_k(const PencilArgs<T>& a, int col, int slab, bool cv)
+{
+    TileK<T> k{T(0), T(0), T(0), T(0), 0.0, 0.0, nullptr};
+    if (a.op == OP_NONE) return k;
+    const int kx = col, ky = slab;
+    k.kxf = kappa_full(kx, a.Mx, a.dk[0]);
+    k.kyf = kappa_full(ky, a.My, a.dk[1]);
+    const bool nyx = (2 * kx == a.Mx), nyy = (2 * ky == a.My);
+    k.kxh = nyx ? T(0) : (T)k.kxf;
+    k.kxn = nyx ? (T)k.kxf : T(0);
+    k.kyh = nyy ? T(0) : (T)k.kyf;
+    k.kyn = nyy ? (T)k.kyf : T(0);
+    const int kyo = (ky <= a.My - ky) ? ky : a.My - ky;
+    if (cv && a.G) k.grow = a.G + kyo * a.g_ky + kx;
+    return k;
+}
+
+// k-space operator of the fused z pass on one thread's E spectrum values
+// (solve3d.cpp:329-396 in the Hermitian-part form of the half spectrum).
+// Output o of the NC inputs v goes to w; positions are kz = t + TP u.
+template <typename T, int E, int NC>
+__device__ __forceinline__ void kspace_op(cplx<T> (&w)[E], const cplx<T> (&v)[NC][E], const PencilArgs<T>& a,
+                                          const TileK<T>& tk, int o, int comp, int t, int TP)
+{
+    using C = cplx<T>;
+    const int op = a.op;
+    const int vi = (NC == 3) ? 1 : 0, wi = (NC == 3) ? 2 : 0;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+        const int kz = t + TP * u;
+        if (op == OP_NONE) {
+            w[u] = v[0][u];
+            continue;
+        }
+        const int kzo = (kz <= a.Mz - kz) ? kz : a.Mz - kz;
+        const T g = tk.grow ? tk.grow[kzo * a.g_kz] : T(0);
+        if (op == OP_PLAIN) {
+            w[u] = cscale(v[0][u], g);
+            continue;
+        }
+        const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
+        const bool nyz = (2 * kz == a.Mz);
+        const T kzh = nyz ? T(0) : (T)kzf;
+        const T kzn = nyz ? (T)kzf : T(0);
+        C r;
+        if (op == OP_GRAD) {
+            const T kd = (comp == 0) ? tk.kxh : (comp == 1) ? tk.kyh : kzh;
+            const C s = v[0][u];
+            r = C{-kd * g * s.y, kd * g * s.x};
+        } else if (op == OP_DIV || op == OP_CURL) {
+            const C sx = v[0][u], sy = v[vi][u], sz = v[wi][u];
+            C q;
+            if (op == OP_DIV) {
+                q = C{tk.kxh * sx.x + tk.kyh * sy.x + kzh * sz.x, tk.kxh * sx.y + tk.kyh * sy.y + kzh * sz.y};
+            } else if (o == 0) {
+                q = C{tk.kyh * sz.x - kzh * sy.x, tk.kyh * sz.y - kzh * sy.y};
+            } else if (o == 1) {
+                q = C{kzh * sx.x - tk.kxh * sz.x, kzh * sx.y - tk.kxh * sz.y};
+            } else {
+                q = C{tk.kxh * sy.x - tk.kyh * sx.x, tk.kxh * sy.y - tk.kyh * sx.y};
+            }
+            r = C{-g * q.y, g * q.x};   // g * i * q
+        } else {
+            // filtered (and reprojected) rhs: zeta(|k|) / N
+            double zf = a.invN;
+            if (a.reg > 0) {
+                const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
+                double sum = 0.0, term = 1.0;
+                for (int n = 0; n < a.reg / 2; ++n) {
+                    sum += term;
+                    term = term * arg / (double)(n + 1);
+                }
+                zf *= sum * exp(-arg);
+            }
+            const T z = (T)zf;
+            const int oc = (NC == 1) ? 0 : o;
+            r = cscale(v[oc][u], z);
+            if (NC == 3 && op == OP_RHSF_PROJ) {
+                const C sx = v[0][u], sy = v[vi][u], sz = v[wi][u];
+                const C dh = C{tk.kxh * sx.x + tk.kyh * sy.x + kzh * sz.x, tk.kxh * sx.y + tk.kyh * sy.y + kzh * sz.y};
+                const C dn = C{tk.kxn * sx.x + tk.kyn * sy.x + kzn * sz.x, tk.kxn * sx.y + tk.kyn * sy.y + kzn * sz.y};
+                const T ah = (o == 0) ? tk.kxh : (o == 1) ? tk.kyh : kzh;
+                const T an = (o == 0) ? tk.kxn : (o == 1) ? tk.kyn : kzn;
+                r = C{r.x - g * (ah * dh.x + an * dn.x), r.y - g * (ah * dh.y + an * dn.y)};
+            }
+        }
+        w[u] = r;
+    }
+}
+
+// One-component operators (plain, grad, filtered rhs) in place on v.
+template <typename T, int E>
+__device__ __forceinline__ void kspace_op_inplace(cplx<T> (&v)[E], const PencilArgs<T>& a, const TileK<T>& tk,
+                                                  int comp, int t, int TP)
+{
+    using C = cplx<T>;
+    const int op = a.op;
+    if (op == OP_NONE) return;
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+        const int kz = t + TP * u;
+        const int kzo = (kz <= a.Mz - kz) ? kz : a.Mz - kz;
+        const T g = tk.grow ? tk.grow[kzo * a.g_kz] : T(0);
+        if (op == OP_PLAIN) {
+            v[u] = cscale(v[u], g);
+        } else if (op == OP_GRAD) {
+            T kd;
+            if (comp == 0) kd = tk.kxh;
+            else if (comp == 1) kd = tk.kyh;
+            else kd = (2 * kz == a.Mz) ? T(0) : (T)kappa_full(kz, a.Mz, a.dk[2]);
+            const C s = v[u];
+            v[u] = C{-kd * g * s.y, kd * g * s.x};
+        } else {
+            const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
+            double zf = a.invN;
+            if (a.reg > 0) {
+                const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
+                double sum = 0.0, term = 1.0;
+                for (int n = 0; n < a.reg / 2; ++n) {
+                    sum += term;
+                    term = term * arg / (double)(n + 1);
+                }
+                zf *= sum * exp(-arg);
+            }
+            v[u] = cscale(v[u], (T)zf);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -40,12 +199,13 @@ __global__ void __launch_bounds__(kMaxThreads) k_pencil(PencilArgs<T> a)
     const int tid = threadIdx.x;
     const int p = tid % B, t = tid / B;
     const long nblk = gridDim.x;
-    const long lin = xcd_remap(blockIdx.x, nblk);
+    const long lin = a.xcd_remap ? xcd_remap(blockIdx.x, nblk) : (long)blockIdx.x;
     const long ntile = (long)a.ncoltile * a.nslab;
     const int comp = (int)(lin / ntile);
     const long rem = lin - (long)comp * ntile;
-    const int slab = (int)(rem / a.ncoltile);
-    const int ct = (int)(rem - (long)slab * a.ncoltile);
+    const int slab0 = (int)(rem / a.ncoltile);
+    const int ct = (int)(rem - (long)slab0 * a.ncoltile);
+    const int slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
     const int col = ct * B + p;
     const bool cv = col < a.ncol;
     C* my = lds + p;
@@ -54,17 +214,17 @@ __global__ void __launch_bounds__(kMaxThreads) k_pencil(PencilArgs<T> a)
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         const int cc = (NC == 1) ? comp : c;
-        const long base = cc * a.in_comp + slab * a.in_slab + col * a.in_col;
+        const long base = cc * a.ia.comp + col * a.ia.col + boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
 #pragma unroll
         for (int u = 0; u < E; ++u) {
             const int pos = t + TP * u;
             C x = C{0, 0};
             if (cv && pos < a.n_in) {
                 if (a.load_mode == LOAD_COMPLEX) {
-                    x = reinterpret_cast<const C*>(a.in)[base + pos * a.in_pos];
+                    x = reinterpret_cast<const C*>(a.in)[base + boff(pos, a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi)];
                 } else {
                     const int n = (pos <= L - pos) ? pos : L - pos;
-                    x.x = reinterpret_cast<const T*>(a.in)[base + n * a.in_pos];
+                    x.x = reinterpret_cast<const T*>(a.in)[base + boff(n, a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi)];
                 }
             }
             v[c][u] = x;
@@ -77,99 +237,23 @@ __global__ void __launch_bounds__(kMaxThreads) k_pencil(PencilArgs<T> a)
 
     const int op = a.op;
     const int nout = (NC == 1 || op == OP_DIV) ? 1 : 3;
-    // per-tile wavenumbers: x from the column, y from the slab (Hermitian
-    // part kh: Nyquist zeroed; kn: Nyquist only)
-    const int kx = col, ky = slab;
-    T kxh = 0, kyh = 0, kxn = 0, kyn = 0;
-    double kxf = 0, kyf = 0;
-    const T* grow = nullptr;
-    if (op != OP_NONE) {
-        kxf = kappa_full(kx, a.Mx, a.dk[0]);
-        kyf = kappa_full(ky, a.My, a.dk[1]);
-        const bool nyx = (2 * kx == a.Mx), nyy = (2 * ky == a.My);
-        kxh = nyx ? T(0) : (T)kxf;
-        kxn = nyx ? (T)kxf : T(0);
-        kyh = nyy ? T(0) : (T)kyf;
-        kyn = nyy ? (T)kyf : T(0);
-        const int kyo = (ky <= a.My - ky) ? ky : a.My - ky;
-        if (cv && a.G) grow = a.G + kyo * a.g_ky + kx;
-    }
+    const TileK<T> tk = tile_k(a, col, slab, cv);
 
     for (int o = 0; o < nout; ++o) {
         C w[E];
-#pragma unroll
-        for (int u = 0; u < E; ++u) {
-            const int kz = t + TP * u;
-            if (op == OP_NONE) {
-                w[u] = v[0][u];
-                continue;
-            }
-            const int kzo = (kz <= a.Mz - kz) ? kz : a.Mz - kz;
-            const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
-            const bool nyz = (2 * kz == a.Mz);
-            const T kzh = nyz ? T(0) : (T)kzf;
-            const T kzn = nyz ? (T)kzf : T(0);
-            const T g = grow ? grow[kzo * a.g_kz] : T(0);
-            C r;
-            if (op == OP_PLAIN) {
-                r = cscale(v[0][u], g);
-            } else if (op == OP_GRAD) {
-                const T kd = (comp == 0) ? kxh : (comp == 1) ? kyh : kzh;
-                const C s = v[0][u];
-                r = C{-kd * g * s.y, kd * g * s.x};
-            } else if (op == OP_DIV || op == OP_CURL) {
-                const int vi = (NC == 3) ? 1 : 0, wi = (NC == 3) ? 2 : 0;
-                const C sx = v[0][u], sy = v[vi][u], sz = v[wi][u];
-                C q;
-                if (op == OP_DIV) {
-                    q = C{kxh * sx.x + kyh * sy.x + kzh * sz.x, kxh * sx.y + kyh * sy.y + kzh * sz.y};
-                } else if (o == 0) {
-                    q = C{kyh * sz.x - kzh * sy.x, kyh * sz.y - kzh * sy.y};
-                } else if (o == 1) {
-                    q = C{kzh * sx.x - kxh * sz.x, kzh * sx.y - kxh * sz.y};
-                } else {
-                    q = C{kxh * sy.x - kyh * sx.x, kxh * sy.y - kyh * sx.y};
-                }
-                r = C{-g * q.y, g * q.x};   // g * i * q
-            } else {
-                // filtered (and reprojected) rhs: zeta(|k|) / N
-                double zf = a.invN;
-                if (a.reg > 0) {
-                    const double arg = a.zarg * (kxf * kxf + kyf * kyf + kzf * kzf);
-                    double sum = 0.0, term = 1.0;
-                    for (int n = 0; n < a.reg / 2; ++n) {
-                        sum += term;
-                        term = term * arg / (double)(n + 1);
-                    }
-                    zf *= sum * exp(-arg);
-                }
-                const T z = (T)zf;
-                const int oc = (NC == 1) ? 0 : o;
-                r = cscale(v[oc][u], z);
-                if (NC == 3 && op == OP_RHSF_PROJ) {
-                    const int vi = (NC == 3) ? 1 : 0, wi = (NC == 3) ? 2 : 0;
-                    const C sx = v[0][u], sy = v[vi][u], sz = v[wi][u];
-                    const C dh = C{kxh * sx.x + kyh * sy.x + kzh * sz.x, kxh * sx.y + kyh * sy.y + kzh * sz.y};
-                    const C dn = C{kxn * sx.x + kyn * sy.x + kzn * sz.x, kxn * sx.y + kyn * sy.y + kzn * sz.y};
-                    const T ah = (o == 0) ? kxh : (o == 1) ? kyh : kzh;
-                    const T an = (o == 0) ? kxn : (o == 1) ? kyn : kzn;
-                    r = C{r.x - g * (ah * dh.x + an * dn.x), r.y - g * (ah * dh.y + an * dn.y)};
-                }
-            }
-            w[u] = r;
-        }
+        kspace_op<T, E, NC>(w, v, a, tk, o, comp, t, TP);
         if (a.do_inv) fft_pencil<E, +1, T>(w, a.plan, my, B, t);
 
         const int oc = (NC == 1) ? comp : o;
-        const long obase = oc * a.out_comp + slab * a.out_slab + col * a.out_col;
+        const long obase = oc * a.oa.comp + col * a.oa.col + boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi);
 #pragma unroll
         for (int u = 0; u < E; ++u) {
             const int pos = t + TP * u;
             if (cv && pos < a.n_out) {
                 if (a.store_mode == STORE_COMPLEX)
-                    reinterpret_cast<C*>(a.out)[obase + pos * a.out_pos] = w[u];
+                    reinterpret_cast<C*>(a.out)[obase + boff(pos, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi)] = w[u];
                 else
-                    reinterpret_cast<T*>(a.out)[obase + pos * a.out_pos] = w[u].x;
+                    reinterpret_cast<T*>(a.out)[obase + boff(pos, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi)] = w[u].x;
             }
         }
     }

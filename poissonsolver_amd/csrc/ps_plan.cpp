@@ -17,6 +17,8 @@ const std::vector<int>& compiled_E()
 }
 
 static const int kRadices[] = {16, 8, 7, 5, 4, 3, 2};
+static int g_pref_E = 16;
+void set_preferred_E(int e) { g_pref_E = e > 0 ? e : 16; }
 
 bool make_plan(int L, int maxE, Plan1D* out, std::string* why)
 {
@@ -62,7 +64,7 @@ bool make_plan(int L, int maxE, Plan1D* out, std::string* why)
         }
         if (rem != 1) continue;
         // fewer stages first (one LDS exchange each), then E closest to 16
-        auto score = [](const Plan1D& q) { return q.nstage * 100 + std::abs(q.E - 16); };
+        auto score = [](const Plan1D& q) { return q.nstage * 100 + std::abs(q.E - g_pref_E); };
         if (!found || score(c) < score(best)) {
             best = c;
             found = true;
@@ -80,10 +82,12 @@ int max_E_for(int precision_bytes, int ncomp)
 {
     if (ncomp >= 3) return 10;
     (void)precision_bytes;
-    return 30;
+    return 32;
 }
 
-static int lds_budget() { return 64 * 1024; }
+static int g_lds_budget = 80 * 1024;
+void set_lds_budget(int bytes) { g_lds_budget = bytes < 16384 ? 16384 : (bytes > 160 * 1024 ? 160 * 1024 : bytes); }
+static int lds_budget() { return g_lds_budget; }
 
 int choose_B(const Plan1D& p, int elem_bytes, int ncol)
 {

@@ -23,6 +23,8 @@ const std::vector<int>& compiled_E();
 // Choose E and radices for length L with E <= maxE.  Returns false (and a
 // reason) when L has a prime factor above 7 or no compiled E fits.
 bool make_plan(int L, int maxE, Plan1D* out, std::string* why);
+// Among plans with the fewest stages, prefer E closest to this value (default 16).
+void set_preferred_E(int e);
 
 // Largest E allowed for a kernel variant: the 3-component fused z pass
 // holds three pencils per thread, so it gets a smaller register budget.
@@ -30,6 +32,8 @@ int max_E_for(int precision_bytes, int ncomp);
 
 // Workgroup shape of a strided pass (columns per tile) and of the x pass
 // (row pairs per workgroup), bounded by kMaxThreads and the LDS budget.
+// LDS budget per workgroup used by choose_B/choose_P (bytes).
+void set_lds_budget(int bytes);
 int choose_B(const Plan1D& p, int elem_bytes, int ncol);
 int choose_P(const Plan1D& p, int elem_bytes, int npairs);
 

@@ -1,0 +1,311 @@
+// Pass kernels specialised at compile time for one transform length L
+// (instantiated by skern.hip, once per L).  Same data layouts, arguments and
+// results as the runtime-plan kernels of ps_kernels_impl.hpp.
+#pragma once
+#include "ps_kernels_impl.hpp"
+#include "ps_sfft.hpp"
+#include "ps_static.hpp"
+
+namespace psamd {
+
+// ---------------------------------------------------------------------------
+// x pass, forward (two real rows per complex pencil).  HALF: rows hold L/2
+// values followed by zero padding (unbounded x).
+// ---------------------------------------------------------------------------
+template <typename T, int L, int HALF>
+__global__ void __launch_bounds__(kMaxThreads, 4) k_x_r2c_s(XArgs<T> a)
+{
+    using C = cplx<T>;
+    using Pol = SPolicy<T, L>;
+    constexpr int E = Pol::E, TP = Pol::TP, P = Pol::P;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int pp = threadIdx.x / TP, t = threadIdx.x % TP;
+    const long pair = (long)blockIdx.x * P + pp;
+    const int comp = blockIdx.y;
+    const long ra = 2 * pair, rb = ra + 1;
+    const bool va = ra < a.nrows, vb = rb < a.nrows;
+    const T* in = reinterpret_cast<const T*>(a.in) + comp * a.r_comp;
+
+    STw<L, E, T> W;
+    load_stwiddles<L, E, T>(W, t, a.plan.tw);
+    C v[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+        const int pos = t + TP * u;
+        C x = C{0, 0};
+        if (!(HALF && u >= E / 2) && (HALF || pos < a.nvalid)) {
+            if (va) x.x = in[ra * a.r_row + pos];
+            if (vb) x.y = in[rb * a.r_row + pos];
+        }
+        v[u] = x;
+    }
+    C* my = lds + pp * lds_len(L);
+    constexpr int WS = (64 % TP == 0) ? 1 : 0;   // each pencil inside one wavefront
+    sfft<L, E, 1, -1, T, WS>(v, W, my, t);
+
+#pragma unroll
+    for (int u = 0; u < E; ++u) my[lds_pos(t + TP * u)] = v[u];
+    xsync<WS>();
+    C* out = reinterpret_cast<C*>(a.out) + comp * a.c_comp;
+    const T half = T(0.5);
+#pragma unroll
+    for (int u = 0; u <= E / 2; ++u) {
+        const int pos = t + TP * u;
+        if (u < E / 2 || pos == L / 2) {
+            const int mp = (pos == 0) ? 0 : L - pos;
+            const C zk = v[u], zm = my[lds_pos(mp)];
+            if (va) out[ra * a.c_row + pos] = C{half * (zk.x + zm.x), half * (zk.y - zm.y)};
+            if (vb) out[rb * a.c_row + pos] = C{half * (zk.y + zm.y), half * (zm.x - zk.x)};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x pass, inverse.  HALF: only the first L/2 outputs are stored (truncation).
+// ---------------------------------------------------------------------------
+template <typename T, int L, int HALF>
+__global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
+{
+    using C = cplx<T>;
+    using Pol = SPolicy<T, L>;
+    constexpr int E = Pol::E, TP = Pol::TP, P = Pol::P;
+    constexpr int H = L / 2 + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int pp = threadIdx.x / TP, t = threadIdx.x % TP;
+    const long pair = (long)blockIdx.x * P + pp;
+    const int comp = blockIdx.y;
+    const long ra = 2 * pair, rb = ra + 1;
+    const bool va = ra < a.nrows, vb = rb < a.nrows;
+    const C* in = reinterpret_cast<const C*>(a.in) + comp * a.c_comp;
+
+    STw<L, E, T> W;
+    load_stwiddles<L, E, T>(W, t, a.plan.tw);
+    C v[E];
+#pragma unroll
+    for (int u = 0; u < E; ++u) {
+        const int pos = t + TP * u;
+        C za = C{0, 0}, zb = C{0, 0};
+        if (pos < H) {
+            if (va) za = in[ra * a.c_row + pos];
+            if (vb) zb = in[rb * a.c_row + pos];
+            if (pos == 0 || 2 * pos == L) {
+                za.y = 0;
+                zb.y = 0;
+            }
+        } else {
+            const int m = L - pos;
+            if (va) za = in[ra * a.c_row + m];
+            if (vb) zb = in[rb * a.c_row + m];
+            za.y = -za.y;
+            zb.y = -zb.y;
+        }
+        v[u] = C{za.x - zb.y, za.y + zb.x};
+    }
+    C* my = lds + pp * lds_len(L);
+    constexpr int WS = (64 % TP == 0) ? 1 : 0;
+    sfft<L, E, 1, +1, T, WS>(v, W, my, t);
+    T* out = reinterpret_cast<T*>(a.out) + comp * a.r_comp;
+#pragma unroll
+    for (int u = 0; u < (HALF ? E / 2 : E); ++u) {
+        const int pos = t + TP * u;
+        if (HALF || pos < a.nvalid) {
+            if (va) out[ra * a.r_row + pos] = v[u].x;
+            if (vb) out[rb * a.r_row + pos] = v[u].y;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// strided pencil pass.  MODE: SM_FWD (y pass), SM_INV (y^-1), SM_FUSED (z pass).
+// HIN: input holds L/2 positions followed by zero padding; HOUT: only the
+// first L/2 positions are stored.
+// ---------------------------------------------------------------------------
+// MINW: occupancy target in waves per SIMD; 4 keeps a kernel within 128
+// VGPRs (two 512-thread workgroups per CU), 2 allows 256.
+template <typename T, int L, int NC, int MODE, int HIN, int HOUT, int TWREG, int MINW, int B>
+__global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
+{
+    using C = cplx<T>;
+    using Pol = SPolicy<T, L>;
+    constexpr int E = Pol::E, TP = Pol::TP;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+
+    const int tid = threadIdx.x;
+    const int p = tid % B, t = tid / B;
+    const long nblk = gridDim.x;
+    const long lin = a.xcd_remap ? xcd_remap(blockIdx.x, nblk) : (long)blockIdx.x;
+    const long ntile = (long)a.ncoltile * a.nslab;
+    const int comp = (int)(lin / ntile);
+    const long rem = lin - (long)comp * ntile;
+    const int slab0 = (int)(rem / a.ncoltile);
+    const int ct = (int)(rem - (long)slab0 * a.ncoltile);
+    const int slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    const int col = ct * B + p;
+    const bool cv = col < a.ncol;
+    const bool full = (ct + 1) * B <= a.ncol;     // uniform: every column of the tile exists
+    C* my = lds + p;
+
+    STw<L, E, T> W;
+    if constexpr (TWREG) load_stwiddles<L, E, T>(W, t, a.plan.tw);
+
+    // position t + TP u sits at off(t) + u * ustride (TP is a multiple of the
+    // position block, checked on the host)
+    const long iu = (a.ia.pos_sh >= 30) ? TP * a.ia.pos_lo : (long)(TP >> a.ia.pos_sh) * a.ia.pos_hi;
+    const long ou = (a.oa.pos_sh >= 30) ? TP * a.oa.pos_lo : (long)(TP >> a.oa.pos_sh) * a.oa.pos_hi;
+
+    C v[NC][E];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int cc = (NC == 1) ? comp : c;
+        const C* ip = reinterpret_cast<const C*>(a.in) + cc * a.ia.comp + col * a.ia.col +
+                      boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi) +
+                      boff(t, a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi);
+        if (full) {
+#pragma unroll
+            for (int u = 0; u < E; ++u) {
+                if (HIN && u >= E / 2) v[c][u] = C{0, 0};
+                else if (HIN || t + TP * u < a.n_in) v[c][u] = ip[u * iu];
+                else v[c][u] = C{0, 0};
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < E; ++u) {
+                C x = C{0, 0};
+                if (!(HIN && u >= E / 2) && cv && (HIN || t + TP * u < a.n_in)) x = ip[u * iu];
+                v[c][u] = x;
+            }
+        }
+    }
+    if constexpr (MODE != SM_INV) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if constexpr (TWREG) sfft<L, E, B, -1, T>(v[c], W, my, t);
+            else sfft_g<L, E, B, -1, T>(v[c], a.plan.tw, my, t);
+        }
+    }
+
+    const int nout = (NC == 1 || a.op == OP_DIV) ? 1 : 3;
+    TileK<T> tk{};
+    if constexpr (MODE == SM_FUSED) tk = tile_k(a, col, slab, cv);
+
+    for (int o = 0; o < nout; ++o) {
+        C wbuf[NC == 1 ? 1 : E];
+        // one-component operators work in place on v[0]
+        C(&w)[E] = *reinterpret_cast<C(*)[E]>(NC == 1 ? &v[0][0] : &wbuf[0]);
+        if constexpr (MODE == SM_FUSED) {
+            if constexpr (NC == 1) kspace_op_inplace<T, E>(v[0], a, tk, comp, t, TP);
+            else kspace_op<T, E, NC>(w, v, a, tk, o, comp, t, TP);
+        }
+        if constexpr (MODE != SM_FWD) {
+            if constexpr (TWREG) sfft<L, E, B, +1, T>(w, W, my, t);
+            else sfft_g<L, E, B, +1, T>(w, a.plan.tw, my, t);
+        }
+
+        const int oc = (NC == 1) ? comp : o;
+        C* op_ = reinterpret_cast<C*>(a.out) + oc * a.oa.comp + col * a.oa.col +
+                 boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi) + boff(t, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
+        if (full) {
+#pragma unroll
+            for (int u = 0; u < (HOUT ? E / 2 : E); ++u)
+                if (HOUT || t + TP * u < a.n_out) op_[u * ou] = w[u];
+        } else if (cv) {
+#pragma unroll
+            for (int u = 0; u < (HOUT ? E / 2 : E); ++u)
+                if (HOUT || t + TP * u < a.n_out) op_[u * ou] = w[u];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T, int L> hipError_t s_launch_x(const XArgs<T>& a, int inverse, int half, hipStream_t s)
+{
+    using Pol = SPolicy<T, L>;
+    const size_t shm = (size_t)Pol::P * lds_len(L) * sizeof(cplx<T>);
+    dim3 grid((unsigned)((a.npairs + Pol::P - 1) / Pol::P), (unsigned)a.ncomp), block(Pol::TP * Pol::P);
+    hipError_t e;
+    if (!inverse) {
+        if (half) {
+            if ((e = allow_lds(k_x_r2c_s<T, L, 1>, shm)) != hipSuccess) return e;
+            hipLaunchKernelGGL((k_x_r2c_s<T, L, 1>), grid, block, shm, s, a);
+        } else {
+            if ((e = allow_lds(k_x_r2c_s<T, L, 0>, shm)) != hipSuccess) return e;
+            hipLaunchKernelGGL((k_x_r2c_s<T, L, 0>), grid, block, shm, s, a);
+        }
+    } else {
+        if (half) {
+            if ((e = allow_lds(k_x_c2r_s<T, L, 1>, shm)) != hipSuccess) return e;
+            hipLaunchKernelGGL((k_x_c2r_s<T, L, 1>), grid, block, shm, s, a);
+        } else {
+            if ((e = allow_lds(k_x_c2r_s<T, L, 0>, shm)) != hipSuccess) return e;
+            hipLaunchKernelGGL((k_x_c2r_s<T, L, 0>), grid, block, shm, s, a);
+        }
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int L, int NC, int MODE, int HIN, int HOUT, int TWREG, int MINW, int B>
+hipError_t s_launch_pencil_b(const PencilArgs<T>& a, hipStream_t s)
+{
+    using Pol = SPolicy<T, L>;
+    if (a.B != B) return hipErrorInvalidValue;
+    const size_t shm = (size_t)B * lds_len(L) * sizeof(cplx<T>);
+    hipError_t e = allow_lds(k_pencil_s<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, B>, shm);
+    if (e != hipSuccess) return e;
+    const long nwork = (long)a.ncoltile * a.nslab * (NC == 1 ? a.ncomp : 1);
+    hipLaunchKernelGGL((k_pencil_s<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, B>), dim3((unsigned)nwork),
+                       dim3(Pol::TP * B), shm, s, a);
+    return hipGetLastError();
+}
+
+// tile width: the policy's B, or half of it (more, smaller workgroups per CU)
+template <typename T, int L, int NC, int MODE, int HIN, int HOUT, int TWREG, int MINW>
+hipError_t s_launch_pencil_tw(const PencilArgs<T>& a, hipStream_t s)
+{
+    using Pol = SPolicy<T, L>;
+    if constexpr (Pol::B >= 2) {
+        if (a.B == Pol::B / 2) return s_launch_pencil_b<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, Pol::B / 2>(a, s);
+    }
+    return s_launch_pencil_b<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, Pol::B>(a, s);
+}
+
+template <typename T, int L, int NC, int MODE, int HIN, int HOUT>
+hipError_t s_launch_pencil_one(const PencilArgs<T>& a, hipStream_t s)
+{
+    if constexpr (NC == 3) {
+        if (a.twreg) return s_launch_pencil_tw<T, L, NC, MODE, HIN, HOUT, 1, 2>(a, s);
+        return s_launch_pencil_tw<T, L, NC, MODE, HIN, HOUT, 0, 2>(a, s);
+    } else if constexpr (MODE == SM_FUSED) {
+        if (a.occ4) {
+            if (a.twreg) return s_launch_pencil_tw<T, L, NC, MODE, HIN, HOUT, 1, 4>(a, s);
+            return s_launch_pencil_tw<T, L, NC, MODE, HIN, HOUT, 0, 4>(a, s);
+        }
+        if (a.twreg) return s_launch_pencil_tw<T, L, NC, MODE, HIN, HOUT, 1, 2>(a, s);
+        return s_launch_pencil_tw<T, L, NC, MODE, HIN, HOUT, 0, 2>(a, s);
+    } else {
+        if (a.twreg) return s_launch_pencil_tw<T, L, NC, MODE, HIN, HOUT, 1, 4>(a, s);
+        return s_launch_pencil_tw<T, L, NC, MODE, HIN, HOUT, 0, 4>(a, s);
+    }
+}
+
+template <typename T, int L>
+hipError_t s_launch_pencil(const PencilArgs<T>& a, int nc, int mode, int half, hipStream_t s)
+{
+    if (mode == SM_FWD) {
+        if (nc != 1) return hipErrorInvalidValue;
+        return half ? s_launch_pencil_one<T, L, 1, SM_FWD, 1, 0>(a, s) : s_launch_pencil_one<T, L, 1, SM_FWD, 0, 0>(a, s);
+    }
+    if (mode == SM_INV) {
+        if (nc != 1) return hipErrorInvalidValue;
+        return half ? s_launch_pencil_one<T, L, 1, SM_INV, 0, 1>(a, s) : s_launch_pencil_one<T, L, 1, SM_INV, 0, 0>(a, s);
+    }
+    if (nc == 1)
+        return half ? s_launch_pencil_one<T, L, 1, SM_FUSED, 1, 1>(a, s) : s_launch_pencil_one<T, L, 1, SM_FUSED, 0, 0>(a, s);
+    return half ? s_launch_pencil_one<T, L, 3, SM_FUSED, 1, 1>(a, s) : s_launch_pencil_one<T, L, 3, SM_FUSED, 0, 0>(a, s);
+}
+
+}  // namespace psamd

@@ -1,0 +1,53 @@
+// Lengths with compile-time specialised pass kernels, their tile policy and
+// the host entry points (defined in skern.hip, one translation unit per L).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ps_kernels.hpp"
+
+#define PS_FOR_EACH_SL(X) X(64) X(128) X(256) X(512) X(1024) X(2048)
+
+namespace psamd {
+
+enum StaticMode : int { SM_FWD = 0, SM_INV = 1, SM_FUSED = 2 };
+
+constexpr int s_lds_len(int L) { return L + (L >> 5) + 1; }
+
+// columns per strided tile: 128-byte row segments when 512 threads and an
+// 80 KB LDS budget (two workgroups per CU) allow it
+constexpr int s_pick_B(int TP, int L, int cb)
+{
+    int B = 128 / cb;
+    while (B > 1 && (TP * B > 512 || B * s_lds_len(L) * cb > 80 * 1024)) B /= 2;
+    return B;
+}
+
+// row pairs per x-pass workgroup
+constexpr int s_pick_P(int TP, int L, int cb)
+{
+    int P = 8;
+    while (P > 1 && (TP * P > 512 || P * s_lds_len(L) * cb > 40 * 1024)) P /= 2;
+    while (TP * P < 64 && TP * P * 2 <= 512) P *= 2;
+    return P;
+}
+
+template <typename T, int L> struct SPolicy {
+    static constexpr int E = 16;
+    static constexpr int TP = L / E;
+    static constexpr int B = s_pick_B(TP, L, 2 * (int)sizeof(T));
+    static constexpr int P = s_pick_P(TP, L, 2 * (int)sizeof(T));
+};
+
+// Host query: whether L has specialised kernels, and their E / B / P.
+bool static_policy(int L, int elem_bytes, int* E, int* B, int* P);
+
+template <typename T, int L> hipError_t s_launch_x(const XArgs<T>& a, int inverse, int half, hipStream_t s);
+template <typename T, int L>
+hipError_t s_launch_pencil(const PencilArgs<T>& a, int nc, int mode, int half, hipStream_t s);
+
+// runtime dispatch on L (ps_dispatch.cpp)
+template <typename T> hipError_t static_launch_x(int L, const XArgs<T>& a, int inverse, int half, hipStream_t s);
+template <typename T>
+hipError_t static_launch_pencil(int L, const PencilArgs<T>& a, int nc, int mode, int half, hipStream_t s);
+
+}  // namespace psamd
