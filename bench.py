@@ -86,10 +86,22 @@ def main(argv=None):
     ap.add_argument("--op", default="plain", choices=["plain", "curl"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (HBM counter calibration) and exit")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         return 2
+
+    if a.calib_copy:
+        import torch
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_read": src.numel() * 4, "calib_copy_bytes_written": dst.numel() * 4}))
+        return 0
 
     import poissonsolver_amd as P
 

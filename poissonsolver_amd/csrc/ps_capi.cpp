@@ -48,7 +48,7 @@ struct Tuning {
     int twreg_z = 1;              // z pass: stage twiddles in registers (1) or loaded per stage (0)
     int twreg_y = 0;              // y passes: same choice
     int occ4_z = 0;               // z pass: 4 waves/SIMD target (1) or 2 (0)
-    int halfb_z = 1, halfb_y = 0; // specialised strided passes: half-width tiles (measured: z faster, y slower)
+    int halfb_z = 1, halfb_y = 0; // tile-width divisor of the specialised strided passes: z 0/1/2 -> B, B/2, B/4; y 0/1 -> B, B/2
 };
 
 int env_int(const char* name, int dflt)
@@ -431,8 +431,11 @@ template <typename T> int setup_impl(ps_ctx* c)
             d->p.E = E;
             d->p.TP = d->p.L / E;
             d->tile = (d == &c->px) ? P : B;
-            if (B >= 2 && ((d == &c->py && tn.halfb_y) || ((d == &c->pz1 || d == &c->pz3) && tn.halfb_z)))
-                d->tile = B / 2;
+            // tile-width divisor: z (fused) pass 1, 2 or 4; y passes 1 or 2
+            const bool zp = (d == &c->pz1 || d == &c->pz3);
+            int div = zp ? tn.halfb_z : (d == &c->py ? tn.halfb_y : 0);
+            div = zp ? (div >= 2 ? 4 : (div == 1 ? 2 : 1)) : (div ? 2 : 1);
+            if (d != &c->px && B / div >= 1 && B % div == 0) d->tile = B / div;
         }
     }
     if (c->zb > 0) {

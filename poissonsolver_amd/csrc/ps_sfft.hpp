@@ -150,12 +150,26 @@ __device__ __forceinline__ void sfft_from(cplx<T> (&v)[E], const STw<L, E, T>& W
                 const int b = t + i * TP;
                 const int k = b % Ns;
                 const int base = (b - k) * R + k;
+                if constexpr (Ns % 32 == 0) {
+                    // lds_pos(base + q Ns) = lds_pos(base) + q (Ns + Ns/32)
+                    C* w0 = lds + lds_pos(base) * LS;
 #pragma unroll
-                for (int q = 0; q < R; ++q) lds[lds_pos(base + q * Ns) * LS] = v[i + q * NB];
+                    for (int q = 0; q < R; ++q) w0[q * (Ns + Ns / 32) * LS] = v[i + q * NB];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < R; ++q) lds[lds_pos(base + q * Ns) * LS] = v[i + q * NB];
+                }
             }
             xsync<WS>();
+            if constexpr (TP % 32 == 0) {
+                // lds_pos(t + TP u) = lds_pos(t) + u (TP + TP/32): immediate offsets
+                const C* r0 = lds + lds_pos(t) * LS;
 #pragma unroll
-            for (int u = 0; u < E; ++u) v[u] = lds[lds_pos(t + TP * u) * LS];
+                for (int u = 0; u < E; ++u) v[u] = r0[u * (TP + TP / 32) * LS];
+            } else {
+#pragma unroll
+                for (int u = 0; u < E; ++u) v[u] = lds[lds_pos(t + TP * u) * LS];
+            }
             xsync<WS>();
         }
         sfft_from<L, E, LS, DIR, S + 1, TWREG, WS, T>(v, W, tw, lds, t);

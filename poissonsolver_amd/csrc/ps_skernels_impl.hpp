@@ -44,8 +44,14 @@ __global__ void __launch_bounds__(kMaxThreads, 4) k_x_r2c_s(XArgs<T> a)
     constexpr int WS = (64 % TP == 0) ? 1 : 0;   // each pencil inside one wavefront
     sfft<L, E, 1, -1, T, WS>(v, W, my, t);
 
+    if constexpr (TP % 32 == 0) {
+        C* w0 = my + lds_pos(t);
 #pragma unroll
-    for (int u = 0; u < E; ++u) my[lds_pos(t + TP * u)] = v[u];
+        for (int u = 0; u < E; ++u) w0[u * (TP + TP / 32)] = v[u];
+    } else {
+#pragma unroll
+        for (int u = 0; u < E; ++u) my[lds_pos(t + TP * u)] = v[u];
+    }
     xsync<WS>();
     C* out = reinterpret_cast<C*>(a.out) + comp * a.c_comp;
     const T half = T(0.5);
@@ -269,6 +275,9 @@ hipError_t s_launch_pencil_tw(const PencilArgs<T>& a, hipStream_t s)
     using Pol = SPolicy<T, L>;
     if constexpr (Pol::B >= 2) {
         if (a.B == Pol::B / 2) return s_launch_pencil_b<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, Pol::B / 2>(a, s);
+    }
+    if constexpr (Pol::B >= 4 && MODE == SM_FUSED) {
+        if (a.B == Pol::B / 4) return s_launch_pencil_b<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, Pol::B / 4>(a, s);
     }
     return s_launch_pencil_b<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, Pol::B>(a, s);
 }
