@@ -44,7 +44,7 @@ struct Tuning {
     int lds_kb = 80;              // LDS budget per workgroup
     int xcd = 1;                  // XCD-contiguous work ids in strided passes
     int stat = 1;                 // use the specialised kernels where they exist
-    int stat3 = 0;                // ... also for the 3-component z pass
+    int stat3 = 1;                // ... also for the 3-component z pass
     int twreg_z = 1;              // z pass: stage twiddles in registers (1) or loaded per stage (0)
     int twreg_y = 0;              // y passes: same choice
     int occ4_z = 0;               // z pass: 4 waves/SIMD target (1) or 2 (0)
@@ -378,7 +378,7 @@ template <typename T> int setup_impl(ps_ctx* c)
     tn.lds_kb = env_int("PS_TUNE_LDS_KB", 80);
     tn.xcd = env_int("PS_TUNE_XCD", 1);
     tn.stat = env_int("PS_TUNE_STATIC", 1);
-    tn.stat3 = env_int("PS_TUNE_STATIC3", 0);
+    tn.stat3 = env_int("PS_TUNE_STATIC3", 1);
     tn.twreg_z = env_int("PS_TUNE_TWREG_Z", 1);
     tn.twreg_y = env_int("PS_TUNE_TWREG_Y", 0);
     tn.occ4_z = env_int("PS_TUNE_OCC4_Z", 0);
@@ -426,7 +426,7 @@ template <typename T> int setup_impl(ps_ctx* c)
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) {
         int E = 0, B = 0, P = 0;
         const bool allowed = tn.stat && (d != &c->pz3 || tn.stat3);
-        if (allowed && static_policy(d->p.L, eb, &E, &B, &P)) {
+        if (allowed && static_policy(d->p.L, eb, d == &c->pz3 ? 3 : 1, &E, &B, &P)) {
             d->stat = true;
             d->p.E = E;
             d->p.TP = d->p.L / E;

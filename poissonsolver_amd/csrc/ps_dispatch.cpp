@@ -43,16 +43,22 @@ template <typename T> hipError_t launch_pencil(int E, int NC, const PencilArgs<T
     }
 }
 
-bool static_policy(int L, int elem_bytes, int* E, int* B, int* P)
+bool static_policy(int L, int elem_bytes, int ncomp, int* E, int* B, int* P)
 {
     switch (L) {
-#define PS_CASE(l)                                                                            \
-    case l:                                                                                   \
-        if (elem_bytes == 4) {                                                                \
-            *E = SPolicy<float, l>::E; *B = SPolicy<float, l>::B; *P = SPolicy<float, l>::P;   \
-        } else {                                                                              \
-            *E = SPolicy<double, l>::E; *B = SPolicy<double, l>::B; *P = SPolicy<double, l>::P; \
-        }                                                                                     \
+#define PS_CASE(l)                                                                                  \
+    case l:                                                                                         \
+        if (ncomp == 3) {                                                                           \
+            if (elem_bytes == 4) {                                                                  \
+                *E = SPolicy<float, l, 3>::E; *B = SPolicy<float, l, 3>::B; *P = SPolicy<float, l, 3>::P;   \
+            } else {                                                                                \
+                *E = SPolicy<double, l, 3>::E; *B = SPolicy<double, l, 3>::B; *P = SPolicy<double, l, 3>::P; \
+            }                                                                                       \
+        } else if (elem_bytes == 4) {                                                               \
+            *E = SPolicy<float, l>::E; *B = SPolicy<float, l>::B; *P = SPolicy<float, l>::P;         \
+        } else {                                                                                    \
+            *E = SPolicy<double, l>::E; *B = SPolicy<double, l>::B; *P = SPolicy<double, l>::P;      \
+        }                                                                                           \
         return true;
         PS_FOR_EACH_SL(PS_CASE)
 #undef PS_CASE

@@ -239,7 +239,10 @@ __global__ void __launch_bounds__(kMaxThreads) k_pencil(PencilArgs<T> a)
     const int nout = (NC == 1 || op == OP_DIV) ? 1 : 3;
     const TileK<T> tk = tile_k(a, col, slab, cv);
 
-    for (int o = 0; o < nout; ++o) {
+    // o is a compile-time index in each unrolled copy, so v[o] stays in registers
+#pragma unroll
+    for (int o = 0; o < (NC == 1 ? 1 : 3); ++o) {
+        if (o >= nout) break;
         C w[E];
         kspace_op<T, E, NC>(w, v, a, tk, o, comp, t, TP);
         if (a.do_inv) fft_pencil<E, +1, T>(w, a.plan, my, B, t);

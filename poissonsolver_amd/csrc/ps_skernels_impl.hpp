@@ -134,7 +134,7 @@ template <typename T, int L, int NC, int MODE, int HIN, int HOUT, int TWREG, int
 __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
 {
     using C = cplx<T>;
-    using Pol = SPolicy<T, L>;
+    using Pol = SPolicy<T, L, NC>;
     constexpr int E = Pol::E, TP = Pol::TP;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
@@ -197,7 +197,10 @@ __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
     TileK<T> tk{};
     if constexpr (MODE == SM_FUSED) tk = tile_k(a, col, slab, cv);
 
-    for (int o = 0; o < nout; ++o) {
+    // o is a compile-time index in each unrolled copy, so v[o] stays in registers
+#pragma unroll
+    for (int o = 0; o < (NC == 1 ? 1 : 3); ++o) {
+        if (o >= nout) break;
         C wbuf[NC == 1 ? 1 : E];
         // one-component operators work in place on v[0]
         C(&w)[E] = *reinterpret_cast<C(*)[E]>(NC == 1 ? &v[0][0] : &wbuf[0]);
@@ -257,7 +260,7 @@ template <typename T, int L> hipError_t s_launch_x(const XArgs<T>& a, int invers
 template <typename T, int L, int NC, int MODE, int HIN, int HOUT, int TWREG, int MINW, int B>
 hipError_t s_launch_pencil_b(const PencilArgs<T>& a, hipStream_t s)
 {
-    using Pol = SPolicy<T, L>;
+    using Pol = SPolicy<T, L, NC>;
     if (a.B != B) return hipErrorInvalidValue;
     const size_t shm = (size_t)B * lds_len(L) * sizeof(cplx<T>);
     hipError_t e = allow_lds(k_pencil_s<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, B>, shm);
@@ -272,7 +275,7 @@ hipError_t s_launch_pencil_b(const PencilArgs<T>& a, hipStream_t s)
 template <typename T, int L, int NC, int MODE, int HIN, int HOUT, int TWREG, int MINW>
 hipError_t s_launch_pencil_tw(const PencilArgs<T>& a, hipStream_t s)
 {
-    using Pol = SPolicy<T, L>;
+    using Pol = SPolicy<T, L, NC>;
     if constexpr (Pol::B >= 2) {
         if (a.B == Pol::B / 2) return s_launch_pencil_b<T, L, NC, MODE, HIN, HOUT, TWREG, MINW, Pol::B / 2>(a, s);
     }

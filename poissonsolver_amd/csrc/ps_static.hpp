@@ -31,15 +31,18 @@ constexpr int s_pick_P(int TP, int L, int cb)
     return P;
 }
 
-template <typename T, int L> struct SPolicy {
-    static constexpr int E = 16;
+// NC = 3 (the z pass of curl, div and reprojection) holds three pencils per
+// thread, so it uses E = 8 to stay in registers.
+template <typename T, int L, int NC = 1> struct SPolicy {
+    static constexpr int E = (NC == 3) ? 8 : 16;
     static constexpr int TP = L / E;
     static constexpr int B = s_pick_B(TP, L, 2 * (int)sizeof(T));
     static constexpr int P = s_pick_P(TP, L, 2 * (int)sizeof(T));
 };
 
-// Host query: whether L has specialised kernels, and their E / B / P.
-bool static_policy(int L, int elem_bytes, int* E, int* B, int* P);
+// Host query: whether L has specialised kernels, and their E / B / P
+// (ncomp = 3 gives the policy of the three-component z pass).
+bool static_policy(int L, int elem_bytes, int ncomp, int* E, int* B, int* P);
 
 template <typename T, int L> hipError_t s_launch_x(const XArgs<T>& a, int inverse, int half, hipStream_t s);
 template <typename T, int L>
