@@ -76,6 +76,70 @@ def cpu_baseline(n_target, cin, op):
                       "%d reps, %.2f s/solve, scaled to %d^3 by n^3 log n (x%.0f)" % (ns, reps, per, n_target, scale)}
 
 
+CALIB_BYTES = 1 << 30
+
+
+def is_zpass_kernel(name, nc):
+    """k_pencil_s<T, L, NC, MODE, ...> with MODE 2 (fused z pass) and the given NC."""
+    if "k_pencil_s<" not in name:
+        return False
+    args = [x.strip() for x in name.split("k_pencil_s<", 1)[1].split(">", 1)[0].split(",")]
+    return len(args) >= 4 and args[2] == str(nc) and args[3] == "2"
+
+
+def calib_copy():
+    """A plain 1 GiB device-to-device copy: the reference kernel for scaling
+    the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    src = torch.ones(CALIB_BYTES // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def measure_traffic(argv_child, kernel_key, timeout=240):
+    """HBM bytes per dispatch of the dominant kernel from two rocprofv3 --pmc
+    runs (FETCH_SIZE, then WRITE_SIZE) of this script in a child process, each
+    scaled by what the counter reads for the 1 GiB calibration copy.
+    Returns (traffic_bytes, detail) or (None, reason)."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    out = {}
+    env = dict(os.environ, TMPDIR="/tmp")
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="ps_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", ctr, "--output-format", "csv", "-d", d, "-o", "p", "--",
+               sys.executable, os.path.abspath(__file__)] + argv_child
+        try:
+            subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           timeout=timeout, check=False)
+        except subprocess.TimeoutExpired:
+            return None, "rocprofv3 %s pass timed out" % ctr
+        rows = []
+        for root, _, files in os.walk(d):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    rows += list(csv.DictReader(open(os.path.join(root, f))))
+        shutil.rmtree(d, ignore_errors=True)
+        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        kern = [float(r["Counter_Value"]) for r in rows if kernel_key(r["Kernel_Name"])]
+        if not copy or not kern:
+            return None, "%s: counters missing (copy %d, kernel %d rows)" % (ctr, len(copy), len(kern))
+        factor = (CALIB_BYTES / 1024.0) / max(copy)      # KiB the counter should read / KiB it read
+        out[ctr] = dict(kib=sum(kern) / len(kern), factor=factor)
+    fetch = out["FETCH_SIZE"]["kib"] * out["FETCH_SIZE"]["factor"] * 1024
+    write = out["WRITE_SIZE"]["kib"] * out["WRITE_SIZE"]["factor"] * 1024
+    return fetch + write, dict(read_bytes=fetch, write_bytes=write,
+                               fetch_factor=out["FETCH_SIZE"]["factor"], write_factor=out["WRITE_SIZE"]["factor"])
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,20 +152,18 @@ def main(argv=None):
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration) and exit")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes that fill roofline.traffic")
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         return 2
 
-    if a.calib_copy:
-        import torch
-        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        print(json.dumps({"calib_copy_bytes_read": src.numel() * 4, "calib_copy_bytes_written": dst.numel() * 4}))
-        return 0
+    if a.calib_copy or a.pmc_child:
+        calib_copy()
+        if a.calib_copy:
+            return 0
 
     import poissonsolver_amd as P
 
@@ -162,6 +224,13 @@ def main(argv=None):
         "pass_gbs": {k: B[k] / (prof[k] * 1e-3) / 1e9 for k in ("x", "y", "z", "y_inv", "x_inv")},
         "wall_s_timed": wall,
     }
+    if not (a.no_traffic or a.pmc_child):
+        child = ["--pmc-child", "--no-traffic", "--no-cpu", "--no-check", "--steps", "2", "--warmup", "1",
+                 "--n", str(n), "--prec", a.prec, "--op", a.op]
+        nc = 3 if a.op == "curl" else 1
+        traffic, detail = measure_traffic(child, lambda name: is_zpass_kernel(name, nc))
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_detail"] = detail
     if not a.no_check:
         from scipy.special import erf
         lhs = [np.empty_like(b) for _ in range(cout)]

@@ -155,9 +155,11 @@ template <typename T> int upload_twiddles(ps_ctx* c, DevPlan* d)
 int plan_dim(ps_ctx* c, DevPlan* d, int L, int ncomp, const char* what)
 {
     std::string why;
-    if (!make_plan(L, max_E_for(real_bytes(c), ncomp), &d->p, &why))
-        return fail(c, PS_EUNSUPPORTED, std::string(what) + ": " + why);
-    return PS_OK;
+    if (make_plan(L, max_E_for(real_bytes(c), ncomp), &d->p, &why)) return PS_OK;
+    // a length whose radices need a larger E than the multi-component budget:
+    // use the one-component bound (correct, more register pressure)
+    if (ncomp > 1 && make_plan(L, max_E_for(real_bytes(c), 1), &d->p, &why)) return PS_OK;
+    return fail(c, PS_EUNSUPPORTED, std::string(what) + ": " + why);
 }
 
 int ensure_comp_buffers(ps_ctx* c, int ncomp)
@@ -794,7 +796,9 @@ int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int
 {
     Plan1D p;
     std::string why;
-    if (!make_plan(L, max_E_for(precision_bytes, ncomp), &p, &why)) return PS_EUNSUPPORTED;
+    if (!make_plan(L, max_E_for(precision_bytes, ncomp), &p, &why) &&
+        !(ncomp > 1 && make_plan(L, max_E_for(precision_bytes, 1), &p, &why)))
+        return PS_EUNSUPPORTED;
     if (E) *E = p.E;
     if (tp) *tp = p.TP;
     if (nstage) *nstage = p.nstage;

@@ -7,7 +7,7 @@
 #include "ps_kernels.hpp"
 
 // E values and the radices each one allows (radix must divide E).
-#define PS_FOR_EACH_E(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12) X(14) X(15) X(16) X(20) X(30) X(32)
+#define PS_FOR_EACH_E(X) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(12) X(14) X(15) X(16) X(20) X(21) X(30) X(32) X(35) X(42) X(70)
 
 namespace psamd {
 

@@ -82,7 +82,7 @@ int max_E_for(int precision_bytes, int ncomp)
 {
     if (ncomp >= 3) return 10;
     (void)precision_bytes;
-    return 32;
+    return 70;
 }
 
 static int g_lds_budget = 80 * 1024;

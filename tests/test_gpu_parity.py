@@ -84,3 +84,18 @@ def test_device_special_functions_match_reference_golden():
     # dominates, so the bound is absolute + relative.
     assert np.all(np.abs(si - g["si_y"]) <= 1e-13 * np.abs(g["si_y"]))
     assert np.all(np.abs(ji - g["ji0_y"]) <= 1e-15 + 1e-13 * np.abs(g["ji0_y"]))
+
+
+ODD_CASES = [
+    # lengths served by E = 21, 35, 42, 14, 7 and the 3-component fallback (E = 30)
+    ((21, 35, 42), (1, 1, 1), (1 / 21, 1 / 35, 1 / 42), "curl"),
+    ((21, 15, 10), (0, 0, 0), (0.1, 0.1, 0.1), "curl"),
+    ((60, 8, 8), (0, 0, 0), (0.05, 0.2, 0.2), "curl"),
+    ((14, 7, 35), (1, 1, 0), (1 / 14, 1 / 7, 0.05), "grad"),
+]
+
+
+@pytest.mark.parametrize("case", ODD_CASES, ids=lambda c: "%s-%s-%s" % (c[1], "x".join(map(str, c[0])), c[3]))
+def test_fp64_parity_odd_radices(case):
+    n, bc, dx, op = case
+    _compare(n, bc, dx, 4, op, True, "fp64", FP64_TOL, 19)
