@@ -48,7 +48,8 @@ struct Tuning {
     int twreg_z = 1;              // z pass: stage twiddles in registers (1) or loaded per stage (0)
     int twreg_y = 0;              // y passes: same choice
     int occ4_z = 0;               // z pass: 4 waves/SIMD target (1) or 2 (0)
-    int halfb_z = 1, halfb_y = 0; // tile-width divisor of the specialised strided passes: z 0/1/2 -> B, B/2, B/4; y 0/1 -> B, B/2
+    int zwave = 1;                // z pass: pencil-per-wavefront kernel where the length allows
+    int halfb_z = 0, halfb_y = 0; // tile-width divisor of the specialised strided passes: z 0/1/2 -> B, B/2, B/4; y 0/1 -> B, B/2
 };
 
 int env_int(const char* name, int dflt)
@@ -85,6 +86,8 @@ struct ps_ctx {
     DevPlan px, py, pz1, pz3;
     Tuning tune;
     int zb = 0;             // A2 layout: log2 z block (-1 = [ky][z][kx])
+    int zwave = 0;          // z pass uses the pencil-per-wavefront kernel
+    int g_layout = 0;       // Gk octant: 0 [ky][kz][kx], 1 [ky][kx][kz]
     Addr a2_ypass;          // A2 addressing seen by the y passes (slab = z, pos = ky)
     Addr a2_zpass;          // A2 addressing seen by the z pass (slab = ky, pos = z)
 
@@ -257,8 +260,10 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
     a.store_mode = STORE_COMPLEX;
     a.do_fwd = a.do_inv = 1;
     a.G = reinterpret_cast<const T*>(c->G);
-    a.g_ky = (long)c->O[0] * c->O[2];                 // octant stored [ky][kz][kx]
-    a.g_kz = c->O[0];
+    a.g_ky = (long)c->O[0] * c->O[2];
+    a.g_kx = c->g_layout ? c->O[2] : 1;
+    a.g_kz = c->g_layout ? 1 : c->O[0];
+    a.zwave = c->zwave;
     a.Mx = c->M[0];
     a.My = c->M[1];
     a.Mz = c->M[2];
@@ -384,7 +389,8 @@ template <typename T> int setup_impl(ps_ctx* c)
     tn.twreg_z = env_int("PS_TUNE_TWREG_Z", 1);
     tn.twreg_y = env_int("PS_TUNE_TWREG_Y", 0);
     tn.occ4_z = env_int("PS_TUNE_OCC4_Z", 0);
-    tn.halfb_z = env_int("PS_TUNE_HALFB_Z", 1);
+    tn.zwave = env_int("PS_TUNE_ZWAVE", 1);
+    tn.halfb_z = env_int("PS_TUNE_HALFB_Z", 0);
     tn.halfb_y = env_int("PS_TUNE_HALFB_Y", 0);
     set_lds_budget(tn.lds_kb * 1024);
     set_preferred_E(env_int("PS_TUNE_E", 16));
@@ -450,8 +456,13 @@ template <typename T> int setup_impl(ps_ctx* c)
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3})
         if ((r = upload_twiddles<T>(c, d))) return r;
 
+    // pencil-per-wavefront z kernel: specialised one-component plan, pencil
+    // inside one wavefront, two or more stages; it reads Gk along kz
+    c->zwave = tn.zwave && c->pz1.stat && (64 % c->pz1.p.TP == 0) && c->pz1.p.nstage >= 2;
+    c->g_layout = c->zwave ? 1 : 0;
     CK(hipMalloc(&c->G, (size_t)c->O[0] * c->O[1] * c->O[2] * sizeof(T)));
-    r = build_greens<T>(c->ncell, c->bc, c->dx, c->reg, reinterpret_cast<T*>(c->G), c->stream, &c->err);
+    r = build_greens<T>(c->ncell, c->bc, c->dx, c->reg, reinterpret_cast<T*>(c->G), c->g_layout, c->stream,
+                        &c->err);
     if (r) return r;
     return PS_OK;
 }

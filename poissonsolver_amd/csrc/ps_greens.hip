@@ -78,7 +78,7 @@ __device__ inline double kap(int m, int M, double dk)
 template <typename T>
 __global__ void k_finalize(T* out, const double* gsp, int mode, const double* g2d, const double* g1d, int Ox,
                            int Oy, int Oz, int Mx, int My, int Mz, double dkx, double dky, double dkz,
-                           double zarg, int reg, double invN)
+                           double zarg, int reg, double invN, int layout)
 {
     const long n = (long)Ox * Oy * Oz;
     for (long id = blockIdx.x * (long)blockDim.x + threadIdx.x; id < n; id += (long)gridDim.x * blockDim.x) {
@@ -108,8 +108,10 @@ __global__ void k_finalize(T* out, const double* gsp, int mode, const double* g2
             }
             z = sum * exp(-a);
         }
-        // stored as [ky][kz][kx]: a z pencil tile (fixed ky) reads one contiguous slab
-        out[((long)j * Oz + k) * Ox + i] = (T)(g * z * invN);
+        // layout 0: [ky][kz][kx]; layout 1: [ky][kx][kz].  Either way a z-pass
+        // tile (fixed ky) reads one contiguous slab.
+        const long o = layout ? ((long)j * Ox + i) * Oz + k : ((long)j * Oz + k) * Ox + i;
+        out[o] = (T)(g * z * invN);
     }
 }
 
@@ -194,8 +196,8 @@ static unsigned grid_for(long n)
 }  // namespace
 
 template <typename T>
-int build_greens(const int ncell[3], const int bc[3], const double dx[3], int reg, T* out, hipStream_t s,
-                 std::string* err)
+int build_greens(const int ncell[3], const int bc[3], const double dx[3], int reg, T* out, int layout,
+                 hipStream_t s, std::string* err)
 {
     int M[3], O[3];
     for (int d = 0; d < 3; ++d) {
@@ -256,15 +258,15 @@ int build_greens(const int ncell[3], const int bc[3], const double dx[3], int re
     hipLaunchKernelGGL((k_finalize<T>), dim3(grid_for(n)), dim3(256), 0, s, out, (const double*)g3.p, mode,
                        (const double*)g2.p, (const double*)g1.p, O[0], O[1], O[2], M[0], M[1], M[2],
                        1.0 / ((double)M[0] * dx[0]), 1.0 / ((double)M[1] * dx[1]), 1.0 / ((double)M[2] * dx[2]),
-                       zarg, regm, invN);
+                       zarg, regm, invN, layout);
     PS_HIP(hipGetLastError());
     PS_HIP(hipStreamSynchronize(s));
     return 0;
 }
 
-template int build_greens<float>(const int[3], const int[3], const double[3], int, float*, hipStream_t,
+template int build_greens<float>(const int[3], const int[3], const double[3], int, float*, int, hipStream_t,
                                  std::string*);
-template int build_greens<double>(const int[3], const int[3], const double[3], int, double*, hipStream_t,
+template int build_greens<double>(const int[3], const int[3], const double[3], int, double*, int, hipStream_t,
                                   std::string*);
 
 // Si and Ji0 evaluated by the device code, for tests of the special functions.

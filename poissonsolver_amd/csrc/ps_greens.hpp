@@ -14,11 +14,11 @@ constexpr int kGreensEHIP = PS_EHIP;
 constexpr int kGreensEUNSUPPORTED = PS_EUNSUPPORTED;
 
 // Ghat * zeta / (Mx My Mz) on the octant kx <= Mx/2, ky <= My/2, kz <= Mz/2,
-// stored [ky][kz][kx] (kx fastest) in `out` (device memory of that many T).
-// Returns a PS_* status.
+// stored [ky][kz][kx] (layout 0) or [ky][kx][kz] (layout 1) in `out` (device
+// memory of that many T).  Returns a PS_* status.
 template <typename T>
-int build_greens(const int ncell[3], const int bc[3], const double dx[3], int reg, T* out, hipStream_t s,
-                 std::string* err);
+int build_greens(const int ncell[3], const int bc[3], const double dx[3], int reg, T* out, int layout,
+                 hipStream_t s, std::string* err);
 
 // Device Si (which = 0) or Ji0 (which = 1) at n host arguments.
 int eval_special_device(const double* x, double* y, long n, int which, std::string* err);

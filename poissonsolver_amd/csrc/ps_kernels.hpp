@@ -55,10 +55,11 @@ template <typename T> struct PencilArgs {
     int xcd_remap;   // 1: contiguous work-id ranges per XCD
     int twreg;       // specialised kernels: 1 = stage twiddles held in registers
     int occ4;        // specialised fused kernel: target 4 waves/SIMD (else 2)
+    int zwave;       // specialised fused kernel: pencil-per-wavefront variant (k_zfused_w)
     int slab_pair;   // 1: visit slabs as 0, M/2, 1, M-1, 2, M-2, ... (z pass: ky and My-ky share a row of Gk)
     // k-space operator (z pass): slab = ky, col = kx, pos = kz
     const T* G;
-    long g_ky, g_kz;
+    long g_ky, g_kx, g_kz;
     int Mx, My, Mz;
     double dk[3];    // 1/(M_d dx_d)
     double zarg;     // 0.5 (2h)^2

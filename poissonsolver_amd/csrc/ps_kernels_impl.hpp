@@ -69,7 +69,7 @@ __device__ __forceinline__ TileK<T> tile_k(const PencilArgs<T>& a, int col, int 
     k.kyh = nyy ? T(0) : (T)k.kyf;
     k.kyn = nyy ? (T)k.kyf : T(0);
     const int kyo = (ky <= a.My - ky) ? ky : a.My - ky;
-    if (cv && a.G) k.grow = a.G + kyo * a.g_ky + kx;
+    if (cv && a.G) k.grow = a.G + kyo * a.g_ky + kx * a.g_kx;
     return k;
 }
 

@@ -176,6 +176,82 @@ __device__ __forceinline__ void sfft_from(cplx<T> (&v)[E], const STw<L, E, T>& W
     }
 }
 
+// ---------------------------------------------------------------------------
+// Stage-level pieces, for kernels that change the thread <-> pencil mapping
+// between stages (sstage: butterflies of one stage; sxchg: its LDS exchange).
+// ---------------------------------------------------------------------------
+template <int L, int E, int DIR, int S, int TWREG, typename T>
+__device__ __forceinline__ void sstage(cplx<T> (&v)[E], const STw<L, E, T>& W, const cplx<T>* __restrict__ tw,
+                                       int t)
+{
+    using C = cplx<T>;
+    constexpr CPlan cp = SFft<L, E>::cp;
+    constexpr int R = cp.r[S];
+    constexpr int Ns = cplan_ns(cp, S);
+    constexpr int NB = E / R;
+    constexpr int TP = L / E;
+    constexpr int off = cplan_twoff(cp, E, S);
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        C a[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) a[m] = v[i + m * NB];
+        if constexpr (S > 0) {
+            const int k = (t + i * TP) % Ns;
+#pragma unroll
+            for (int m = 1; m < R; ++m) {
+                C w;
+                if constexpr (TWREG) w = W[off + i * (R - 1) + m - 1];
+                else w = tw[m * k * (L / (Ns * R))];
+                if constexpr (DIR > 0) w.y = -w.y;
+                a[m] = cmul(a[m], w);
+            }
+        }
+        dft<R, DIR>(a);
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[i + q * NB] = a[q];
+    }
+}
+
+// Write the outputs of stage S held by slot tw_ into the pencil at lw (unit
+// position stride), synchronise (WS = 1: wavefront, 0: workgroup), then read
+// positions tr + TP u of the pencil at lr.  With WS = 1 the caller guarantees
+// that only this wavefront touches lw and lr.
+template <int L, int E, int S, int WS, typename T>
+__device__ __forceinline__ void sxchg(cplx<T> (&v)[E], cplx<T>* lw, int tw_, const cplx<T>* lr, int tr)
+{
+    using C = cplx<T>;
+    constexpr CPlan cp = SFft<L, E>::cp;
+    constexpr int R = cp.r[S];
+    constexpr int Ns = cplan_ns(cp, S);
+    constexpr int NB = E / R;
+    constexpr int TP = L / E;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+        const int b = tw_ + i * TP;
+        const int k = b % Ns;
+        const int base = (b - k) * R + k;
+        if constexpr (Ns % 32 == 0) {
+            C* w0 = lw + lds_pos(base);
+#pragma unroll
+            for (int q = 0; q < R; ++q) w0[q * (Ns + Ns / 32)] = v[i + q * NB];
+        } else {
+#pragma unroll
+            for (int q = 0; q < R; ++q) lw[lds_pos(base + q * Ns)] = v[i + q * NB];
+        }
+    }
+    xsync<WS>();
+    if constexpr (TP % 32 == 0) {
+        const C* r0 = lr + lds_pos(tr);
+#pragma unroll
+        for (int u = 0; u < E; ++u) v[u] = r0[u * (TP + TP / 32)];
+    } else {
+#pragma unroll
+        for (int u = 0; u < E; ++u) v[u] = lr[lds_pos(tr + TP * u)];
+    }
+    if constexpr (WS) wave_sync();
+}
+
 // Transform one pencil held as v[u] = x[t + TP u]; on return v[u] = X[t + TP u].
 // `lds` points at this pencil's first position; positions are LS elements apart.
 // WS = 1: the pencil's TP threads are lanes of one wavefront, so exchanges

@@ -229,6 +229,123 @@ __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------
+// Fused z pass with two thread mappings (one-component operators).
+//   coalesced mapping  tid = t*B + p : global loads/stores of B adjacent columns
+//   wavefront mapping  tid = p*TP + t: each pencil inside one wavefront
+// The first forward stage runs in the coalesced mapping on the loaded data;
+// its exchange (one workgroup barrier) hands each pencil to its wavefront;
+// the remaining forward stages, the k-space operator and all inverse stages
+// but the last exchange through the pencil's own LDS region with wavefront
+// ordering only; the last exchange (second barrier) returns the coalesced
+// mapping for the final inverse stage and the store.
+// ---------------------------------------------------------------------------
+template <int L, int E, int DIR, int S, int SEND, typename T>
+__device__ __forceinline__ void wstages(cplx<T> (&v)[E], const STw<L, E, T>& W, cplx<T>* reg, int t)
+{
+    if constexpr (S < SEND) {
+        sstage<L, E, DIR, S, 1, T>(v, W, nullptr, t);
+        if constexpr (S + 1 < SEND) sxchg<L, E, S, 1, T>(v, reg, t, reg, t);
+        wstages<L, E, DIR, S + 1, SEND, T>(v, W, reg, t);
+    }
+}
+
+template <typename T, int L, int B, int HIN, int HOUT>
+__global__ void __launch_bounds__(kMaxThreads, 2) k_zfused_w(PencilArgs<T> a)
+{
+    using C = cplx<T>;
+    constexpr int E = 16, TP = L / E;
+    constexpr int NS = SFft<L, E>::cp.n;
+    static_assert(NS >= 2, "two or more stages");
+    static_assert(64 % TP == 0, "each pencil inside one wavefront");
+    constexpr int LP = lds_len(L);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+
+    const int tid = threadIdx.x;
+    const int tl = tid / B, pl = tid % B;
+    const int pc = tid / TP, tc = tid % TP;
+    const long nblk = gridDim.x;
+    const long lin = a.xcd_remap ? xcd_remap(blockIdx.x, nblk) : (long)blockIdx.x;
+    const long ntile = (long)a.ncoltile * a.nslab;
+    const int comp = (int)(lin / ntile);
+    const long rem = lin - (long)comp * ntile;
+    const int slab0 = (int)(rem / a.ncoltile);
+    const int ct = (int)(rem - (long)slab0 * a.ncoltile);
+    const int slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    const int col0 = ct * B;
+    const bool full = col0 + B <= a.ncol;
+    const bool cvl = col0 + pl < a.ncol, cvc = col0 + pc < a.ncol;
+
+    STw<L, E, T> W;
+    load_stwiddles<L, E, T>(W, tc, a.plan.tw);
+
+    const long iu = (a.ia.pos_sh >= 30) ? TP * a.ia.pos_lo : (long)(TP >> a.ia.pos_sh) * a.ia.pos_hi;
+    const long ou = (a.oa.pos_sh >= 30) ? TP * a.oa.pos_lo : (long)(TP >> a.oa.pos_sh) * a.oa.pos_hi;
+
+    C v[E];
+    {
+        const C* ip = reinterpret_cast<const C*>(a.in) + comp * a.ia.comp + (col0 + pl) * a.ia.col +
+                      boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi) +
+                      boff(tl, a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi);
+        const bool ok = full || cvl;
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            C x = C{0, 0};
+            if (!(HIN && u >= E / 2) && ok && (HIN || tl + TP * u < a.n_in)) x = ip[u * iu];
+            v[u] = x;
+        }
+    }
+    sstage<L, E, -1, 0, 1, T>(v, W, nullptr, tl);
+    sxchg<L, E, 0, 0, T>(v, lds + pl * LP, tl, lds + pc * LP, tc);
+    C* reg = lds + pc * LP;
+    wstages<L, E, -1, 1, NS, T>(v, W, reg, tc);
+
+    const TileK<T> tk = tile_k(a, col0 + pc, slab, cvc);
+    kspace_op_inplace<T, E>(v, a, tk, comp, tc, TP);
+
+    wstages<L, E, +1, 0, NS - 1, T>(v, W, reg, tc);
+    sxchg<L, E, NS - 2, 0, T>(v, reg, tc, lds + pl * LP, tl);
+    sstage<L, E, +1, NS - 1, 0, T>(v, W, a.plan.tw, tl);
+
+    if (full || cvl) {
+        C* op_ = reinterpret_cast<C*>(a.out) + comp * a.oa.comp + (col0 + pl) * a.oa.col +
+                 boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi) + boff(tl, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
+#pragma unroll
+        for (int u = 0; u < (HOUT ? E / 2 : E); ++u)
+            if (HOUT || tl + TP * u < a.n_out) op_[u * ou] = v[u];
+    }
+}
+
+template <typename T, int L, int B>
+hipError_t s_launch_zwave_b(const PencilArgs<T>& a, int half, hipStream_t s)
+{
+    const size_t shm = (size_t)B * lds_len(L) * sizeof(cplx<T>);
+    const long nwork = (long)a.ncoltile * a.nslab * a.ncomp;
+    constexpr int TP = L / 16;
+    hipError_t e;
+    if (half) {
+        if ((e = allow_lds(k_zfused_w<T, L, B, 1, 1>, shm)) != hipSuccess) return e;
+        hipLaunchKernelGGL((k_zfused_w<T, L, B, 1, 1>), dim3((unsigned)nwork), dim3(TP * B), shm, s, a);
+    } else {
+        if ((e = allow_lds(k_zfused_w<T, L, B, 0, 0>, shm)) != hipSuccess) return e;
+        hipLaunchKernelGGL((k_zfused_w<T, L, B, 0, 0>), dim3((unsigned)nwork), dim3(TP * B), shm, s, a);
+    }
+    return hipGetLastError();
+}
+
+template <typename T, int L> hipError_t s_launch_zwave(const PencilArgs<T>& a, int half, hipStream_t s)
+{
+    using Pol = SPolicy<T, L>;
+    if constexpr (64 % (L / 16) == 0 && SFft<L, 16>::cp.n >= 2) {
+        if (a.B == Pol::B) return s_launch_zwave_b<T, L, Pol::B>(a, half, s);
+        if constexpr (Pol::B >= 2) {
+            if (a.B == Pol::B / 2) return s_launch_zwave_b<T, L, Pol::B / 2>(a, half, s);
+        }
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
 template <typename T, int L> hipError_t s_launch_x(const XArgs<T>& a, int inverse, int half, hipStream_t s)
@@ -315,6 +432,7 @@ hipError_t s_launch_pencil(const PencilArgs<T>& a, int nc, int mode, int half, h
         if (nc != 1) return hipErrorInvalidValue;
         return half ? s_launch_pencil_one<T, L, 1, SM_INV, 0, 1>(a, s) : s_launch_pencil_one<T, L, 1, SM_INV, 0, 0>(a, s);
     }
+    if (nc == 1 && a.zwave) return s_launch_zwave<T, L>(a, half, s);
     if (nc == 1)
         return half ? s_launch_pencil_one<T, L, 1, SM_FUSED, 1, 1>(a, s) : s_launch_pencil_one<T, L, 1, SM_FUSED, 0, 0>(a, s);
     return half ? s_launch_pencil_one<T, L, 3, SM_FUSED, 1, 1>(a, s) : s_launch_pencil_one<T, L, 3, SM_FUSED, 0, 0>(a, s);

@@ -22,6 +22,8 @@ VARIANTS = [
     {"PS_TUNE_TWREG_Z": "1", "PS_TUNE_TWREG_Y": "1"},
     {"PS_TUNE_OCC4_Z": "1"},
     {"PS_TUNE_HALFB_Z": "1", "PS_TUNE_HALFB_Y": "1"},
+    {"PS_TUNE_ZWAVE": "0"},
+    {"PS_TUNE_ZWAVE": "1", "PS_TUNE_HALFB_Z": "0"},
 ]
 
 
