@@ -8,6 +8,10 @@ region).  Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--prec fp32|mixed|fp64]
                   [--n 512] [--op plain|curl] [--no-cpu] [--no-check]
+  python bench.py --workload batch [--nprob 64] [--autotune]
+      64 independent unbounded scalar problems, n from {32,...,192} (seed 4),
+      sources = sums of 1-4 Gaussian blobs; one step = one hipGraph launch
+      solving the whole batch.
 """
 from __future__ import annotations
 
@@ -140,6 +144,107 @@ def measure_traffic(argv_child, kernel_key, timeout=240):
                                fetch_factor=out["FETCH_SIZE"]["factor"], write_factor=out["WRITE_SIZE"]["factor"])
 
 
+BATCH_SIZES = (32, 48, 64, 96, 128, 160, 192)
+
+
+def batch_problems(nprob, seed=4):
+    """Uneven batch: cubic grids on [-1,1]^3 with n drawn from BATCH_SIZES, each
+    source a sum of 1-4 Gaussian blobs (centres in [-0.5,0.5]^3, widths in
+    [0.05,0.15], amplitudes in [-1,1])."""
+    rng = np.random.default_rng(seed)
+    probs, srcs = [], []
+    for _ in range(nprob):
+        n = int(rng.choice(BATCH_SIZES))
+        h = 2.0 / n
+        x = -1.0 + (np.arange(n) + 0.5) * h
+        k = int(rng.integers(1, 5))
+        src = np.zeros((n, n, n))
+        for _ in range(k):
+            c = rng.uniform(-0.5, 0.5, 3)
+            w = rng.uniform(0.05, 0.15)
+            amp = rng.uniform(-1.0, 1.0)
+            gz = np.exp(-(x - c[2]) ** 2 / (2 * w * w))
+            gy = np.exp(-(x - c[1]) ** 2 / (2 * w * w))
+            gx = np.exp(-(x - c[0]) ** 2 / (2 * w * w))
+            src += amp * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+        probs.append(((n, n, n), (0, 0, 0), (h, h, h), 2))
+        srcs.append(src)
+    return probs, srcs
+
+
+def bench_batch(a):
+    import poissonsolver_amd as P
+
+    probs, srcs = batch_problems(a.nprob)
+    prec = {"fp32": P.PS_FP32, "mixed": P.PS_MIXED, "fp64": P.PS_FP64}[a.prec]
+    t0 = time.perf_counter()
+    b = P.batch(probs, prec)
+    setup_s = time.perf_counter() - t0
+    tune_s = 0.0
+    if a.autotune:
+        t0 = time.perf_counter()
+        b.autotune(iters=3)
+        tune_s = time.perf_counter() - t0
+    for i, src in enumerate(srcs):
+        b.push(i, src)
+    for _ in range(max(a.warmup, 2)):        # eager pass, then capture + launch
+        b.solve()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        b.solve()                            # one graph launch + wait
+    wall = time.perf_counter() - t0
+    ms = 1000.0 * wall / a.steps
+    gpu_ms = b.profile(a.steps)              # graph launches only, HIP events
+    eb = 8 if a.prec == "fp64" else 4
+    alg = sum(alg_bytes(p[0][0], 1, 1, eb)["solve"] for p in probs)
+    out = {
+        "metric": METRIC,
+        "value": a.nprob * 1000.0 / ms,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": max(a.warmup, 2),
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if a.prec == "fp64" else "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "batch of %d independent fully unbounded scalar solves, n^3 with n drawn (seed 4) from "
+                        "%s, reg order 2, Gaussian-blob sources, %s, one hipGraph launch per step"
+                        % (a.nprob, list(BATCH_SIZES), a.prec),
+            "sizes": [p[0][0] for p in probs], "autotune": bool(a.autotune), "precision": a.prec,
+        },
+        "gpu_ms_per_batch": gpu_ms,
+        "alg_bytes_per_batch": alg,
+        "batch_hbm_gbs": alg / (gpu_ms * 1e-3) / 1e9,
+        "setup_s": setup_s, "autotune_s": tune_s,
+    }
+    if not a.no_check:
+        errs = []
+        for i, (p, src) in enumerate(zip(probs, srcs)):
+            n = p[0]
+            got = np.empty((n[2], n[1], n[0]))
+            b.pull(i, got)
+            if a.prec == "fp64":
+                continue
+            r = P.poisson_solver(P.PS_FP64)
+            r.setup3d(*p)
+            r.push(src)
+            r.solve3d()
+            ref = np.empty_like(got)
+            r.pull(ref)
+            r.close()
+            errs.append(float(np.sqrt(np.sum((got - ref) ** 2) / np.sum(ref ** 2))))
+        if errs:
+            out["rel_l2_vs_fp64_gpu"] = {"max": max(errs), "median": float(np.median(errs)),
+                                         "per_problem": errs}
+    b.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +260,9 @@ def main(argv=None):
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes that fill roofline.traffic")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--workload", default="single", choices=["single", "batch"])
+    ap.add_argument("--nprob", type=int, default=64)
+    ap.add_argument("--autotune", action="store_true", help="batch: autotune each distinct size first")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
@@ -164,6 +272,8 @@ def main(argv=None):
         calib_copy()
         if a.calib_copy:
             return 0
+    if a.workload == "batch":
+        return bench_batch(a)
 
     import poissonsolver_amd as P
 

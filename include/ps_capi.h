@@ -98,6 +98,68 @@ int ps_profile_solve(ps_ctx* ctx, int iters, double ms[6]);
 int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int* nstage, int radix[14]);
 /* evaluate the device Si (which = 0) or Ji0 (which = 1) at n host arguments */
 int ps_eval_special(const double* x, double* y, long n, int which);
+/* enqueue ps_solve3d on the context's stream and return without waiting */
+int ps_solve3d_async(ps_ctx* ctx);
+
+/* --- extension: kernel tuning and the autotuner (no reference counterpart) ---
+   Shapes and layouts the setup picks for the pass kernels.  Every field has a
+   planner default; the PS_TUNE_* environment variables of the same names
+   override the defaults.  No setting changes results beyond round-off. */
+typedef struct ps_tuning {
+    int zblk;         /* z blocking of the y/z-transposed spectrum: log2 block > 0,
+                         0 = [z][ky][kx], -1 = [ky][z][kx], -2 = planner (PS_TUNE_ZBLK) */
+    int tile_x;       /* row pairs per workgroup, x passes; 0 = planner (PS_TUNE_PX) */
+    int tile_y;       /* columns per workgroup, y passes; 0 = planner (PS_TUNE_BY) */
+    int tile_z;       /* columns per workgroup, z pass; 0 = planner (PS_TUNE_BZ) */
+    int radix_e;      /* radix split: preferred elements per thread of the generic
+                         kernels (PS_TUNE_E, default 16) */
+    int lds_kb;       /* LDS budget per workgroup for the planner, KB (PS_TUNE_LDS_KB, 80) */
+    int xcd;          /* XCD-contiguous work ids in the strided passes (PS_TUNE_XCD, 1) */
+    int specialised;  /* compile-time specialised kernels where the length has one (PS_TUNE_STATIC, 1) */
+    int specialised3; /* ... also for the 3-component z pass (PS_TUNE_STATIC3, 1) */
+    int twreg_z;      /* z pass keeps stage twiddles in registers (PS_TUNE_TWREG_Z, 1) */
+    int twreg_y;      /* y passes keep stage twiddles in registers (PS_TUNE_TWREG_Y, 0) */
+    int occ4_z;       /* z pass targets 4 waves per SIMD instead of 2 (PS_TUNE_OCC4_Z, 0) */
+    int zwave;        /* pencil-per-wavefront z kernel where the length allows (PS_TUNE_ZWAVE, 1) */
+    int halfb_z;      /* z tile-width divisor 0/1/2 -> B, B/2, B/4 (PS_TUNE_HALFB_Z, 0) */
+    int halfb_y;      /* y tile-width divisor 0/1 -> B, B/2 (PS_TUNE_HALFB_Y, 0) */
+} ps_tuning;
+/* the defaults (with PS_TUNE_* overrides applied) */
+int ps_tuning_default(ps_tuning* t);
+/* tuning used by the next ps_setup3d on this context; NULL returns to the defaults */
+int ps_set_tuning(ps_ctx* ctx, const ps_tuning* t);
+/* tuning the last ps_setup3d used */
+int ps_get_tuning(const ps_ctx* ctx, ps_tuning* t);
+/* time candidate tunings on this context's grid, options and pushed components
+   (X only when none is pushed), by coordinate descent from the current tuning
+   over layout, tile, radix-split and kernel-variant choices, `iters` solves per
+   candidate; leaves the context set up with the fastest (pushed data and
+   solutions are discarded: push again).  best / best_ms may be NULL. */
+int ps_autotune(ps_ctx* ctx, int iters, ps_tuning* best, double* best_ms);
+
+/* --- extension: batches of independent solves of uneven size ---------------
+   One context per problem (own plans, tiles and Green's kernel); ps_batch_solve
+   replays every problem's passes as one captured hipGraph launch, the problems
+   running on parallel branches of the graph.  Options apply to all problems. */
+typedef struct ps_batch ps_batch;
+/* ncell, bc, dx: nprob x 3; reg_order: nprob entries */
+int ps_batch_create(ps_batch** out, int nprob, const int* ncell, const int* bc, const double* dx,
+                    const int* reg_order, int precision);
+/* autotune every distinct (ncell, bc) once with the batch's options and apply the
+   result to all problems of that shape; pushed data is discarded (push again) */
+int ps_batch_autotune(ps_batch* b, int iters);
+int ps_batch_destroy(ps_batch* b);
+int ps_batch_set_ops(ps_batch* b, int grad, int div, int curl, int reproject);
+int ps_batch_context(ps_batch* b, int prob, ps_ctx** ctx);
+int ps_batch_push(ps_batch* b, int prob, const double* bx, const double* by, const double* bz);
+int ps_batch_push_device(ps_batch* b, int prob, const void* bx, const void* by, const void* bz);
+/* solve every problem: one graph launch (captured on the first call after a push
+   pattern or option change), then wait */
+int ps_batch_solve(ps_batch* b);
+int ps_batch_pull(ps_batch* b, int prob, double* ax, double* ay, double* az, double* bx, double* by, double* bz);
+/* run `iters` batch solves, return the average wall time of one graph launch in ms */
+int ps_batch_profile(ps_batch* b, int iters, double* ms);
+const char* ps_batch_last_error(const ps_batch* b);
 
 #ifdef __cplusplus
 }

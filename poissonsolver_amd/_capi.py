@@ -17,6 +17,20 @@ PS_FP64, PS_FP32, PS_MIXED = 0, 1, 2
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)
 
+TUNING_FIELDS = ("zblk", "tile_x", "tile_y", "tile_z", "radix_e", "lds_kb", "xcd", "specialised",
+                 "specialised3", "twreg_z", "twreg_y", "occ4_z", "zwave", "halfb_z", "halfb_y")
+
+
+class ps_tuning(ctypes.Structure):
+    """struct ps_tuning of ps_capi.h."""
+    _fields_ = [(f, ctypes.c_int) for f in TUNING_FIELDS]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f in TUNING_FIELDS}
+
+
+_tun_p = ctypes.POINTER(ps_tuning)
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     "ps_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
@@ -45,6 +59,23 @@ SIGNATURES = {
     "ps_plan_describe": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                                         _c_int_p, _c_int_p]),
     "ps_eval_special": (ctypes.c_int, [_c_dbl_p, _c_dbl_p, ctypes.c_long, ctypes.c_int]),
+    "ps_solve3d_async": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_tuning_default": (ctypes.c_int, [_tun_p]),
+    "ps_set_tuning": (ctypes.c_int, [ctypes.c_void_p, _tun_p]),
+    "ps_get_tuning": (ctypes.c_int, [ctypes.c_void_p, _tun_p]),
+    "ps_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _tun_p, _c_dbl_p]),
+    "ps_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p,
+                                       _c_int_p, ctypes.c_int]),
+    "ps_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_batch_set_ops": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_int] * 4),
+    "ps_batch_context": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "ps_batch_push": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3),
+    "ps_batch_push_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 3),
+    "ps_batch_solve": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_batch_pull": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6),
+    "ps_batch_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p]),
+    "ps_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 
 _LIB = None

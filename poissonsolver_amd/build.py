@@ -55,7 +55,7 @@ def sources():
         units.append(("kern_e.hip", ["-DPS_E=%d" % e], "kern_e%d.o" % e))
     for L in sl_values():
         units.append(("skern.hip", ["-DPS_SL=%d" % L], "skern_%d.o" % L))
-    for f in ("ps_greens.hip", "ps_dispatch.cpp", "ps_plan.cpp", "ps_capi.cpp", "poisson_solver.cpp"):
+    for f in ("ps_greens.hip", "ps_dispatch.cpp", "ps_plan.cpp", "ps_capi.cpp", "ps_batch.cpp", "poisson_solver.cpp"):
         units.append((f, [], os.path.splitext(f)[0] + ".o"))
     return units
 

@@ -36,27 +36,35 @@ struct DevPlan {
     bool stat = false;       // compile-time specialised kernels for this length
 };
 
-// Tile shapes and the z blocking of A2.  Defaults come from the planner;
-// PS_TUNE_* environment variables override them (read at setup3d).
-struct Tuning {
-    int By = 0, Bz = 0, Px = 0;   // 0: planner's choice
-    int zblk = -2;                // log2 z block of A2; -1: [ky][z][kx]; 0: [z][ky][kx]; -2: planner
-    int lds_kb = 80;              // LDS budget per workgroup
-    int xcd = 1;                  // XCD-contiguous work ids in strided passes
-    int stat = 1;                 // use the specialised kernels where they exist
-    int stat3 = 1;                // ... also for the 3-component z pass
-    int twreg_z = 1;              // z pass: stage twiddles in registers (1) or loaded per stage (0)
-    int twreg_y = 0;              // y passes: same choice
-    int occ4_z = 0;               // z pass: 4 waves/SIMD target (1) or 2 (0)
-    int zwave = 1;                // z pass: pencil-per-wavefront kernel where the length allows
-    int halfb_z = 0, halfb_y = 0; // tile-width divisor of the specialised strided passes: z 0/1/2 -> B, B/2, B/4; y 0/1 -> B, B/2
-};
+// Tile shapes, layouts and kernel variants (ps_tuning in ps_capi.h).
+using Tuning = ps_tuning;
 
 int env_int(const char* name, int dflt)
 {
     const char* v = getenv(name);
     if (!v || !*v) return dflt;
     return atoi(v);
+}
+
+Tuning tuning_default()
+{
+    Tuning t;
+    t.zblk = env_int("PS_TUNE_ZBLK", -2);
+    t.tile_x = env_int("PS_TUNE_PX", 0);
+    t.tile_y = env_int("PS_TUNE_BY", 0);
+    t.tile_z = env_int("PS_TUNE_BZ", 0);
+    t.radix_e = env_int("PS_TUNE_E", 16);
+    t.lds_kb = env_int("PS_TUNE_LDS_KB", 80);
+    t.xcd = env_int("PS_TUNE_XCD", 1);
+    t.specialised = env_int("PS_TUNE_STATIC", 1);
+    t.specialised3 = env_int("PS_TUNE_STATIC3", 1);
+    t.twreg_z = env_int("PS_TUNE_TWREG_Z", 1);
+    t.twreg_y = env_int("PS_TUNE_TWREG_Y", 0);
+    t.occ4_z = env_int("PS_TUNE_OCC4_Z", 0);
+    t.zwave = env_int("PS_TUNE_ZWAVE", 1);
+    t.halfb_z = env_int("PS_TUNE_HALFB_Z", 0);
+    t.halfb_y = env_int("PS_TUNE_HALFB_Y", 0);
+    return t;
 }
 
 }  // namespace
@@ -84,7 +92,9 @@ struct ps_ctx {
     void* G = nullptr;      // octant of Ghat*zeta/N, real
 
     DevPlan px, py, pz1, pz3;
-    Tuning tune;
+    Tuning tune = {};        // used by the last setup
+    Tuning tune_req = {};    // set by ps_set_tuning
+    bool tune_set = false;   // tune_req replaces the defaults
     int zb = 0;             // A2 layout: log2 z block (-1 = [ky][z][kx])
     int zwave = 0;          // z pass uses the pencil-per-wavefront kernel
     int g_layout = 0;       // Gk octant: 0 [ky][kz][kx], 1 [ky][kx][kz]
@@ -158,10 +168,11 @@ template <typename T> int upload_twiddles(ps_ctx* c, DevPlan* d)
 int plan_dim(ps_ctx* c, DevPlan* d, int L, int ncomp, const char* what)
 {
     std::string why;
-    if (make_plan(L, max_E_for(real_bytes(c), ncomp), &d->p, &why)) return PS_OK;
+    const int pref = c->tune.radix_e;
+    if (make_plan(L, max_E_for(real_bytes(c), ncomp), &d->p, &why, pref)) return PS_OK;
     // a length whose radices need a larger E than the multi-component budget:
     // use the one-component bound (correct, more register pressure)
-    if (ncomp > 1 && make_plan(L, max_E_for(real_bytes(c), 1), &d->p, &why)) return PS_OK;
+    if (ncomp > 1 && make_plan(L, max_E_for(real_bytes(c), 1), &d->p, &why, pref)) return PS_OK;
     return fail(c, PS_EUNSUPPORTED, std::string(what) + ": " + why);
 }
 
@@ -378,22 +389,8 @@ template <typename T> int setup_impl(ps_ctx* c)
     c->A1c = (long)nz * ny * c->Hp;
 
     Tuning& tn = c->tune;
-    tn.By = env_int("PS_TUNE_BY", 0);
-    tn.Bz = env_int("PS_TUNE_BZ", 0);
-    tn.Px = env_int("PS_TUNE_PX", 0);
-    tn.zblk = env_int("PS_TUNE_ZBLK", -2);
-    tn.lds_kb = env_int("PS_TUNE_LDS_KB", 80);
-    tn.xcd = env_int("PS_TUNE_XCD", 1);
-    tn.stat = env_int("PS_TUNE_STATIC", 1);
-    tn.stat3 = env_int("PS_TUNE_STATIC3", 1);
-    tn.twreg_z = env_int("PS_TUNE_TWREG_Z", 1);
-    tn.twreg_y = env_int("PS_TUNE_TWREG_Y", 0);
-    tn.occ4_z = env_int("PS_TUNE_OCC4_Z", 0);
-    tn.zwave = env_int("PS_TUNE_ZWAVE", 1);
-    tn.halfb_z = env_int("PS_TUNE_HALFB_Z", 0);
-    tn.halfb_y = env_int("PS_TUNE_HALFB_Y", 0);
-    set_lds_budget(tn.lds_kb * 1024);
-    set_preferred_E(env_int("PS_TUNE_E", 16));
+    tn = c->tune_set ? c->tune_req : tuning_default();
+    const int lds_bytes = tn.lds_kb * 1024;
 
     // A2 layout: z blocked by Zb = 2^zb planes, [z/Zb][ky][z%Zb][kx]; zb = 0 is
     // [z][ky][kx] (y pass writes contiguous planes, z pencils strided by a
@@ -426,14 +423,14 @@ template <typename T> int setup_impl(ps_ctx* c)
     if ((r = plan_dim(c, &c->pz1, c->M[2], 1, "z transform"))) return r;
     if ((r = plan_dim(c, &c->pz3, c->M[2], 3, "z transform (3 components)"))) return r;
     const int eb = real_bytes(c);
-    c->px.tile = tn.Px > 0 ? tn.Px : choose_P(c->px.p, eb, (ny * nz + 1) / 2);
-    c->py.tile = tn.By > 0 ? tn.By : choose_B(c->py.p, eb, c->H);
-    c->pz1.tile = tn.Bz > 0 ? tn.Bz : choose_B(c->pz1.p, eb, c->H);
-    c->pz3.tile = choose_B(c->pz3.p, eb, c->H);
+    c->px.tile = tn.tile_x > 0 ? tn.tile_x : choose_P(c->px.p, eb, (ny * nz + 1) / 2, lds_bytes);
+    c->py.tile = tn.tile_y > 0 ? tn.tile_y : choose_B(c->py.p, eb, c->H, lds_bytes);
+    c->pz1.tile = tn.tile_z > 0 ? tn.tile_z : choose_B(c->pz1.p, eb, c->H, lds_bytes);
+    c->pz3.tile = choose_B(c->pz3.p, eb, c->H, lds_bytes);
     // specialised kernels: fixed E and tile shape per length
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) {
         int E = 0, B = 0, P = 0;
-        const bool allowed = tn.stat && (d != &c->pz3 || tn.stat3);
+        const bool allowed = tn.specialised && (d != &c->pz3 || tn.specialised3);
         if (allowed && static_policy(d->p.L, eb, d == &c->pz3 ? 3 : 1, &E, &B, &P)) {
             d->stat = true;
             d->p.E = E;
@@ -450,9 +447,12 @@ template <typename T> int setup_impl(ps_ctx* c)
         for (DevPlan* d : {&c->pz1, &c->pz3})
             if (d->stat && (d->p.TP % (1 << c->zb)) != 0) d->stat = false;
     }
-    for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3})
-        if (d->p.TP * d->tile > kMaxThreads)
+    for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) {
+        if (d->tile < 1 || d->p.TP * d->tile > kMaxThreads)
             return fail(c, PS_EINVAL, "tile shape exceeds the workgroup size bound");
+        if (!d->stat && (size_t)d->tile * lds_len(d->p.L) * 2 * eb > 160 * 1024)
+            return fail(c, PS_EINVAL, "tile shape exceeds the LDS of a compute unit");
+    }
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3})
         if ((r = upload_twiddles<T>(c, d))) return r;
 
@@ -467,6 +467,18 @@ template <typename T> int setup_impl(ps_ctx* c)
     return PS_OK;
 }
 
+// Host bookkeeping of a solve: which lhs components it wrote and which
+// filter the rhs returned by pull carries.
+void after_solve(ps_ctx* c, const OpSel& s)
+{
+    for (int k = 0; k < s.nout; ++k) c->lhs_present[s.out_first + k * s.out_step] = true;
+    c->solved_after_push = true;
+    c->rhsf_valid = false;
+    const bool all3 = c->rhs_present[0] && c->rhs_present[1] && c->rhs_present[2];
+    c->solve_reproj = c->reproj && all3;                          // solve3d.cpp:388
+    c->solve_filter_active = (c->reg > 0) || c->solve_reproj;
+}
+
 template <typename T> int solve_impl(ps_ctx* c, hipEvent_t* ev)
 {
     OpSel s;
@@ -476,12 +488,7 @@ template <typename T> int solve_impl(ps_ctx* c, hipEvent_t* ev)
     r = run_pipeline<T>(c, c->rhs, s.in_first, s.in_step, s.nin, s.op, s.nc, c->lhs, s.out_first, s.out_step,
                         s.nout, ev);
     if (r) return r;
-    for (int k = 0; k < s.nout; ++k) c->lhs_present[s.out_first + k * s.out_step] = true;
-    c->solved_after_push = true;
-    c->rhsf_valid = false;
-    const bool all3 = c->rhs_present[0] && c->rhs_present[1] && c->rhs_present[2];
-    c->solve_reproj = c->reproj && all3;                          // solve3d.cpp:388
-    c->solve_filter_active = (c->reg > 0) || c->solve_reproj;
+    after_solve(c, s);
     return PS_OK;
 }
 
@@ -683,6 +690,14 @@ int ps_solve3d(ps_ctx* c)
     return PS_OK;
 }
 
+int ps_solve3d_async(ps_ctx* c)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->rhs) return fail(c, PS_ESTATE, "[poisson_solver.solve3d]: no rhs field has been pushed");
+    return (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
+}
+
 int ps_solve2d(ps_ctx* c) { return fail(c, PS_EUNSUPPORTED, "solve2d: the 2D solver is not part of this library"); }
 
 int ps_pull(ps_ctx* c, double* ax, double* ay, double* az, double* bx, double* by, double* bz)
@@ -826,6 +841,143 @@ int ps_eval_special(const double* x, double* y, long n, int which)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return PS_EHIP;
     std::string err;
     return eval_special_device(x, y, n, which, &err);
+}
+
+
+}  // extern "C"
+
+namespace psamd {
+// State update of a solve whose launches were replayed from a captured graph.
+int ctx_mark_solved(ps_ctx* c)
+{
+    OpSel s;
+    int r = check_ready(c);
+    if (r || (r = select_op(c, &s))) return r;
+    after_solve(c, s);
+    return PS_OK;
+}
+// Bit mask of the pushed rhs components.
+int ctx_rhs_mask(const ps_ctx* c)
+{
+    return c ? (c->rhs_present[0] ? 1 : 0) | (c->rhs_present[1] ? 2 : 0) | (c->rhs_present[2] ? 4 : 0) : 0;
+}
+}  // namespace psamd
+
+extern "C" {
+
+int ps_tuning_default(ps_tuning* t)
+{
+    if (!t) return PS_EINVAL;
+    *t = tuning_default();
+    return PS_OK;
+}
+
+int ps_set_tuning(ps_ctx* c, const ps_tuning* t)
+{
+    if (!c) return PS_EINVAL;
+    if (!t) {
+        c->tune_set = false;
+        return PS_OK;
+    }
+    if (t->zblk < -2 || t->zblk > 10 || t->tile_x < 0 || t->tile_x > 64 || t->tile_y < 0 || t->tile_y > 64 ||
+        t->tile_z < 0 || t->tile_z > 64 || t->radix_e < 0 || t->radix_e > 70 || t->lds_kb < 16 || t->lds_kb > 160 ||
+        t->halfb_z < 0 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1)
+        return fail(c, PS_EINVAL, "set_tuning: field out of range");
+    c->tune_req = *t;
+    c->tune_set = true;
+    return PS_OK;
+}
+
+int ps_get_tuning(const ps_ctx* c, ps_tuning* t)
+{
+    if (!c || !t) return PS_EINVAL;
+    *t = c->ready ? c->tune : (c->tune_set ? c->tune_req : tuning_default());
+    return PS_OK;
+}
+
+// Coordinate descent over the tuning fields: each field in turn takes the
+// candidate value with the lowest average solve time (a change must win by
+// 1% to count, so timing noise does not move it), the others held.  Fields
+// that a setup ignores (tiles and radix split of specialised passes) are
+// skipped.
+int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (iters < 1) return fail(c, PS_EINVAL, "autotune: iters must be >= 1");
+    int ncell[3], bc[3];
+    double dx[3];
+    for (int d = 0; d < 3; ++d) {
+        ncell[d] = c->ncell[d];
+        bc[d] = c->bc[d];
+        dx[d] = c->dx[d];
+    }
+    const int reg = c->reg;
+    bool present[3] = {c->rhs_present[0], c->rhs_present[1], c->rhs_present[2]};
+    if (c->grad || c->div || c->curl) present[0] = present[1] = present[2] = true;
+    if (!present[0] && !present[1] && !present[2]) present[0] = true;
+
+    auto trial = [&](const Tuning& t, double* ms) -> int {
+        c->tune_req = t;
+        c->tune_set = true;
+        int st = ps_setup3d(c, ncell, bc, dx, reg);
+        if (st) return st;
+        if ((st = ensure_real(c, &c->rhs))) return st;
+        CK(hipMemset(c->rhs, 0, 3 * (size_t)c->n3 * real_bytes(c)));
+        for (int d = 0; d < 3; ++d) c->rhs_present[d] = present[d];
+        double t6[6];
+        if ((st = ps_profile_solve(c, 1, t6))) return st;    // warm-up
+        if ((st = ps_profile_solve(c, iters, t6))) return st;
+        *ms = t6[5];
+        return PS_OK;
+    };
+
+    Tuning best = c->tune;
+    double best_ms = 0.0;
+    if ((r = trial(best, &best_ms))) return r;
+    const bool all_stat = c->px.stat && c->py.stat && c->pz1.stat;
+
+    struct Knob {
+        int Tuning::*field;
+        std::vector<int> values;
+        bool generic_only;   // only read by the generic (non-specialised) kernels
+    };
+    const std::vector<Knob> knobs = {
+        {&Tuning::zblk, {-2, -1, 0, 2, 3, 4, 5}, false},
+        {&Tuning::specialised, {1, 0}, false},
+        {&Tuning::zwave, {1, 0}, false},
+        {&Tuning::halfb_z, {0, 1, 2}, false},
+        {&Tuning::halfb_y, {0, 1}, false},
+        {&Tuning::twreg_z, {1, 0}, false},
+        {&Tuning::twreg_y, {0, 1}, false},
+        {&Tuning::radix_e, {16, 8, 32}, true},
+        {&Tuning::tile_y, {0, 4, 8, 16, 32}, true},
+        {&Tuning::tile_z, {0, 4, 8, 16, 32}, true},
+        {&Tuning::tile_x, {0, 2, 4, 8, 16}, true},
+        {&Tuning::xcd, {1, 0}, false},
+    };
+    bool best_all_stat = all_stat;
+    for (const Knob& k : knobs) {
+        if (k.generic_only && best_all_stat && best.specialised) continue;
+        for (int v : k.values) {
+            if (best.*k.field == v) continue;
+            Tuning t = best;
+            t.*k.field = v;
+            double ms = 0.0;
+            if (trial(t, &ms) != PS_OK) continue;     // shape not admissible here
+            if (ms < 0.99 * best_ms) {
+                best = t;
+                best_ms = ms;
+                best_all_stat = c->px.stat && c->py.stat && c->pz1.stat;
+            }
+        }
+    }
+    c->tune_req = best;
+    c->tune_set = true;
+    if ((r = ps_setup3d(c, ncell, bc, dx, reg))) return r;
+    if (best_out) *best_out = c->tune;
+    if (best_ms_out) *best_ms_out = best_ms;
+    return PS_OK;
 }
 
 }  // extern "C"

@@ -17,11 +17,9 @@ const std::vector<int>& compiled_E()
 }
 
 static const int kRadices[] = {16, 8, 7, 5, 4, 3, 2};
-static int g_pref_E = 16;
-void set_preferred_E(int e) { g_pref_E = e > 0 ? e : 16; }
-
-bool make_plan(int L, int maxE, Plan1D* out, std::string* why)
+bool make_plan(int L, int maxE, Plan1D* out, std::string* why, int prefE)
 {
+    if (prefE < 1) prefE = 16;
     if (L < 1) {
         if (why) *why = "transform length must be positive";
         return false;
@@ -63,8 +61,8 @@ bool make_plan(int L, int maxE, Plan1D* out, std::string* why)
             rem /= pick;
         }
         if (rem != 1) continue;
-        // fewer stages first (one LDS exchange each), then E closest to 16
-        auto score = [](const Plan1D& q) { return q.nstage * 100 + std::abs(q.E - g_pref_E); };
+        // fewer stages first (one LDS exchange each), then E closest to prefE
+        auto score = [prefE](const Plan1D& q) { return q.nstage * 100 + std::abs(q.E - prefE); };
         if (!found || score(c) < score(best)) {
             best = c;
             found = true;
@@ -85,29 +83,29 @@ int max_E_for(int precision_bytes, int ncomp)
     return 70;
 }
 
-static int g_lds_budget = 80 * 1024;
-void set_lds_budget(int bytes) { g_lds_budget = bytes < 16384 ? 16384 : (bytes > 160 * 1024 ? 160 * 1024 : bytes); }
-static int lds_budget() { return g_lds_budget; }
+static int clamp_lds(int bytes) { return bytes < 16384 ? 16384 : (bytes > 160 * 1024 ? 160 * 1024 : bytes); }
 
-int choose_B(const Plan1D& p, int elem_bytes, int ncol)
+int choose_B(const Plan1D& p, int elem_bytes, int ncol, int lds_bytes)
 {
+    const int budget = clamp_lds(lds_bytes);
     const int cbytes = 2 * elem_bytes;
     const int target = (elem_bytes == 4) ? 16 : 8;   // 128-byte row segments
     int B = target;
-    while (B > 1 && (p.TP * B > kMaxThreads || B * lds_len(p.L) * cbytes > lds_budget())) B /= 2;
+    while (B > 1 && (p.TP * B > kMaxThreads || B * lds_len(p.L) * cbytes > budget)) B /= 2;
     while (B > 1 && B / 2 >= ncol) B /= 2;
     return B < 1 ? 1 : B;
 }
 
-int choose_P(const Plan1D& p, int elem_bytes, int npairs)
+int choose_P(const Plan1D& p, int elem_bytes, int npairs, int lds_bytes)
 {
+    const int budget = clamp_lds(lds_bytes);
     const int cbytes = 2 * elem_bytes;
     int P = 8;
-    while (P > 1 && (p.TP * P > kMaxThreads || P * lds_len(p.L) * cbytes > lds_budget() / 2)) P /= 2;
+    while (P > 1 && (p.TP * P > kMaxThreads || P * lds_len(p.L) * cbytes > budget / 2)) P /= 2;
     while (P > 1 && P / 2 >= npairs) P /= 2;
     // at least one wavefront per workgroup when the pencils are short
     while (p.TP * P < 64 && P * 2 <= npairs && p.TP * P * 2 <= kMaxThreads &&
-           2 * P * lds_len(p.L) * cbytes <= lds_budget())
+           2 * P * lds_len(p.L) * cbytes <= budget)
         P *= 2;
     return P < 1 ? 1 : P;
 }

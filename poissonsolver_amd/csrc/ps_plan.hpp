@@ -20,22 +20,20 @@ struct Plan1D {
 // Elements-per-thread values compiled into the library (PS_FOR_EACH_E).
 const std::vector<int>& compiled_E();
 
-// Choose E and radices for length L with E <= maxE.  Returns false (and a
-// reason) when L has a prime factor above 7 or no compiled E fits.
-bool make_plan(int L, int maxE, Plan1D* out, std::string* why);
-// Among plans with the fewest stages, prefer E closest to this value (default 16).
-void set_preferred_E(int e);
+// Choose E and radices for length L with E <= maxE: among plans with the
+// fewest stages, E closest to prefE.  Returns false (and a reason) when L has
+// a prime factor above 7 or no compiled E fits.
+bool make_plan(int L, int maxE, Plan1D* out, std::string* why, int prefE = 16);
 
 // Largest E allowed for a kernel variant: the 3-component fused z pass
 // holds three pencils per thread, so it gets a smaller register budget.
 int max_E_for(int precision_bytes, int ncomp);
 
 // Workgroup shape of a strided pass (columns per tile) and of the x pass
-// (row pairs per workgroup), bounded by kMaxThreads and the LDS budget.
-// LDS budget per workgroup used by choose_B/choose_P (bytes).
-void set_lds_budget(int bytes);
-int choose_B(const Plan1D& p, int elem_bytes, int ncol);
-int choose_P(const Plan1D& p, int elem_bytes, int npairs);
+// (row pairs per workgroup), bounded by kMaxThreads and an LDS budget per
+// workgroup in bytes (clamped to [16 KB, 160 KB]).
+int choose_B(const Plan1D& p, int elem_bytes, int ncol, int lds_bytes = 80 * 1024);
+int choose_P(const Plan1D& p, int elem_bytes, int npairs, int lds_bytes = 80 * 1024);
 
 // Twiddle table exp(-2 pi i j/L), j < L, computed in long double.
 void twiddles(int L, std::vector<double>& re, std::vector<double>& im);

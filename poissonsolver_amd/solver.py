@@ -146,6 +146,33 @@ class poisson_solver:
     def synchronize(self):
         self._report(self._lib.ps_synchronize(self._ctx), "synchronize")
 
+    def tuning(self) -> dict:
+        """Tuning the last setup3d used (before setup: the one it will use)."""
+        t = _capi.ps_tuning()
+        self._report(self._lib.ps_get_tuning(self._ctx, ctypes.byref(t)), "get_tuning")
+        return t.as_dict()
+
+    def set_tuning(self, tuning=None, **fields):
+        """Tuning for the next setup3d: a dict and/or keyword fields over the
+        defaults; set_tuning(None) returns to the defaults."""
+        if tuning is None and not fields:
+            return self._report(self._lib.ps_set_tuning(self._ctx, None), "set_tuning")
+        t = _capi.ps_tuning()
+        self._lib.ps_tuning_default(ctypes.byref(t))
+        for k, v in dict(tuning or {}, **fields).items():
+            setattr(t, k, int(v))
+        return self._report(self._lib.ps_set_tuning(self._ctx, ctypes.byref(t)), "set_tuning")
+
+    def autotune(self, iters: int = 5):
+        """Pick the fastest tuning for this grid and options; pushed data is
+        discarded.  Returns (tuning dict, ms per solve) or None."""
+        t = _capi.ps_tuning()
+        ms = ctypes.c_double()
+        if not self._report(self._lib.ps_autotune(self._ctx, int(iters), ctypes.byref(t), ctypes.byref(ms)),
+                            "autotune"):
+            return None
+        return t.as_dict(), ms.value
+
     def profile_solve(self, iters: int = 1):
         """Per-pass average times (ms) measured with HIP events on the
         context's stream: dict x, y, z, y_inv, x_inv, solve."""
@@ -153,6 +180,76 @@ class poisson_solver:
         if not self._report(self._lib.ps_profile_solve(self._ctx, int(iters), ms), "profile_solve"):
             return None
         return dict(zip(("x", "y", "z", "y_inv", "x_inv", "solve"), list(ms)))
+
+
+class batch:
+    """Independent solves of uneven size replayed as one hipGraph launch
+    (ps_batch_* of ps_capi.h).  problems: list of (ncell, bc, dx, reg_order)."""
+
+    def __init__(self, problems, precision: int = _capi.PS_FP32):
+        self._lib = _capi.lib()
+        self._b = ctypes.c_void_p()
+        n = len(problems)
+        cell = (ctypes.c_int * (3 * n))(*[int(v) for p in problems for v in p[0]])
+        bc = (ctypes.c_int * (3 * n))(*[int(v) for p in problems for v in p[1]])
+        dx = (ctypes.c_double * (3 * n))(*[float(v) for p in problems for v in p[2]])
+        reg = (ctypes.c_int * n)(*[int(p[3]) for p in problems])
+        self.shapes = [tuple(int(v) for v in p[0]) for p in problems]
+        st = self._lib.ps_batch_create(ctypes.byref(self._b), n, cell, bc, dx, reg, int(precision))
+        if st != _capi.PS_OK:
+            raise RuntimeError("ps_batch_create: " + _capi.strerror(st))
+
+    def _check(self, st, where):
+        if st != _capi.PS_OK:
+            raise RuntimeError("%s: %s: %s" % (where, _capi.strerror(st), self._lib.ps_batch_last_error(self._b).decode()))
+
+    def close(self):
+        if getattr(self, "_b", None) and self._b.value:
+            self._lib.ps_batch_destroy(self._b)
+            self._b = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_ops(self, grad=False, div=False, curl=False, reproject=False):
+        self._check(self._lib.ps_batch_set_ops(self._b, int(grad), int(div), int(curl), int(reproject)), "set_ops")
+
+    def autotune(self, iters: int = 3):
+        self._check(self._lib.ps_batch_autotune(self._b, int(iters)), "autotune")
+
+    def tuning(self, prob: int) -> dict:
+        c = ctypes.c_void_p()
+        self._check(self._lib.ps_batch_context(self._b, int(prob), ctypes.byref(c)), "context")
+        t = _capi.ps_tuning()
+        self._lib.ps_get_tuning(c, ctypes.byref(t))
+        return t.as_dict()
+
+    def push(self, prob: int, bx=None, by=None, bz=None):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (bx, by, bz)]
+        self._check(self._lib.ps_batch_push(self._b, int(prob), *[_ptr(a) for a in arrs]), "push")
+
+    def push_device(self, prob: int, bx=None, by=None, bz=None):
+        p = [None if v is None else ctypes.c_void_p(int(v)) for v in (bx, by, bz)]
+        self._check(self._lib.ps_batch_push_device(self._b, int(prob), *p), "push_device")
+
+    def solve(self):
+        self._check(self._lib.ps_batch_solve(self._b), "solve")
+
+    def pull(self, prob: int, ax=None, ay=None, az=None, bx=None, by=None, bz=None):
+        for a in (ax, ay, az, bx, by, bz):
+            if a is not None and (a.dtype != np.float64 or not a.flags.c_contiguous or not a.flags.writeable):
+                raise TypeError("output arrays must be writable C-contiguous float64")
+        self._check(self._lib.ps_batch_pull(self._b, int(prob), *[_ptr(a) for a in (ax, ay, az, bx, by, bz)]),
+                    "pull")
+
+    def profile(self, iters: int = 10) -> float:
+        """Average ms of one graph launch over `iters` launches."""
+        ms = ctypes.c_double()
+        self._check(self._lib.ps_batch_profile(self._b, int(iters), ctypes.byref(ms)), "profile")
+        return ms.value
 
 
 def plan_describe(L: int, ncomp: int = 1, precision_bytes: int = 8):
