@@ -100,6 +100,10 @@ int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int
 int ps_eval_special(const double* x, double* y, long n, int which);
 /* enqueue ps_solve3d on the context's stream and return without waiting */
 int ps_solve3d_async(ps_ctx* ctx);
+/* kernel family of the one-component z pass chosen at setup: 0 runtime-plan,
+   1 compile-time specialised, 2 pencil-per-wavefront, 3 matrix-core (PS_MIXED);
+   negative: not set up */
+int ps_zpass_variant(const ps_ctx* ctx);
 
 /* --- extension: kernel tuning and the autotuner (no reference counterpart) ---
    Shapes and layouts the setup picks for the pass kernels.  Every field has a
@@ -123,6 +127,8 @@ typedef struct ps_tuning {
     int zwave;        /* pencil-per-wavefront z kernel where the length allows (PS_TUNE_ZWAVE, 1) */
     int halfb_z;      /* z tile-width divisor 0/1/2 -> B, B/2, B/4 (PS_TUNE_HALFB_Z, 0) */
     int halfb_y;      /* y tile-width divisor 0/1 -> B, B/2 (PS_TUNE_HALFB_Y, 0) */
+    int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT-32) kernels where the length
+                         has them, else the FP32 kernels (PS_TUNE_MFMA, 1) */
 } ps_tuning;
 /* the defaults (with PS_TUNE_* overrides applied) */
 int ps_tuning_default(ps_tuning* t);

@@ -18,7 +18,7 @@ _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)
 
 TUNING_FIELDS = ("zblk", "tile_x", "tile_y", "tile_z", "radix_e", "lds_kb", "xcd", "specialised",
-                 "specialised3", "twreg_z", "twreg_y", "occ4_z", "zwave", "halfb_z", "halfb_y")
+                 "specialised3", "twreg_z", "twreg_y", "occ4_z", "zwave", "halfb_z", "halfb_y", "mfma")
 
 
 class ps_tuning(ctypes.Structure):
@@ -60,6 +60,7 @@ SIGNATURES = {
                                         _c_int_p, _c_int_p]),
     "ps_eval_special": (ctypes.c_int, [_c_dbl_p, _c_dbl_p, ctypes.c_long, ctypes.c_int]),
     "ps_solve3d_async": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_zpass_variant": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_tuning_default": (ctypes.c_int, [_tun_p]),
     "ps_set_tuning": (ctypes.c_int, [ctypes.c_void_p, _tun_p]),
     "ps_get_tuning": (ctypes.c_int, [ctypes.c_void_p, _tun_p]),

@@ -55,6 +55,8 @@ def sources():
         units.append(("kern_e.hip", ["-DPS_E=%d" % e], "kern_e%d.o" % e))
     for L in sl_values():
         units.append(("skern.hip", ["-DPS_SL=%d" % L], "skern_%d.o" % L))
+    # the matrix-core kernels take magnitudes with fmax; no NaN handling needed
+    units.append(("mkern.hip", ["-fno-honor-nans"], "mkern.o"))
     for f in ("ps_greens.hip", "ps_dispatch.cpp", "ps_plan.cpp", "ps_capi.cpp", "ps_batch.cpp", "poisson_solver.cpp"):
         units.append((f, [], os.path.splitext(f)[0] + ".o"))
     return units

@@ -1,0 +1,87 @@
+// Mixed-precision (matrix-core) pass kernels: instantiation and launchers.
+#include "ps_mfma.hpp"
+#include "ps_mfma_launch.hpp"
+
+#include <cmath>
+
+namespace psamd {
+
+unsigned short f16_bits(double v)
+{
+    const unsigned short sign = std::signbit(v) ? 0x8000 : 0;
+    const double a = std::fabs(v);
+    if (a == 0.0) return sign;
+    int e = 0;
+    (void)std::frexp(a, &e);   // a = m 2^e, m in [0.5, 1)
+    int E = e - 1;             // a = 1.f 2^E
+    if (E >= -14) {
+        double r = std::nearbyint(std::ldexp(a, 10 - E));   // in [1024, 2048]
+        if (r >= 2048.0) {
+            r = 1024.0;
+            E += 1;
+        }
+        return (unsigned short)(sign | ((E + 15) << 10) | ((int)r - 1024));
+    }
+    // subnormal: k 2^-24 (k = 1024 is the smallest normal, same bits)
+    return (unsigned short)(sign | (int)std::nearbyint(std::ldexp(a, 24)));
+}
+
+double f16_value(unsigned short h)
+{
+    const int E = (h >> 10) & 31, f = h & 1023;
+    const double m = (E == 0) ? std::ldexp((double)f, -24) : std::ldexp((double)(1024 + f), E - 25);
+    return (h & 0x8000) ? -m : m;
+}
+
+void zmfma_fragments(unsigned short* out)
+{
+    for (int lane = 0; lane < 64; ++lane) {
+        const int i = lane & 15, qa = lane >> 4;
+        for (int c = 0; c < 2; ++c) {
+            for (int e8 = 0; e8 < 8; ++e8) {
+                const int m = 4 * qa + (e8 >> 1) + 16 * c;
+                const int jj = (i * m) & 31;
+                const double cs = Cos32::c[jj];
+                const double sn = Cos32::c[(jj + 24) & 31];   // sin(2 pi jj / 32)
+                const double vr = (e8 & 1) ? sn : cs;
+                const double vi = (e8 & 1) ? cs : -sn;
+                const unsigned short rh = f16_bits(vr), ih = f16_bits(vi);
+                unsigned short* o = out + (lane * 8 + 4 * c) * 8 + e8;
+                o[0] = rh;
+                o[8] = f16_bits(vr - f16_value(rh));
+                o[16] = ih;
+                o[24] = f16_bits(vi - f16_value(ih));
+            }
+        }
+    }
+}
+
+template <int OPK> static hipError_t launch_zmfma_op(PencilArgs<float> a, hipStream_t s)
+{
+    const size_t shm = MfmaZ::lds_bytes();
+    hipError_t e = allow_lds(k_zmfma1024<OPK>, shm);
+    if (e != hipSuccess) return e;
+    a.ncoltile = (a.ncol + MfmaZ::P - 1) / MfmaZ::P;
+    const long nwork = (long)a.ncoltile * a.nslab * a.ncomp;
+    hipLaunchKernelGGL((k_zmfma1024<OPK>), dim3((unsigned)nwork), dim3(64 * MfmaZ::NW), shm, s, a);
+    return hipGetLastError();
+}
+
+bool zmfma_supported(int L, int half, int op)
+{
+    return L == MfmaZ::L && half && (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF);
+}
+
+hipError_t launch_zmfma(const PencilArgs<float>& a, hipStream_t s)
+{
+    if (a.plan.L != MfmaZ::L || a.n_in != MfmaZ::L / 2 || a.n_out != MfmaZ::L / 2) return hipErrorInvalidValue;
+    switch (a.op) {
+    case OP_NONE: return launch_zmfma_op<OP_NONE>(a, s);
+    case OP_PLAIN: return launch_zmfma_op<OP_PLAIN>(a, s);
+    case OP_GRAD: return launch_zmfma_op<OP_GRAD>(a, s);
+    case OP_RHSF: return launch_zmfma_op<OP_RHSF>(a, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace psamd

@@ -22,6 +22,7 @@
 #include "../../include/ps_capi.h"
 #include "ps_greens.hpp"
 #include "ps_launch.hpp"
+#include "ps_mfma_launch.hpp"
 #include "ps_plan.hpp"
 #include "ps_static.hpp"
 
@@ -64,6 +65,7 @@ Tuning tuning_default()
     t.zwave = env_int("PS_TUNE_ZWAVE", 1);
     t.halfb_z = env_int("PS_TUNE_HALFB_Z", 0);
     t.halfb_y = env_int("PS_TUNE_HALFB_Y", 0);
+    t.mfma = env_int("PS_TUNE_MFMA", 1);
     return t;
 }
 
@@ -90,6 +92,7 @@ struct ps_ctx {
     long A1c = 0, A2c = 0;  // complex elements per component
     int ncomp_alloc = 0;
     void* G = nullptr;      // octant of Ghat*zeta/N, real
+    void* mfrag = nullptr;  // matrix-core DFT fragments (PS_MIXED)
 
     DevPlan px, py, pz1, pz3;
     Tuning tune = {};        // used by the last setup
@@ -97,6 +100,7 @@ struct ps_ctx {
     bool tune_set = false;   // tune_req replaces the defaults
     int zb = 0;             // A2 layout: log2 z block (-1 = [ky][z][kx])
     int zwave = 0;          // z pass uses the pencil-per-wavefront kernel
+    bool zmfma = false;     // PS_MIXED: one-component z pass on the matrix cores
     int g_layout = 0;       // Gk octant: 0 [ky][kz][kx], 1 [ky][kx][kz]
     Addr a2_ypass;          // A2 addressing seen by the y passes (slab = z, pos = ky)
     Addr a2_zpass;          // A2 addressing seen by the z pass (slab = ky, pos = z)
@@ -146,6 +150,7 @@ void release(ps_ctx* c)
     free_ptr(c->A1);
     free_ptr(c->A2);
     free_ptr(c->G);
+    free_ptr(c->mfrag);
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) free_ptr(d->tw);
     c->ncomp_alloc = 0;
     c->ready = false;
@@ -285,6 +290,7 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
     if (regm > 0 && regm % 2) regm += 1;
     a.reg = regm > 0 ? regm : 0;
     a.invN = 1.0 / ((double)c->M[0] * c->M[1] * c->M[2]);
+    a.mfrag = c->mfrag;
 }
 
 // The five passes.  `src` holds nin real components starting at in_first
@@ -346,7 +352,15 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     fill_zargs<T>(c, za, zp);
     za.op = op;
     za.ncomp = (nc == 3) ? 1 : nin;
-    if (zp.stat) CK(static_launch_pencil<T>(zp.p.L, za, nc, SM_FUSED, c->bc[2] == 0, s));
+    bool mfma_done = false;
+    if constexpr (sizeof(T) == 4) {
+        if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, op)) {
+            CK(launch_zmfma(za, s));
+            mfma_done = true;
+        }
+    }
+    if (mfma_done) {
+    } else if (zp.stat) CK(static_launch_pencil<T>(zp.p.L, za, nc, SM_FUSED, c->bc[2] == 0, s));
     else CK(launch_pencil<T>(zp.p.E, nc, za, s));
     if (ev) CK(hipEventRecord(ev[3], s));
 
@@ -459,7 +473,17 @@ template <typename T> int setup_impl(ps_ctx* c)
     // pencil-per-wavefront z kernel: specialised one-component plan, pencil
     // inside one wavefront, two or more stages; it reads Gk along kz
     c->zwave = tn.zwave && c->pz1.stat && (64 % c->pz1.p.TP == 0) && c->pz1.p.nstage >= 2;
-    c->g_layout = c->zwave ? 1 : 0;
+    // matrix-core z pass (PS_MIXED): its lanes read Gk along kx; positions
+    // z0 + 32 u must be linear in u (z blocks of at most 32 planes, or plain)
+    c->zmfma = c->prec == PS_MIXED && tn.mfma && zmfma_supported(c->M[2], c->bc[2] == 0, OP_PLAIN) &&
+               c->zb <= 5;
+    c->g_layout = (c->zwave && !c->zmfma) ? 1 : 0;
+    if (c->zmfma) {
+        std::vector<unsigned short> fr(kZmfmaFragHalves);
+        zmfma_fragments(fr.data());
+        CK(hipMalloc(&c->mfrag, fr.size() * sizeof(unsigned short)));
+        CK(hipMemcpy(c->mfrag, fr.data(), fr.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
+    }
     CK(hipMalloc(&c->G, (size_t)c->O[0] * c->O[1] * c->O[2] * sizeof(T)));
     r = build_greens<T>(c->ncell, c->bc, c->dx, c->reg, reinterpret_cast<T*>(c->G), c->g_layout, c->stream,
                         &c->err);
@@ -690,6 +714,14 @@ int ps_solve3d(ps_ctx* c)
     return PS_OK;
 }
 
+int ps_zpass_variant(const ps_ctx* c)
+{
+    if (!c || !c->ready) return -1;
+    if (c->zmfma) return 3;
+    if (c->zwave) return 2;
+    return c->pz1.stat ? 1 : 0;
+}
+
 int ps_solve3d_async(ps_ctx* c)
 {
     int r = check_ready(c);
@@ -881,7 +913,7 @@ int ps_set_tuning(ps_ctx* c, const ps_tuning* t)
     }
     if (t->zblk < -2 || t->zblk > 10 || t->tile_x < 0 || t->tile_x > 64 || t->tile_y < 0 || t->tile_y > 64 ||
         t->tile_z < 0 || t->tile_z > 64 || t->radix_e < 0 || t->radix_e > 70 || t->lds_kb < 16 || t->lds_kb > 160 ||
-        t->halfb_z < 0 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1)
+        t->halfb_z < 0 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1 || t->mfma < 0 || t->mfma > 1)
         return fail(c, PS_EINVAL, "set_tuning: field out of range");
     c->tune_req = *t;
     c->tune_set = true;
@@ -955,6 +987,7 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
         {&Tuning::tile_z, {0, 4, 8, 16, 32}, true},
         {&Tuning::tile_x, {0, 2, 4, 8, 16}, true},
         {&Tuning::xcd, {1, 0}, false},
+        {&Tuning::mfma, {1, 0}, false},
     };
     bool best_all_stat = all_stat;
     for (const Knob& k : knobs) {

@@ -1,0 +1,508 @@
+// Mixed-precision z pass on the matrix cores (PS_MIXED), transform length
+// L = 1024 = 32 x 32 (Mz of a 512-cell unbounded z direction).
+//
+// The fused z pass (pad nz -> Mz, FFT, k-space operator, IFFT, keep nz; the
+// reference's solve3d.cpp:243-441) is computed as a four-step transform whose
+// two 32-point DFT stages run as v_mfma_f32_16x16x32_f16 products:
+//
+//   X[b' + 32 q'] = sum_b W32^{b q'} W1024^{b b'} sum_{m<16} W32^{m b'} x[b + 32 m]
+//   x[32 n1 + n2] = sum_b' W32^{-n1 b'} W1024^{-n2 b'} sum_q' W32^{-n2 q'} Z[b' + 32 q']
+//
+// Each DFT stage multiplies the 32 x 32 DFT matrix (in real 2x2-block form,
+// split once into fp16 hi + lo) by 16 columns of fp16 hi + lo data:
+//   D = A_hi B_hi + A_hi B_lo + A_lo B_hi   (fp32 accumulation)
+// which keeps about 22 significant bits per product (the dropped A_lo B_lo
+// term is 2^-22 relative).  fp16 has a narrow exponent range, so every stage
+// input is scaled by a power of two that maps its largest magnitude below
+// 2^10; a 32-point DFT then grows values by at most 32 sqrt(2) < 2^5.5, which
+// keeps the next stage's fp16 inputs below 65504.  Scales are per DFT column
+// (stages whose input lives in one column's four lanes) or per pencil (stages
+// whose input comes from LDS after a workgroup barrier).  The twiddles
+// W1024^{b b'} are applied in fp32 on the vector ALUs.
+//
+// Workgroup: P = 8 pencils (adjacent kx columns of one ky slab) x the whole z
+// extent, 8 wavefronts.  An MFMA column is (pencil p = j & 7, butterfly
+// b = 2 blk + (j >> 3)); wavefront w owns blocks blk = w and w + 8 in every
+// stage.  Lane l = 16 q + j holds, in stage input and output alike, the
+// points 4q..4q+3 and 16+4q..16+4q+3 of its column, so the forward stage-2
+// output feeds the inverse stage-1 input in registers, with the k-space
+// operator in between.  The two data exchanges go through one LDS array
+// [p][row][col]; the inverse stage-1 output is written into the row its
+// forward input was read from (that row belongs to the same column), so only
+// two workgroup barriers separate the four stages.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ps_kernels_impl.hpp"
+
+namespace psamd {
+
+typedef _Float16 mf_h8 __attribute__((ext_vector_type(8)));
+typedef float mf_f4 __attribute__((ext_vector_type(4)));
+
+struct MfmaZ {
+    static constexpr int L = 1024;
+    static constexpr int R = 32;
+    static constexpr int P = 8;          // pencils per workgroup
+    static constexpr int NW = 8;         // wavefronts per workgroup
+    static constexpr int RS = 32;        // LDS row stride (complex)
+    static constexpr int PS = 32 * 32 + 3;   // LDS pencil stride (complex): at most 2-way bank conflicts
+    static constexpr size_t lds_bytes()
+    {
+        return (size_t)P * PS * 8 + (size_t)L * 8 + 2 * P * 4;
+    }
+};
+
+// cos(2 pi j / 32), j = 0..31
+struct Cos32 {
+    static constexpr double c[32] = {
+        1.0, 0.98078528040323044913, 0.92387953251128675613, 0.83146961230254523708,
+        0.70710678118654752440, 0.55557023301960222474, 0.38268343236508977173, 0.19509032201612826785,
+        0.0, -0.19509032201612826785, -0.38268343236508977173, -0.55557023301960222474,
+        -0.70710678118654752440, -0.83146961230254523708, -0.92387953251128675613, -0.98078528040323044913,
+        -1.0, -0.98078528040323044913, -0.92387953251128675613, -0.83146961230254523708,
+        -0.70710678118654752440, -0.55557023301960222474, -0.38268343236508977173, -0.19509032201612826785,
+        0.0, 0.19509032201612826785, 0.38268343236508977173, 0.55557023301960222474,
+        0.70710678118654752440, 0.83146961230254523708, 0.92387953251128675613, 0.98078528040323044913};
+};
+
+// ---------------------------------------------------------------------------
+// fp16 hi/lo split of scaled fp32 values with the mixed-precision FMA:
+//   hi = f16(x s),  lo = f16(x s - hi)   (x s - hi is exact before rounding)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned mf_hi_pair(float x, float y, float sx, float sy)
+{
+    unsigned h;
+    asm("v_fma_mixlo_f16 %0, %1, %2, 0 op_sel_hi:[0,0,0]" : "=v"(h) : "v"(x), "v"(sx));
+    asm("v_fma_mixhi_f16 %0, %1, %2, 0 op_sel_hi:[0,0,0]" : "+v"(h) : "v"(y), "v"(sy));
+    return h;
+}
+
+__device__ __forceinline__ unsigned mf_lo_pair(float x, float y, float sx, float sy, unsigned h)
+{
+    unsigned r;
+    asm("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel:[0,0,0] op_sel_hi:[0,0,1]" : "=v"(r) : "v"(x), "v"(sx), "v"(h));
+    asm("v_fma_mixhi_f16 %0, %1, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(r) : "v"(y), "v"(sy), "v"(h));
+    return r;
+}
+
+// unscaled pair: hi = f16(x), f16(y) (one packed conversion), lo as above
+__device__ __forceinline__ unsigned mf_hi_pair1(float x, float y)
+{
+    unsigned h;
+    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(h) : "v"(x), "v"(y));
+    return h;
+}
+
+__device__ __forceinline__ unsigned mf_lo_pair1(float x, float y, unsigned h)
+{
+    unsigned r;
+    asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel:[0,0,0] op_sel_hi:[0,0,1]" : "=v"(r) : "v"(x), "v"(h));
+    asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(r) : "v"(y), "v"(h));
+    return r;
+}
+
+struct MfFrag {
+    mf_h8 hi, lo;
+};
+
+// B fragment of four complex points (re, im interleaved) scaled by s; the
+// imaginary parts are scaled by sim (= -s for the conjugate)
+__device__ __forceinline__ MfFrag mf_split4(const float2 (&v)[4], float s, float sim)
+{
+    unsigned H[4], Lo[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        H[e] = mf_hi_pair(v[e].x, v[e].y, s, sim);
+        Lo[e] = mf_lo_pair(v[e].x, v[e].y, s, sim, H[e]);
+    }
+    MfFrag f;
+    f.hi = __builtin_bit_cast(mf_h8, H);
+    f.lo = __builtin_bit_cast(mf_h8, Lo);
+    return f;
+}
+
+// B fragment of four complex points already inside the fp16 range
+__device__ __forceinline__ MfFrag mf_split4u(const float2 (&v)[4])
+{
+    unsigned H[4], Lo[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        H[e] = mf_hi_pair1(v[e].x, v[e].y);
+        Lo[e] = mf_lo_pair1(v[e].x, v[e].y, H[e]);
+    }
+    MfFrag f;
+    f.hi = __builtin_bit_cast(mf_h8, H);
+    f.lo = __builtin_bit_cast(mf_h8, Lo);
+    return f;
+}
+
+// negate points 1 and 3 of a fragment: rows k + 16 of the 32-point DFT are
+// rows k with the input sign (-1)^m, and m = 4q + e + 16c has the parity of e
+__device__ __forceinline__ mf_h8 mf_neg_odd(mf_h8 x)
+{
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    u4 u = __builtin_bit_cast(u4, x);
+    u.y ^= 0x80008000u;
+    u.w ^= 0x80008000u;
+    return __builtin_bit_cast(mf_h8, u);
+}
+
+// A fragments of rows 0..15 of the 32-point forward DFT in real form, for K
+// chunk c (points m = 16 c + 0..15, re/im interleaved):
+//   Yr row k: ( cos t,  sin t),  Yi row k: (-sin t, cos t),  t = 2 pi k m / 32
+// Table layout (built by zmfma_fragments): [lane][c][rh, rl, ih, il][8 halves].
+struct MfDft32 {
+    mf_h8 rh[2], rl[2], ih[2], il[2];
+};
+
+__device__ __forceinline__ MfDft32 mf_dft32_frags(const void* table, int lane)
+{
+    const mf_h8* t = reinterpret_cast<const mf_h8*>(table) + lane * 8;
+    MfDft32 F;
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        F.rh[c] = t[4 * c + 0];
+        F.rl[c] = t[4 * c + 1];
+        F.ih[c] = t[4 * c + 2];
+        F.il[c] = t[4 * c + 3];
+    }
+    return F;
+}
+
+// four independent accumulator chains (rows 0..15 and 16..31 of Yr and Yi)
+// over NCH K chunks, issued round-robin so no product waits on the previous
+struct MfAcc {
+    mf_f4 r0, i0, r1, i1;
+};
+
+template <int NCH>
+__device__ __forceinline__ MfAcc mf_dft32_rows32(const MfDft32& F, const MfFrag (&B)[2], const MfFrag (&Bn)[2])
+{
+    MfAcc d;
+    d.r0 = d.i0 = d.r1 = d.i1 = mf_f4{0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], B[c].hi, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], B[c].hi, d.i0, 0, 0, 0);
+        d.r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], Bn[c].hi, d.r1, 0, 0, 0);
+        d.i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], Bn[c].hi, d.i1, 0, 0, 0);
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], B[c].lo, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], B[c].lo, d.i0, 0, 0, 0);
+        d.r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], Bn[c].lo, d.r1, 0, 0, 0);
+        d.i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], Bn[c].lo, d.i1, 0, 0, 0);
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rl[c], B[c].hi, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.il[c], B[c].hi, d.i0, 0, 0, 0);
+        d.r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rl[c], Bn[c].hi, d.r1, 0, 0, 0);
+        d.i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.il[c], Bn[c].hi, d.i1, 0, 0, 0);
+    }
+    return d;
+}
+
+// same product with one fragment set: rows 0..15 first, then the odd points
+// are negated in place for rows 16..31 (no second copy of the operands)
+template <int NCH>
+__device__ __forceinline__ MfAcc mf_dft32_rows32i(const MfDft32& F, MfFrag (&B)[2])
+{
+    MfAcc d;
+    d.r0 = d.i0 = d.r1 = d.i1 = mf_f4{0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], B[c].hi, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], B[c].hi, d.i0, 0, 0, 0);
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], B[c].lo, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], B[c].lo, d.i0, 0, 0, 0);
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rl[c], B[c].hi, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.il[c], B[c].hi, d.i0, 0, 0, 0);
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        B[c].hi = mf_neg_odd(B[c].hi);
+        B[c].lo = mf_neg_odd(B[c].lo);
+    }
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+        d.r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], B[c].hi, d.r1, 0, 0, 0);
+        d.i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], B[c].hi, d.i1, 0, 0, 0);
+        d.r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], B[c].lo, d.r1, 0, 0, 0);
+        d.i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], B[c].lo, d.i1, 0, 0, 0);
+        d.r1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rl[c], B[c].hi, d.r1, 0, 0, 0);
+        d.i1 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.il[c], B[c].hi, d.i1, 0, 0, 0);
+    }
+    return d;
+}
+
+// rows 0..15 only (the truncated last inverse stage): two chains
+__device__ __forceinline__ MfAcc mf_dft32_rows16(const MfDft32& F, const MfFrag (&B)[2])
+{
+    MfAcc d;
+    d.r0 = d.i0 = d.r1 = d.i1 = mf_f4{0, 0, 0, 0};
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], B[c].hi, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], B[c].hi, d.i0, 0, 0, 0);
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh[c], B[c].lo, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih[c], B[c].lo, d.i0, 0, 0, 0);
+        d.r0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rl[c], B[c].hi, d.r0, 0, 0, 0);
+        d.i0 = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.il[c], B[c].hi, d.i0, 0, 0, 0);
+    }
+    return d;
+}
+
+// byte offset pieces of position pos = base + 32 u along a pencil whose
+// addressing is blocked by 2^sh (sh <= 5) or plain (sh >= 30): linear in u
+__device__ __forceinline__ long mf_pos_stride32(int sh, long lo, long hi)
+{
+    return (sh >= 30) ? 32 * lo : (long)(32 >> sh) * hi;
+}
+
+// power-of-two scale mapping magnitudes below 2^e (m < 2^e) to below 2^10
+__device__ __forceinline__ void mf_scale(float m, float& s, float& inv)
+{
+    int e = 0;
+    (void)frexpf(m, &e);
+    e = e < -100 ? -100 : (e > 100 ? 100 : e);
+    s = ldexpf(1.0f, 10 - e);
+    inv = ldexpf(1.0f, e - 10);
+}
+
+__device__ __forceinline__ float mf_absmax(float2 v, float m)
+{
+    return fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+}
+
+// largest magnitude over the four lanes j, j+16, j+32, j+48 of a column:
+// lane ^ 16 by a swizzle inside each half-wave, lane ^ 32 by a permute
+// (bp32 = 4 (lane ^ 32), precomputed)
+__device__ __forceinline__ float mf_colmax(float m, int bp32)
+{
+    m = fmaxf(m, __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(__builtin_bit_cast(int, m), 0x401F)));
+    m = fmaxf(m, __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(bp32, __builtin_bit_cast(int, m))));
+    return m;
+}
+
+// k-space multiplier of one spectrum point (one-component operators):
+// plain Gk; grad i k_d Gk (Hermitian part, Nyquist zeroed); filtered rhs
+// zeta/N; none 1.
+// k-space multiplier of spectrum point kz for the one-component operators,
+// computed before the DFT stages (with the input loads), so that only FP32
+// products sit between the matrix-core stages:
+//   plain  Gk                        (real)
+//   grad   k_d Gk, applied as i k_d Gk (Hermitian part: Nyquist k_d = 0)
+//   filtered rhs  zeta / N           (real)
+//   none   1
+template <int OPK>
+__device__ __forceinline__ float mf_kfactor(const PencilArgs<float>& a, const TileK<float>& tk, int comp, int kz)
+{
+    if constexpr (OPK == OP_NONE) {
+        return 1.0f;
+    } else if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD) {
+        const int kzo = (kz <= a.Mz - kz) ? kz : a.Mz - kz;
+        const float g = tk.grow ? tk.grow[kzo * a.g_kz] : 0.0f;
+        if constexpr (OPK == OP_PLAIN) {
+            return g;
+        } else {
+            float kd;
+            if (comp == 0) kd = tk.kxh;
+            else if (comp == 1) kd = tk.kyh;
+            else kd = (2 * kz == a.Mz) ? 0.0f : (float)kappa_full(kz, a.Mz, a.dk[2]);
+            return kd * g;
+        }
+    } else {
+        const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
+        double zf = a.invN;
+        if (a.reg > 0) {
+            const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
+            double sum = 0.0, term = 1.0;
+            for (int n = 0; n < a.reg / 2; ++n) {
+                sum += term;
+                term = term * arg / (double)(n + 1);
+            }
+            zf *= sum * exp(-arg);
+        }
+        return (float)zf;
+    }
+}
+
+template <int OPK>
+__global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
+{
+    using M = MfmaZ;
+    constexpr int P = M::P, RS = M::RS, PSTR = M::PS, L = M::L;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* sd = reinterpret_cast<float2*>(smem_raw);
+    float2* stw = sd + P * PSTR;
+    unsigned* smax = reinterpret_cast<unsigned*>(stw + L);
+
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, l = tid & 63, q = l >> 4, j = l & 15;
+    const int p = j & 7, bsub = j >> 3;
+    const int bp32 = (l ^ 32) << 2;
+
+    const long nblk = gridDim.x;
+    const long lin = a.xcd_remap ? xcd_remap(blockIdx.x, nblk) : (long)blockIdx.x;
+    const long ntile = (long)a.ncoltile * a.nslab;
+    const int comp = (int)(lin / ntile);
+    const long rem = lin - (long)comp * ntile;
+    const int slab0 = (int)(rem / a.ncoltile);
+    const int ct = (int)(rem - (long)slab0 * a.ncoltile);
+    const int slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    const int col = ct * P + p;
+    const bool cv = col < a.ncol;
+    // columns past the grid repeat the last one: loads stay in bounds, their
+    // results are never stored, and no pencil's data mixes with another's
+    const int colr = cv ? col : a.ncol - 1;
+
+    for (int i = tid; i < L; i += 512) stw[i] = a.plan.tw[i];
+    if (tid < 2 * P) smax[tid] = 0u;
+    const MfDft32 F = mf_dft32_frags(a.mfrag, l);
+
+    // position z = z0 + 32 u of a pencil sits at base + off(z0) + u * s32
+    const float2* ip = reinterpret_cast<const float2*>(a.in) + comp * a.ia.comp + colr * a.ia.col +
+                       boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
+    float2* op_ = reinterpret_cast<float2*>(a.out) + comp * a.oa.comp + col * a.oa.col +
+                  boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi);
+    const long is32 = mf_pos_stride32(a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi);
+    const long os32 = mf_pos_stride32(a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
+    float2* myp = sd + p * PSTR;
+
+    // k-space multipliers of this lane's 16 spectrum points, computed with the
+    // input loads so their latency overlaps and the DFT stages see only FP32
+    // products
+    const TileK<float> tk = tile_k(a, col, slab, cv);
+    float gk[2][8];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int bp = 2 * (w + 8 * bi) + bsub;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            gk[bi][e] = mf_kfactor<OPK>(a, tk, comp, bp + 32 * (4 * q + e));
+            gk[bi][4 + e] = mf_kfactor<OPK>(a, tk, comp, bp + 32 * (16 + 4 * q + e));
+        }
+    }
+
+    // ---- load x[b + 32 m] (m < 16; the rest is zero padding) ----
+    float2 x[2][4];
+    float mx = 0.0f;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int b = 2 * (w + 8 * bi) + bsub;
+        const float2* src = ip + boff(b, a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) + (long)(4 * q) * is32;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            x[bi][e] = src[e * is32];
+            mx = mf_absmax(x[bi][e], mx);
+        }
+    }
+    __syncthreads();                       // twiddle table and max slots initialised
+    atomicMax(&smax[p], __float_as_uint(mx));
+    __syncthreads();
+    // one scale per pencil: the forward stage-1 output (<= 16 sqrt 2 2^10)
+    // is the stage-2 input without rescaling
+    float s0, inv0;
+    mf_scale(__uint_as_float(smax[p]), s0, inv0);
+
+    // ---- forward stage 1: DFT over m, 32 outputs k1 per column (p, b) ----
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int b = 2 * (w + 8 * bi) + bsub;
+        MfFrag B[2];
+        B[0] = mf_split4(x[bi], s0, s0);
+        const MfAcc d = mf_dft32_rows32i<1>(F, B);
+        // twiddle W1024^{b k1}; stored transposed as row k1, column b
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int k1 = 4 * q + e + 16 * h;
+                const float2 y = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                myp[k1 * RS + b] = cmul(y, stw[(b * k1) & (L - 1)]);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- forward stage 2, k-space operator, inverse stage 1 (registers) ----
+    float m3 = 0.0f;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int bp = 2 * (w + 8 * bi) + bsub;
+        float2* row = myp + bp * RS;
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = row[4 * q + e];
+            v1[e] = row[16 + 4 * q + e];
+        }
+        MfFrag B[2];
+        B[0] = mf_split4u(v0);
+        B[1] = mf_split4u(v1);
+        MfAcc d = mf_dft32_rows32i<2>(F, B);
+        // s0 X[bp + 32 q'], q' = 4q + e (r0/i0) and 16 + 4q + e (r1/i1);
+        // the k-space operator keeps the factor s0
+        float2 z0[4], z1[4];
+        float mz = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
+            if constexpr (OPK == OP_GRAD) {
+                // i k_d Gk: purely imaginary
+                z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
+                z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};
+            } else {
+                z0[e] = float2{d.r0[e] * m0, d.i0[e] * m0};
+                z1[e] = float2{d.r1[e] * m1, d.i1[e] * m1};
+            }
+            mz = mf_absmax(z0[e], mf_absmax(z1[e], mz));
+        }
+        float s2, inv2;
+        mf_scale(mf_colmax(mz, bp32), s2, inv2);
+        // inverse: conj(F) z = conj(F conj(z))
+        B[0] = mf_split4(z0, s2, -s2);
+        B[1] = mf_split4(z1, s2, -s2);
+        d = mf_dft32_rows32i<2>(F, B);
+        // U[n2] = conj of the inverse DFT output (times s0 s2); apply
+        // W1024^{-n2 bp} and undo both scales: conj(U W) inv0 inv2, written
+        // in place as row bp, column n2
+        const float f = inv0 * inv2;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int n2 = 4 * q + e + 16 * h;
+                const float2 u = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                const float2 t = cmul(u, stw[(n2 * bp) & (L - 1)]);
+                const float2 r = float2{t.x * f, -t.y * f};
+                row[n2] = r;
+                m3 = mf_absmax(r, m3);
+            }
+        }
+    }
+    atomicMax(&smax[P + p], __float_as_uint(m3));
+    __syncthreads();
+
+    // ---- inverse stage 2: column (p, n2), inputs over b', outputs n1 < 16 ----
+    float s3, inv3;
+    mf_scale(__uint_as_float(smax[P + p]), s3, inv3);
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int n2 = 2 * (w + 8 * bi) + bsub;
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = myp[(4 * q + e) * RS + n2];
+            v1[e] = myp[(16 + 4 * q + e) * RS + n2];
+        }
+        MfFrag B[2];
+        B[0] = mf_split4(v0, s3, -s3);
+        B[1] = mf_split4(v1, s3, -s3);
+        const MfAcc d = mf_dft32_rows16(F, B);
+        if (cv) {
+            float2* dst = op_ + boff(n2, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) + (long)(4 * q) * os32;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) dst[e * os32] = float2{d.r0[e] * inv3, -d.i0[e] * inv3};
+        }
+    }
+}
+
+}  // namespace psamd

@@ -1,0 +1,28 @@
+// Host entry points of the mixed-precision (matrix-core) pass kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ps_kernels.hpp"
+
+namespace psamd {
+
+// whether the fused z pass of a length-L transform (half: unbounded z, the
+// input is nz = L/2 planes and only nz planes are kept) with operator `op`
+// has a matrix-core kernel
+bool zmfma_supported(int L, int half, int op);
+
+// fused z pass on the matrix cores (FP16 hi/lo split DFT-32 stages, FP32
+// accumulation); `a` as for the FP32 fused z pass, one-component operators
+hipError_t launch_zmfma(const PencilArgs<float>& a, hipStream_t s);
+
+// per-lane fp16 hi/lo fragments of the 32-point DFT matrix that the
+// matrix-core kernels multiply by (binary16 bit patterns, kZmfmaFragHalves
+// entries; PencilArgs::mfrag points at a device copy)
+constexpr int kZmfmaFragHalves = 64 * 8 * 8;
+void zmfma_fragments(unsigned short* out);
+// round-to-nearest-even binary16 bit pattern of a finite double below 65504
+// in magnitude, and its value
+unsigned short f16_bits(double v);
+double f16_value(unsigned short h);
+
+}  // namespace psamd

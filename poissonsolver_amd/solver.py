@@ -146,6 +146,13 @@ class poisson_solver:
     def synchronize(self):
         self._report(self._lib.ps_synchronize(self._ctx), "synchronize")
 
+    ZPASS_VARIANTS = {0: "runtime-plan", 1: "specialised", 2: "pencil-per-wavefront", 3: "matrix-core"}
+
+    def zpass_variant(self) -> str:
+        """Kernel family of the one-component z pass chosen by setup3d."""
+        v = self._lib.ps_zpass_variant(self._ctx)
+        return self.ZPASS_VARIANTS.get(v, "not set up")
+
     def tuning(self) -> dict:
         """Tuning the last setup3d used (before setup: the one it will use)."""
         t = _capi.ps_tuning()

@@ -1,0 +1,99 @@
+"""Mixed-precision form (PS_MIXED) vs the oracle: the z pass of a 512-cell
+unbounded z direction (transform length 1024) runs on the matrix cores as
+FP16 hi/lo DFT-32 products with FP32 accumulation; every other pass is FP32.
+Bar: rel-L2 <= 1e-6 against the FP64 oracle (the north-star tolerance of the
+reduced-precision form).  x and y stay small so the oracle finishes in
+seconds."""
+import numpy as np
+import pytest
+
+import poissonsolver_amd as P
+from oracle import oracle as O
+from tests.gpu_util import announce, gpu_solve, random_rhs
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 1e-6
+
+CASES = [
+    ((8, 6, 512), (0, 0, 0), (0.1, 0.12, 2 / 512)),
+    ((12, 10, 512), (0, 0, 0), (0.05, 0.06, 1 / 256)),
+    ((16, 12, 512), (1, 1, 0), (1 / 16, 1 / 12, 2 / 512)),
+    ((9, 7, 512), (0, 0, 0), (0.1, 0.1, 0.004)),
+]
+
+
+def _variant(n, bc, dx, reg):
+    s = P.poisson_solver(P.PS_MIXED)
+    s.setup3d(n, bc, dx, reg)
+    v = s.zpass_variant()
+    s.close()
+    return v
+
+
+def _compare(n, bc, dx, reg, op, b, want_rhs=False, tol=MIXED_TOL):
+    lo, ro = O.solve(n, bc, dx, reg, b, op=op, reproject=False)
+    lg, rg = gpu_solve(n, bc, dx, reg, b, op=op, prec="mixed", want_rhs=want_rhs)
+    got = [lg[d] for d in range(3) if lo[d] is not None]
+    ref = [lo[d] for d in range(3) if lo[d] is not None]
+    err = O.rel_l2(got, ref)
+    assert err <= tol, ("lhs", err)
+    if want_rhs:
+        gr = [rg[d] for d in range(3) if ro[d] is not None]
+        rr = [ro[d] for d in range(3) if ro[d] is not None]
+        err_r = O.rel_l2(gr, rr)
+        assert err_r <= tol, ("rhs", err_r)
+    return err
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("reg", [0, 2, 6])
+def test_mixed_plain(case, reg):
+    n, bc, dx = case
+    announce("mixed plain", n, bc, reg)
+    assert _variant(n, bc, dx, reg) == "matrix-core"
+    b = random_rhs(n, 23)
+    _compare(n, bc, dx, reg, "plain", b)
+
+
+@pytest.mark.parametrize("case", CASES[:3], ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+def test_mixed_grad_and_filtered_rhs(case):
+    # grad: one-component z operator i k_d Gk per component; the pulled rhs
+    # with reg > 0 runs the filter pipeline (zeta / N) through the same kernel
+    n, bc, dx = case
+    announce("mixed grad + filtered rhs", n, bc)
+    b = random_rhs(n, 29)
+    _compare(n, bc, dx, 2, "grad", b, want_rhs=True)
+
+
+def test_mixed_dynamic_range():
+    # pencils whose magnitudes span 2^-60 .. 2^60: every stage scales its fp16
+    # operands per pencil or per DFT column
+    n, bc, dx = (16, 8, 512), (0, 0, 0), (0.1, 0.1, 2 / 512)
+    announce("mixed dynamic range", n)
+    rng = np.random.default_rng(31)
+    amp = np.exp2(np.linspace(-60, 60, n[0]))[None, None, :] * np.exp2(np.linspace(-20, 20, n[1]))[None, :, None]
+    b = [rng.standard_normal((n[2], n[1], n[0])) * amp]
+    _compare(n, bc, dx, 0, "plain", b)
+    # a source at 2^+-30 times that (still inside the FP32 range) gives the
+    # solution scaled by the same power of two
+    b2 = rng.standard_normal((n[2], n[1], n[0])) * np.exp2(np.linspace(-30, 30, n[0]))[None, None, :]
+    lg1, _ = gpu_solve(n, bc, dx, 0, [b2 * 2.0 ** 40], prec="mixed")
+    lg2, _ = gpu_solve(n, bc, dx, 0, [b2 * 2.0 ** -40], prec="mixed")
+    assert np.isfinite(lg1[0]).all() and np.isfinite(lg2[0]).all()
+    assert O.rel_l2([lg1[0] * 2.0 ** -80], [lg2[0]]) <= 1e-6
+
+
+def test_mixed_zero_source():
+    n, bc, dx = (8, 8, 512), (0, 0, 0), (0.1, 0.1, 2 / 512)
+    announce("mixed zero source", n)
+    lg, _ = gpu_solve(n, bc, dx, 2, [np.zeros((n[2], n[1], n[0]))], prec="mixed")
+    assert np.all(lg[0] == 0.0)
+
+
+def test_mixed_other_lengths_use_fp32_kernels():
+    n, bc, dx = (16, 12, 20), (0, 0, 0), (0.1, 0.1, 0.1)
+    announce("mixed fallback", n)
+    assert _variant(n, bc, dx, 2) != "matrix-core"
+    b = random_rhs(n, 37)
+    _compare(n, bc, dx, 2, "curl", b, tol=2e-6)
