@@ -128,7 +128,8 @@ typedef struct ps_tuning {
     int halfb_z;      /* z tile-width divisor 0/1/2 -> B, B/2, B/4 (PS_TUNE_HALFB_Z, 0) */
     int halfb_y;      /* y tile-width divisor 0/1 -> B, B/2 (PS_TUNE_HALFB_Y, 0) */
     int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT-32) kernels where the length
-                         has them, else the FP32 kernels (PS_TUNE_MFMA, 1) */
+                         has them: 1 one tile per workgroup, 2 persistent workgroups that
+                         prefetch; 0 the FP32 kernels (PS_TUNE_MFMA, 1) */
 } ps_tuning;
 /* the defaults (with PS_TUNE_* overrides applied) */
 int ps_tuning_default(ps_tuning* t);

@@ -2,6 +2,7 @@
 #include "ps_mfma.hpp"
 #include "ps_mfma_launch.hpp"
 
+#include <algorithm>
 #include <cmath>
 
 namespace psamd {
@@ -56,14 +57,24 @@ void zmfma_fragments(unsigned short* out)
     }
 }
 
-template <int OPK> static hipError_t launch_zmfma_op(PencilArgs<float> a, hipStream_t s)
+template <int OPK> static hipError_t launch_zmfma_op(PencilArgs<float> a, int persistent, hipStream_t s)
 {
     const size_t shm = MfmaZ::lds_bytes();
-    hipError_t e = allow_lds(k_zmfma1024<OPK>, shm);
-    if (e != hipSuccess) return e;
     a.ncoltile = (a.ncol + MfmaZ::P - 1) / MfmaZ::P;
     const long nwork = (long)a.ncoltile * a.nslab * a.ncomp;
-    hipLaunchKernelGGL((k_zmfma1024<OPK>), dim3((unsigned)nwork), dim3(64 * MfmaZ::NW), shm, s, a);
+    hipError_t e;
+    if (!persistent) {
+        if ((e = allow_lds(k_zmfma1024<OPK>, shm)) != hipSuccess) return e;
+        hipLaunchKernelGGL((k_zmfma1024<OPK>), dim3((unsigned)nwork), dim3(64 * MfmaZ::NW), shm, s, a);
+        return hipGetLastError();
+    }
+    // persistent: one workgroup per compute unit (2 wavefronts per SIMD)
+    if ((e = allow_lds(k_zmfma1024p<OPK>, shm)) != hipSuccess) return e;
+    int dev = 0, ncu = 0;
+    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+    if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
+    const long nwg = std::min<long>(nwork, (long)std::max(ncu, 1));
+    hipLaunchKernelGGL((k_zmfma1024p<OPK>), dim3((unsigned)nwg), dim3(64 * MfmaZ::NW), shm, s, a);
     return hipGetLastError();
 }
 
@@ -72,14 +83,14 @@ bool zmfma_supported(int L, int half, int op)
     return L == MfmaZ::L && half && (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF);
 }
 
-hipError_t launch_zmfma(const PencilArgs<float>& a, hipStream_t s)
+hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t s)
 {
     if (a.plan.L != MfmaZ::L || a.n_in != MfmaZ::L / 2 || a.n_out != MfmaZ::L / 2) return hipErrorInvalidValue;
     switch (a.op) {
-    case OP_NONE: return launch_zmfma_op<OP_NONE>(a, s);
-    case OP_PLAIN: return launch_zmfma_op<OP_PLAIN>(a, s);
-    case OP_GRAD: return launch_zmfma_op<OP_GRAD>(a, s);
-    case OP_RHSF: return launch_zmfma_op<OP_RHSF>(a, s);
+    case OP_NONE: return launch_zmfma_op<OP_NONE>(a, persistent, s);
+    case OP_PLAIN: return launch_zmfma_op<OP_PLAIN>(a, persistent, s);
+    case OP_GRAD: return launch_zmfma_op<OP_GRAD>(a, persistent, s);
+    case OP_RHSF: return launch_zmfma_op<OP_RHSF>(a, persistent, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -355,7 +355,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     bool mfma_done = false;
     if constexpr (sizeof(T) == 4) {
         if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, op)) {
-            CK(launch_zmfma(za, s));
+            CK(launch_zmfma(za, c->tune.mfma == 2, s));
             mfma_done = true;
         }
     }
@@ -475,8 +475,9 @@ template <typename T> int setup_impl(ps_ctx* c)
     c->zwave = tn.zwave && c->pz1.stat && (64 % c->pz1.p.TP == 0) && c->pz1.p.nstage >= 2;
     // matrix-core z pass (PS_MIXED): its lanes read Gk along kx; positions
     // z0 + 32 u must be linear in u (z blocks of at most 32 planes, or plain)
+    // 32-bit byte offsets: one component of A2 and the Gk octant below 4 GB
     c->zmfma = c->prec == PS_MIXED && tn.mfma && zmfma_supported(c->M[2], c->bc[2] == 0, OP_PLAIN) &&
-               c->zb <= 5;
+               c->zb <= 5 && c->A2c * 8 < (1L << 32) && (long)c->O[0] * c->O[1] * c->O[2] * 4 < (1L << 32);
     c->g_layout = (c->zwave && !c->zmfma) ? 1 : 0;
     if (c->zmfma) {
         std::vector<unsigned short> fr(kZmfmaFragHalves);
@@ -913,7 +914,7 @@ int ps_set_tuning(ps_ctx* c, const ps_tuning* t)
     }
     if (t->zblk < -2 || t->zblk > 10 || t->tile_x < 0 || t->tile_x > 64 || t->tile_y < 0 || t->tile_y > 64 ||
         t->tile_z < 0 || t->tile_z > 64 || t->radix_e < 0 || t->radix_e > 70 || t->lds_kb < 16 || t->lds_kb > 160 ||
-        t->halfb_z < 0 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1 || t->mfma < 0 || t->mfma > 1)
+        t->halfb_z < 0 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1 || t->mfma < 0 || t->mfma > 2)
         return fail(c, PS_EINVAL, "set_tuning: field out of range");
     c->tune_req = *t;
     c->tune_set = true;
@@ -987,7 +988,7 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
         {&Tuning::tile_z, {0, 4, 8, 16, 32}, true},
         {&Tuning::tile_x, {0, 2, 4, 8, 16}, true},
         {&Tuning::xcd, {1, 0}, false},
-        {&Tuning::mfma, {1, 0}, false},
+        {&Tuning::mfma, {1, 2, 0}, false},
     };
     bool best_all_stat = all_stat;
     for (const Knob& k : knobs) {

@@ -45,8 +45,12 @@ struct MfmaZ {
     static constexpr int R = 32;
     static constexpr int P = 8;          // pencils per workgroup
     static constexpr int NW = 8;         // wavefronts per workgroup
-    static constexpr int RS = 32;        // LDS row stride (complex)
-    static constexpr int PS = 32 * 32 + 3;   // LDS pencil stride (complex): at most 2-way bank conflicts
+    // LDS strides (complex): within each 16-lane group one access (64-bit)
+    // touches 16 distinct bank pairs in both exchange patterns, row-strided
+    // (p PS + bsub) and row-contiguous (p PS + bsub RS), as PS = 2 and
+    // RS = 1 (mod 16)
+    static constexpr int RS = 33;
+    static constexpr int PS = 32 * 33 + 2;
     static constexpr size_t lds_bytes()
     {
         return (size_t)P * PS * 8 + (size_t)L * 8 + 2 * P * 4;
@@ -284,46 +288,82 @@ __device__ __forceinline__ float mf_colmax(float m, int bp32)
 // k-space multiplier of one spectrum point (one-component operators):
 // plain Gk; grad i k_d Gk (Hermitian part, Nyquist zeroed); filtered rhs
 // zeta/N; none 1.
-// k-space multiplier of spectrum point kz for the one-component operators,
-// computed before the DFT stages (with the input loads), so that only FP32
-// products sit between the matrix-core stages:
+// Geometry of tile `lin` (component, ky slab, kx column tile) and this lane's
+// pencil in it.
+struct MfTile {
+    int comp, slab, col;
+    bool cv;
+};
+
+__device__ __forceinline__ MfTile mf_tile(const PencilArgs<float>& a, long lin, int p)
+{
+    const long ntile = (long)a.ncoltile * a.nslab;
+    MfTile t;
+    t.comp = (int)(lin / ntile);
+    const long rem = lin - (long)t.comp * ntile;
+    const int slab0 = (int)(rem / a.ncoltile);
+    const int ct = (int)(rem - (long)slab0 * a.ncoltile);
+    t.slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    t.col = ct * MfmaZ::P + p;
+    t.cv = t.col < a.ncol;
+    return t;
+}
+
+// k-space multipliers of the eight spectrum points kz = bp + 32 q' (q' = 4q + e
+// and 16 + 4q + e) that lane group q holds for butterfly bp, for the
+// one-component operators; computed before the DFT stages (with the input
+// loads), so that only FP32 products sit between the matrix-core stages:
 //   plain  Gk                        (real)
 //   grad   k_d Gk, applied as i k_d Gk (Hermitian part: Nyquist k_d = 0)
 //   filtered rhs  zeta / N           (real)
 //   none   1
+// goff0 / goff1: byte offsets of Gk rows kzo = kz (q' < 16) and kzo = Mz - kz
+// (q' >= 16) for e = 0, each stepping by -+gstep bytes (32 rows) per e.
 template <int OPK>
-__device__ __forceinline__ float mf_kfactor(const PencilArgs<float>& a, const TileK<float>& tk, int comp, int kz)
+__device__ __forceinline__ void mf_kfactors(float (&f)[8], const PencilArgs<float>& a, const TileK<float>& tk,
+                                            int comp, int bp, int q, int goff0, int goff1, int gstep)
 {
-    if constexpr (OPK == OP_NONE) {
-        return 1.0f;
-    } else if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD) {
-        const int kzo = (kz <= a.Mz - kz) ? kz : a.Mz - kz;
-        const float g = tk.grow ? tk.grow[kzo * a.g_kz] : 0.0f;
-        if constexpr (OPK == OP_PLAIN) {
-            return g;
-        } else {
-            float kd;
-            if (comp == 0) kd = tk.kxh;
-            else if (comp == 1) kd = tk.kyh;
-            else kd = (2 * kz == a.Mz) ? 0.0f : (float)kappa_full(kz, a.Mz, a.dk[2]);
-            return kd * g;
-        }
-    } else {
-        const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
-        double zf = a.invN;
-        if (a.reg > 0) {
-            const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
-            double sum = 0.0, term = 1.0;
-            for (int n = 0; n < a.reg / 2; ++n) {
-                sum += term;
-                term = term * arg / (double)(n + 1);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int kz = bp + 32 * (4 * q + e + 16 * h);
+            float v;
+            if constexpr (OPK == OP_NONE) {
+                v = 1.0f;
+            } else if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD) {
+                const unsigned go = h ? (unsigned)goff1 - (unsigned)e * (unsigned)gstep
+                                      : (unsigned)goff0 + (unsigned)e * (unsigned)gstep;
+                const float g = tk.grow ? *reinterpret_cast<const float*>(reinterpret_cast<const char*>(tk.grow) + go) : 0.0f;
+                if constexpr (OPK == OP_PLAIN) {
+                    v = g;
+                } else {
+                    float kd;
+                    if (comp == 0) kd = tk.kxh;
+                    else if (comp == 1) kd = tk.kyh;
+                    else kd = (2 * kz == a.Mz) ? 0.0f : (float)kappa_full(kz, a.Mz, a.dk[2]);
+                    v = kd * g;
+                }
+            } else {
+                const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
+                double zf = a.invN;
+                if (a.reg > 0) {
+                    const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
+                    double sum = 0.0, term = 1.0;
+                    for (int n = 0; n < a.reg / 2; ++n) {
+                        sum += term;
+                        term = term * arg / (double)(n + 1);
+                    }
+                    zf *= sum * exp(-arg);
+                }
+                v = (float)zf;
             }
-            zf *= sum * exp(-arg);
+            f[4 * h + e] = v;
         }
-        return (float)zf;
     }
 }
 
+// One tile per workgroup (4 wavefronts per SIMD, two workgroups per CU).
 template <int OPK>
 __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
 {
@@ -357,13 +397,15 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
     if (tid < 2 * P) smax[tid] = 0u;
     const MfDft32 F = mf_dft32_frags(a.mfrag, l);
 
-    // position z = z0 + 32 u of a pencil sits at base + off(z0) + u * s32
-    const float2* ip = reinterpret_cast<const float2*>(a.in) + comp * a.ia.comp + colr * a.ia.col +
-                       boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
-    float2* op_ = reinterpret_cast<float2*>(a.out) + comp * a.oa.comp + col * a.oa.col +
-                  boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi);
-    const long is32 = mf_pos_stride32(a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi);
-    const long os32 = mf_pos_stride32(a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
+    // addresses: a uniform base per component plus 32-bit byte offsets (the
+    // host checks that a component and the Gk octant are below 4 GB); the
+    // position z = z0 + 32 u of a pencil sits at off(z0) + u * s32
+    const char* ibase = reinterpret_cast<const char*>(a.in) + (long)comp * a.ia.comp * 8;
+    char* obase = reinterpret_cast<char*>(a.out) + (long)comp * a.oa.comp * 8;
+    const unsigned icol = (unsigned)(colr * a.ia.col + boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi)) * 8u;
+    const unsigned ocol = (unsigned)(col * a.oa.col + boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi)) * 8u;
+    const unsigned is32 = (unsigned)mf_pos_stride32(a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) * 8u;
+    const unsigned os32 = (unsigned)mf_pos_stride32(a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) * 8u;
     float2* myp = sd + p * PSTR;
 
     // k-space multipliers of this lane's 16 spectrum points, computed with the
@@ -371,14 +413,12 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
     // products
     const TileK<float> tk = tile_k(a, col, slab, cv);
     float gk[2][8];
+    const int gkz = (int)a.g_kz;
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
         const int bp = 2 * (w + 8 * bi) + bsub;
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            gk[bi][e] = mf_kfactor<OPK>(a, tk, comp, bp + 32 * (4 * q + e));
-            gk[bi][4 + e] = mf_kfactor<OPK>(a, tk, comp, bp + 32 * (16 + 4 * q + e));
-        }
+        mf_kfactors<OPK>(gk[bi], a, tk, comp, bp, q, (bp + 128 * q) * gkz * 4, (M::L / 2 - bp - 128 * q) * gkz * 4,
+                         32 * gkz * 4);
     }
 
     // ---- load x[b + 32 m] (m < 16; the rest is zero padding) ----
@@ -387,10 +427,10 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) {
         const int b = 2 * (w + 8 * bi) + bsub;
-        const float2* src = ip + boff(b, a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) + (long)(4 * q) * is32;
+        const unsigned o0 = icol + (unsigned)boff(b, a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) * 8u + (unsigned)(4 * q) * is32;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            x[bi][e] = src[e * is32];
+            x[bi][e] = *reinterpret_cast<const float2*>(ibase + (o0 + (unsigned)e * is32));
             mx = mf_absmax(x[bi][e], mx);
         }
     }
@@ -498,10 +538,217 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
         B[1] = mf_split4(v1, s3, -s3);
         const MfAcc d = mf_dft32_rows16(F, B);
         if (cv) {
-            float2* dst = op_ + boff(n2, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) + (long)(4 * q) * os32;
+            const unsigned o0 = ocol + (unsigned)boff(n2, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) * 8u + (unsigned)(4 * q) * os32;
 #pragma unroll
-            for (int e = 0; e < 4; ++e) dst[e * os32] = float2{d.r0[e] * inv3, -d.i0[e] * inv3};
+            for (int e = 0; e < 4; ++e)
+                *reinterpret_cast<float2*>(obase + (o0 + (unsigned)e * os32)) = float2{d.r0[e] * inv3, -d.i0[e] * inv3};
         }
+    }
+}
+
+// Persistent form (2 wavefronts per SIMD, one workgroup per CU, 256
+// registers): workgroup g handles tiles g, g + G, g + 2G, ... (in
+// XCD-contiguous order).  The next tile's input and k-space multipliers are
+// fetched into the registers that the current tile's forward stage 1 and
+// k-space step have just released, so their latency overlaps the remaining
+// stages.
+template <int OPK>
+__global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
+{
+    using M = MfmaZ;
+    constexpr int P = M::P, RS = M::RS, PSTR = M::PS, L = M::L;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* sd = reinterpret_cast<float2*>(smem_raw);
+    float2* stw = sd + P * PSTR;
+    unsigned* smax = reinterpret_cast<unsigned*>(stw + L);
+
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, l = tid & 63, q = l >> 4, j = l & 15;
+    const int p = j & 7, bsub = j >> 3;
+    const int bp32 = (l ^ 32) << 2;
+
+    const long ntot = (long)a.ncoltile * a.nslab * a.ncomp;
+    const long G = gridDim.x;
+
+    for (int i = tid; i < L; i += 512) stw[i] = a.plan.tw[i];
+    if (tid < 2 * P) smax[tid] = 0u;
+    const MfDft32 F = mf_dft32_frags(a.mfrag, l);
+
+    // lane geometry shared by every tile: butterfly / column index of block bi
+    // (the same index set serves as b, b', n2), position offsets of the
+    // points z = b + 32 (4q + e) (input) and z = 32 (4q + e) + n2 (output)
+    const long is32 = mf_pos_stride32(a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi);
+    const long os32 = mf_pos_stride32(a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
+    int bb[2];
+    long ipos[2], opos[2];
+    int goff0[2], goff1[2];
+    const int gkz = (int)a.g_kz;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        bb[bi] = 2 * (w + 8 * bi) + bsub;
+        ipos[bi] = boff(bb[bi], a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) + (long)(4 * q) * is32;
+        opos[bi] = boff(bb[bi], a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) + (long)(4 * q) * os32;
+        goff0[bi] = (bb[bi] + 128 * q) * gkz * 4;
+        goff1[bi] = (M::L / 2 - bb[bi] - 128 * q) * gkz * 4;
+    }
+    const int gstep = 32 * gkz * 4;
+
+    // first tile: input and multipliers
+    long lin = blockIdx.x;
+    float2 x[2][4];
+    float gk[2][8];
+    MfTile T{};
+    if (lin < ntot) {
+        T = mf_tile(a, a.xcd_remap ? xcd_remap(lin, ntot) : lin, p);
+        const int colr = T.cv ? T.col : a.ncol - 1;
+        const float2* ip = reinterpret_cast<const float2*>(a.in) + T.comp * a.ia.comp + colr * a.ia.col +
+                           boff(T.slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
+        const TileK<float> tk = tile_k(a, T.col, T.slab, T.cv);
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x[bi][e] = ip[ipos[bi] + e * is32];
+            mf_kfactors<OPK>(gk[bi], a, tk, T.comp, bb[bi], q, goff0[bi], goff1[bi], gstep);
+        }
+    }
+    __syncthreads();                       // twiddle table and max slots initialised
+
+    for (; lin < ntot; lin += G) {
+        const long nlin = lin + G;
+        const bool more = nlin < ntot;
+        const MfTile T1 = more ? mf_tile(a, a.xcd_remap ? xcd_remap(nlin, ntot) : nlin, p) : T;
+        float2* op_ = reinterpret_cast<float2*>(a.out) + T.comp * a.oa.comp + T.col * a.oa.col +
+                      boff(T.slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi);
+        float2* myp = sd + p * PSTR;
+
+        // ---- one scale per pencil: the forward stage-1 output (<= 16 sqrt 2
+        // 2^10) is the stage-2 input without rescaling ----
+        float mx = 0.0f;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) mx = mf_absmax(x[bi][e], mx);
+        atomicMax(&smax[p], __float_as_uint(mx));
+        __syncthreads();
+        if (tid < P) smax[P + tid] = 0u;   // inverse-stage slots: last read before this barrier
+        float s0, inv0;
+        mf_scale(__uint_as_float(smax[p]), s0, inv0);
+
+        // ---- forward stage 1: DFT over m < 16, 32 outputs k1 per column (p, b) ----
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int b = bb[bi];
+            MfFrag B[2];
+            B[0] = mf_split4(x[bi], s0, s0);
+            const MfAcc d = mf_dft32_rows32i<1>(F, B);
+            // twiddle W1024^{b k1}; stored transposed as row k1, column b
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int k1 = 4 * q + e + 16 * h;
+                    const float2 y = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                    myp[k1 * RS + b] = cmul(y, stw[(b * k1) & (L - 1)]);
+                }
+            }
+        }
+        // the input registers are free: fetch the next tile's
+        if (more) {
+            const int colr = T1.cv ? T1.col : a.ncol - 1;
+            const float2* ip = reinterpret_cast<const float2*>(a.in) + T1.comp * a.ia.comp + colr * a.ia.col +
+                               boff(T1.slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
+#pragma unroll
+            for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[bi][e] = ip[ipos[bi] + e * is32];
+        }
+        __syncthreads();
+        if (tid < P) smax[tid] = 0u;       // forward slots: last read before this barrier
+
+        // ---- forward stage 2, k-space operator, inverse stage 1 (registers) ----
+        const TileK<float> tk1 = more ? tile_k(a, T1.col, T1.slab, T1.cv) : TileK<float>{};
+        float m3 = 0.0f;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int bp = bb[bi];
+            float2* row = myp + bp * RS;
+            float2 v0[4], v1[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v0[e] = row[4 * q + e];
+                v1[e] = row[16 + 4 * q + e];
+            }
+            MfFrag B[2];
+            B[0] = mf_split4u(v0);
+            B[1] = mf_split4u(v1);
+            MfAcc d = mf_dft32_rows32i<2>(F, B);
+            // s0 X[bp + 32 q'], q' = 4q + e (r0/i0) and 16 + 4q + e (r1/i1);
+            // the k-space operator keeps the factor s0
+            float2 z0[4], z1[4];
+            float mz = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
+                if constexpr (OPK == OP_GRAD) {
+                    // i k_d Gk: purely imaginary
+                    z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
+                    z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};
+                } else {
+                    z0[e] = float2{d.r0[e] * m0, d.i0[e] * m0};
+                    z1[e] = float2{d.r1[e] * m1, d.i1[e] * m1};
+                }
+                mz = mf_absmax(z0[e], mf_absmax(z1[e], mz));
+            }
+            // this block's multipliers are used: fetch the next tile's
+            if (more) mf_kfactors<OPK>(gk[bi], a, tk1, T1.comp, bp, q, goff0[bi], goff1[bi], gstep);
+            float s2, inv2;
+            mf_scale(mf_colmax(mz, bp32), s2, inv2);
+            // inverse: conj(F) z = conj(F conj(z))
+            B[0] = mf_split4(z0, s2, -s2);
+            B[1] = mf_split4(z1, s2, -s2);
+            d = mf_dft32_rows32i<2>(F, B);
+            // U[n2] = conj of the inverse DFT output (times s0 s2); apply
+            // W1024^{-n2 bp} and undo both scales: conj(U W) inv0 inv2, written
+            // in place as row bp, column n2
+            const float f = inv0 * inv2;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int n2 = 4 * q + e + 16 * h;
+                    const float2 u = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                    const float2 t = cmul(u, stw[(n2 * bp) & (L - 1)]);
+                    const float2 r = float2{t.x * f, -t.y * f};
+                    row[n2] = r;
+                    m3 = mf_absmax(r, m3);
+                }
+            }
+        }
+        atomicMax(&smax[P + p], __float_as_uint(m3));
+        __syncthreads();
+
+        // ---- inverse stage 2: column (p, n2), inputs over b', outputs n1 < 16 ----
+        float s3, inv3;
+        mf_scale(__uint_as_float(smax[P + p]), s3, inv3);
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int n2 = bb[bi];
+            float2 v0[4], v1[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v0[e] = myp[(4 * q + e) * RS + n2];
+                v1[e] = myp[(16 + 4 * q + e) * RS + n2];
+            }
+            MfFrag B[2];
+            B[0] = mf_split4(v0, s3, -s3);
+            B[1] = mf_split4(v1, s3, -s3);
+            const MfAcc d = mf_dft32_rows16(F, B);
+            if (T.cv) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) op_[opos[bi] + e * os32] = float2{d.r0[e] * inv3, -d.i0[e] * inv3};
+            }
+        }
+        T = T1;
     }
 }
 

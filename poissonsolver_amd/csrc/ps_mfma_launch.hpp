@@ -12,8 +12,10 @@ namespace psamd {
 bool zmfma_supported(int L, int half, int op);
 
 // fused z pass on the matrix cores (FP16 hi/lo split DFT-32 stages, FP32
-// accumulation); `a` as for the FP32 fused z pass, one-component operators
-hipError_t launch_zmfma(const PencilArgs<float>& a, hipStream_t s);
+// accumulation); `a` as for the FP32 fused z pass, one-component operators.
+// persistent = 0: one tile per workgroup, 4 wavefronts per SIMD; 1: one
+// persistent workgroup per CU that prefetches its next tile
+hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t s);
 
 // per-lane fp16 hi/lo fragments of the 32-point DFT matrix that the
 // matrix-core kernels multiply by (binary16 bit patterns, kZmfmaFragHalves

@@ -46,29 +46,39 @@ def _compare(n, bc, dx, reg, op, b, want_rhs=False, tol=MIXED_TOL):
     return err
 
 
+# matrix-core kernel forms: 1 one tile per workgroup, 2 persistent workgroups
+FORMS = ["1", "2"]
+
+
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
 @pytest.mark.parametrize("reg", [0, 2, 6])
-def test_mixed_plain(case, reg):
+@pytest.mark.parametrize("form", FORMS)
+def test_mixed_plain(case, reg, form, monkeypatch):
+    monkeypatch.setenv("PS_TUNE_MFMA", form)
     n, bc, dx = case
-    announce("mixed plain", n, bc, reg)
+    announce("mixed plain", n, bc, reg, "form", form)
     assert _variant(n, bc, dx, reg) == "matrix-core"
     b = random_rhs(n, 23)
     _compare(n, bc, dx, reg, "plain", b)
 
 
 @pytest.mark.parametrize("case", CASES[:3], ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
-def test_mixed_grad_and_filtered_rhs(case):
+@pytest.mark.parametrize("form", FORMS)
+def test_mixed_grad_and_filtered_rhs(case, form, monkeypatch):
     # grad: one-component z operator i k_d Gk per component; the pulled rhs
     # with reg > 0 runs the filter pipeline (zeta / N) through the same kernel
+    monkeypatch.setenv("PS_TUNE_MFMA", form)
     n, bc, dx = case
-    announce("mixed grad + filtered rhs", n, bc)
+    announce("mixed grad + filtered rhs", n, bc, "form", form)
     b = random_rhs(n, 29)
     _compare(n, bc, dx, 2, "grad", b, want_rhs=True)
 
 
-def test_mixed_dynamic_range():
+@pytest.mark.parametrize("form", FORMS)
+def test_mixed_dynamic_range(form, monkeypatch):
     # pencils whose magnitudes span 2^-60 .. 2^60: every stage scales its fp16
     # operands per pencil or per DFT column
+    monkeypatch.setenv("PS_TUNE_MFMA", form)
     n, bc, dx = (16, 8, 512), (0, 0, 0), (0.1, 0.1, 2 / 512)
     announce("mixed dynamic range", n)
     rng = np.random.default_rng(31)
