@@ -2,12 +2,15 @@
 
 Default workload: 512^3 fully unbounded scalar Poisson solve (bc 0,0,0; domain
 doubled to 1024^3 through the R2C x pass and the pruned z pass), regularisation
-order 2, Gaussian source (s = 0.1 on [-1,1]^3), FP32 storage.  One step = one
-`solve3d` with the source already resident in HBM (pushed before the timed
-region).  Prints one JSON line.
+order 2, Gaussian source (s = 0.1 on [-1,1]^3), FP32 storage, in the
+mixed-precision form (z pass on the matrix cores: FP16 hi/lo DFT-32 stages with
+FP32 accumulation); the plain FP32 path is timed beside it on the same input
+and both are checked against the FP64 GPU solve.  One step = one `solve3d`
+with the source already resident in HBM (pushed before the timed region).
+Prints one JSON line.
 
-  python bench.py [--steps K] [--warmup W] [--prec fp32|mixed|fp64]
-                  [--n 512] [--op plain|curl] [--no-cpu] [--no-check]
+  python bench.py [--steps K] [--warmup W] [--prec mixed|fp32|fp64]
+                  [--n 512] [--op plain|curl] [--no-cpu] [--no-check] [--no-compare]
   python bench.py --workload batch [--nprob 64] [--autotune]
       64 independent unbounded scalar problems, n from {32,...,192} (seed 4),
       sources = sums of 1-4 Gaussian blobs; one step = one hipGraph launch
@@ -84,7 +87,11 @@ CALIB_BYTES = 1 << 30
 
 
 def is_zpass_kernel(name, nc):
-    """k_pencil_s<T, L, NC, MODE, ...> with MODE 2 (fused z pass) and the given NC."""
+    """The fused z pass: k_zmfma1024 / k_zmfma1024p (matrix-core, one
+    component), k_zfused_w (pencil per wavefront, one component) or
+    k_pencil_s<T, L, NC, MODE, ...> with MODE 2 and the given NC."""
+    if nc == 1 and ("k_zmfma1024" in name or "k_zfused_w<" in name):
+        return True
     if "k_pencil_s<" not in name:
         return False
     args = [x.strip() for x in name.split("k_pencil_s<", 1)[1].split(">", 1)[0].split(",")]
@@ -251,7 +258,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--prec", default="fp32", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--prec", default="mixed", choices=["fp32", "mixed", "fp64"])
+    ap.add_argument("--no-compare", action="store_true", help="mixed: do not time the FP32 path beside it")
     ap.add_argument("--op", default="plain", choices=["plain", "curl"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -280,29 +288,56 @@ def main(argv=None):
     n = a.n
     b, h = gaussian_source(n)
     cin = 3 if a.op == "curl" else 1
+    cout = 3 if a.op == "curl" else 1
     srcs = [b, 0.5 * b, -0.25 * b][:cin] if a.op == "curl" else [b]
-    prec = {"fp32": P.PS_FP32, "mixed": P.PS_MIXED, "fp64": P.PS_FP64}[a.prec]
-    s = P.poisson_solver(prec)
-    s.setup3d((n,) * 3, (0, 0, 0), (h,) * 3, 2)
-    s.set_return_curl(a.op == "curl")
-    s.push(*srcs)
-    if s.last_status:
-        return 1
-    for _ in range(a.warmup):
-        s.solve3d()
-    s.synchronize()
-    t0 = time.perf_counter()
-    prof = s.profile_solve(a.steps)          # K solves, HIP events on the solver stream
-    s.synchronize()
-    wall = time.perf_counter() - t0
+    codes = {"fp32": P.PS_FP32, "mixed": P.PS_MIXED, "fp64": P.PS_FP64}
+
+    def run(prec):
+        """Time K solves of one precision (inputs resident in HBM); returns
+        (per-pass ms, wall s, solver, z kernel family)."""
+        s = P.poisson_solver(codes[prec])
+        s.setup3d((n,) * 3, (0, 0, 0), (h,) * 3, 2)
+        s.set_return_curl(a.op == "curl")
+        s.push(*srcs)
+        if s.last_status:
+            return None, 0.0, s, ""
+        for _ in range(a.warmup):
+            s.solve3d()
+        s.synchronize()
+        t0 = time.perf_counter()
+        prof = s.profile_solve(a.steps)          # K solves, HIP events on the solver stream
+        s.synchronize()
+        return prof, time.perf_counter() - t0, s, s.zpass_variant()
+
+    ref = None
+
+    def rel_to_fp64(s):
+        nonlocal ref
+        lhs = [np.empty_like(b) for _ in range(cout)]
+        s.pull(*lhs)
+        if ref is None:
+            r = P.poisson_solver(P.PS_FP64)
+            r.setup3d((n,) * 3, (0, 0, 0), (h,) * 3, 2)
+            r.set_return_curl(a.op == "curl")
+            r.push(*srcs)
+            r.solve3d()
+            ref = [np.empty_like(b) for _ in range(cout)]
+            r.pull(*ref)
+            r.close()
+        num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(lhs, ref))
+        den = sum(float(np.sum(y ** 2)) for y in ref)
+        return math.sqrt(num / den), lhs
+
+    prof, wall, s, zvar = run(a.prec)
     if prof is None:
         return 1
     ms = prof["solve"]
     eb = 8 if a.prec == "fp64" else 4
-    cout = 3 if a.op == "curl" else 1
     B = alg_bytes(n, cin, cout, eb)
     dom = "z"
     achieved = B[dom] / (prof[dom] * 1e-3) / 1e9
+    dtype = {"fp64": "f64", "fp32": "f32",
+             "mixed": "mixed: z pass f16 hi/lo MFMA DFT-32 stages, f32 accumulation; f32 elsewhere"}[a.prec]
     out = {
         "metric": METRIC,
         "value": 1000.0 / ms,
@@ -314,7 +349,7 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.prec == "fp64" else "f32",
+        "dtype": dtype,
         "data": "synthetic",
         "config": {
             "workload": "%d^3 fully unbounded %s Poisson solve, FFT domain doubled to %d^3, reg order 2, "
@@ -324,7 +359,8 @@ def main(argv=None):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "z pass (k_pencil NC=%d: pad, FFT, Ghat*zeta*op, IFFT, truncate)" % (3 if a.op == "curl" else 1),
+            "kernel": "z pass (%s kernel, %d component%s: pad, FFT, Ghat*zeta*op, IFFT, truncate)"
+                      % (zvar, cin if a.op == "curl" else 1, "s" if a.op == "curl" else ""),
             "alg_bytes_per_launch": B[dom], "avg_ms": prof[dom],
         },
         "solve_hbm_gbs": B["solve"] / (ms * 1e-3) / 1e9,
@@ -335,28 +371,20 @@ def main(argv=None):
         "wall_s_timed": wall,
     }
     if not (a.no_traffic or a.pmc_child):
-        child = ["--pmc-child", "--no-traffic", "--no-cpu", "--no-check", "--steps", "2", "--warmup", "1",
-                 "--n", str(n), "--prec", a.prec, "--op", a.op]
+        child = ["--pmc-child", "--no-traffic", "--no-cpu", "--no-check", "--no-compare", "--steps", "2",
+                 "--warmup", "1", "--n", str(n), "--prec", a.prec, "--op", a.op]
         nc = 3 if a.op == "curl" else 1
         traffic, detail = measure_traffic(child, lambda name: is_zpass_kernel(name, nc))
         out["roofline"]["traffic"] = traffic
         out["roofline"]["traffic_detail"] = detail
     if not a.no_check:
         from scipy.special import erf
-        lhs = [np.empty_like(b) for _ in range(cout)]
-        s.pull(*lhs)
         if a.prec != "fp64":
-            r = P.poisson_solver(P.PS_FP64)
-            r.setup3d((n,) * 3, (0, 0, 0), (h,) * 3, 2)
-            r.set_return_curl(a.op == "curl")
-            r.push(*srcs)
-            r.solve3d()
-            ref = [np.empty_like(b) for _ in range(cout)]
-            r.pull(*ref)
-            r.close()
-            num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(lhs, ref))
-            den = sum(float(np.sum(y ** 2)) for y in ref)
-            out["rel_l2_vs_fp64_gpu"] = math.sqrt(num / den)
+            err, lhs = rel_to_fp64(s)
+            out["rel_l2_vs_fp64_gpu"] = err
+        else:
+            lhs = [np.empty_like(b) for _ in range(cout)]
+            s.pull(*lhs)
         if a.op == "plain":
             x = -1.0 + (np.arange(n) + 0.5) * h
             r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
@@ -365,6 +393,19 @@ def main(argv=None):
             A = erf(rr / (math.sqrt(2) * se)) / (4 * np.pi * rr)
             out["rel_l2_vs_analytic"] = float(np.sqrt(np.sum((lhs[0] - A) ** 2) / np.sum(A ** 2)))
     s.close()
+    # the plain FP32 GPU FFT path, timed beside the mixed form on the same input
+    if a.prec == "mixed" and not (a.no_compare or a.pmc_child):
+        p32, w32, s32, zv32 = run("fp32")
+        if p32 is not None:
+            f = {"value": 1000.0 / p32["solve"], "unit": "solves/s", "ms_per_step": p32["solve"],
+                 "z_kernel": zv32, "pass_ms": {k: p32[k] for k in ("x", "y", "z", "y_inv", "x_inv")},
+                 "z_achieved_gbs": B[dom] / (p32[dom] * 1e-3) / 1e9}
+            if not a.no_check:
+                f["rel_l2_vs_fp64_gpu"] = rel_to_fp64(s32)[0]
+            out["fp32_path"] = f
+            out["speedup_vs_fp32"] = p32["solve"] / ms
+            out["z_speedup_vs_fp32"] = p32[dom] / prof[dom]
+        s32.close()
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, cin, a.op)
     print(json.dumps(out), flush=True)
