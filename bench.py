@@ -11,6 +11,7 @@ Prints one JSON line.
 
   python bench.py [--steps K] [--warmup W] [--prec mixed|fp32|fp64]
                   [--n 512] [--op plain|curl] [--no-cpu] [--no-check] [--no-compare]
+  python bench.py --workload ppu256|vortex512|large1024   (configurations 2, 3, 5)
   python bench.py --workload batch [--nprob 64] [--autotune]
       64 independent unbounded scalar problems, n from {32,...,192} (seed 4),
       sources = sums of 1-4 Gaussian blobs; one step = one hipGraph launch
@@ -179,6 +180,183 @@ def batch_problems(nprob, seed=4):
     return probs, srcs
 
 
+def separable_blobs(n, lo, hi, centres, widths, amps, nz_first=True):
+    """sum_b amp_b exp(-|x - c_b|^2 / (2 w_b^2)) on an n^3 cell-centred grid of
+    [lo, hi]^3, built plane by plane as rank-B products (x fastest)."""
+    h = (hi - lo) / n
+    x = lo + (np.arange(n) + 0.5) * h
+    gx = np.exp(-(x[None, :] - centres[:, 0:1]) ** 2 / (2 * widths[:, None] ** 2))
+    gy = np.exp(-(x[None, :] - centres[:, 1:2]) ** 2 / (2 * widths[:, None] ** 2))
+    gz = np.exp(-(x[None, :] - centres[:, 2:3]) ** 2 / (2 * widths[:, None] ** 2))
+    out = np.zeros((n, n, n))
+    cut = 1e-18 * float(np.max(np.abs(amps)))      # below FP64 round-off of the largest term
+    for k in range(n):
+        wk = amps * gz[:, k]
+        live = np.abs(wk) > cut
+        if live.any():
+            out[k] = (gy[live].T * wk[live]) @ gx[live]
+    return out, h
+
+
+def vortex_ring_vorticity(n, seed=3, R=0.4, sc=0.05, nseg=2048):
+    """Vorticity of a thin vortex ring on [-1,1]^3: circulation 1, Gaussian core
+    sc, centreline R(t) = R (1 + 0.01 cos(5 t + phi)) in the z = 0 plane with a
+    seeded phase phi.  Each of nseg centreline segments contributes a 3D
+    Gaussian of width sc times its tangent vector, so the sum is a tube with a
+    Gaussian cross-section (2D integral 1) directed along the centreline."""
+    rng = np.random.default_rng(seed)
+    phi = rng.uniform(0, 2 * np.pi)
+    t = (np.arange(nseg) + 0.5) * 2 * np.pi / nseg
+    r = R * (1 + 0.01 * np.cos(5 * t + phi))
+    dr = -R * 0.01 * 5 * np.sin(5 * t + phi)
+    X = np.stack([r * np.cos(t), r * np.sin(t), np.zeros_like(t)], 1)
+    dX = np.stack([dr * np.cos(t) - r * np.sin(t), dr * np.sin(t) + r * np.cos(t), np.zeros_like(t)], 1)
+    dX *= 2 * np.pi / nseg                                   # tangent * dt
+    norm = 1.0 / ((2 * np.pi) ** 1.5 * sc ** 3)
+    w = np.full(nseg, sc)
+    comps = []
+    for c in range(2):
+        f, h = separable_blobs(n, -1.0, 1.0, X, w, norm * dX[:, c])
+        comps.append(f)
+    comps.append(np.zeros_like(comps[0]))
+    return comps, h
+
+
+def ppu_source(n, seed=2):
+    """x, y periodic on [0,1)^2 (h = 1/n), z in [-1,1] (cell-centred, 2/n):
+    sin(2 pi x) sin(4 pi y) exp(-z^2 / (2 0.05^2)) plus a seeded random-phase
+    field in x, y (spectral amplitude exp(-|k|^2 / (2 16^2)), (0,0) mode zero,
+    unit RMS) times the same z envelope."""
+    rng = np.random.default_rng(seed)
+    h = 1.0 / n
+    x = np.arange(n) * h
+    hz = 2.0 / n
+    z = -1.0 + (np.arange(n) + 0.5) * hz
+    env = np.exp(-z ** 2 / (2 * 0.05 ** 2))
+    k = np.fft.fftfreq(n, d=1.0 / n)
+    kk = k[None, :] ** 2 + k[:, None] ** 2
+    amp = np.exp(-kk / (2 * 16.0 ** 2))
+    amp[0, 0] = 0.0
+    spec = amp * np.exp(2j * np.pi * rng.uniform(size=(n, n)))
+    field = np.real(np.fft.ifft2(spec))
+    field /= np.sqrt(np.mean(field ** 2))
+    plane = np.sin(2 * np.pi * x)[None, :] * np.sin(4 * np.pi * x)[:, None] + field   # [y][x]
+    return env[:, None, None] * plane[None, :, :], (h, h, hz)
+
+
+def _profile(s, steps, warmup):
+    for _ in range(warmup):
+        s.solve3d()
+    s.synchronize()
+    return s.profile_solve(steps)
+
+
+def _rel(a, b):
+    num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b))
+    den = sum(float(np.sum(y ** 2)) for y in b)
+    return math.sqrt(num / den)
+
+
+def _rel_to(a, b, scale):
+    num = sum(float(np.sum((x - y) ** 2)) for x, y in zip(a, b))
+    return math.sqrt(num / sum(float(np.sum(y ** 2)) for y in scale))
+
+
+def bench_config(a):
+    """Named workloads of the other configurations (one JSON line each):
+      ppu256     x,y periodic / z unbounded 256^3, reg 6: A and grad A
+      vortex512  512^3 unbounded vortex ring: streamfunction (3 components),
+                 velocity by spectral curl, div of the streamfunction
+      large1024  1024^3 unbounded, reg 10, 256 blobs: A and grad A (FP32 and
+                 mixed only; FP64 check on every 4th point per direction)
+    A step is the workload's solves in sequence; each solve times its passes
+    with HIP events."""
+    import poissonsolver_amd as P
+    codes = {"fp32": P.PS_FP32, "mixed": P.PS_MIXED, "fp64": P.PS_FP64}
+    precs = ["mixed", "fp32"] if a.prec == "mixed" else [a.prec]
+    if a.workload == "ppu256":
+        n = a.n if a.n != 512 else 256
+        src, dx = ppu_source(n)
+        ncell, bc, reg = (n, n, n), (1, 1, 0), 6
+        solves = [("A", "plain", [src]), ("gradA", "grad", [src, src, src])]
+        cin_out = [(1, 1), (3, 3)]
+        desc = "%d^3 x,y periodic z unbounded (doubled in z only), reg 6, sin*sin*gauss + random-phase source: A and grad A" % n
+    elif a.workload == "vortex512":
+        n = a.n
+        comps, h = vortex_ring_vorticity(n)
+        ncell, bc, reg, dx = (n, n, n), (0, 0, 0), 2, (h, h, h)
+        solves = [("psi", "plain", comps), ("u", "curl", comps), ("divpsi", "div", comps)]
+        cin_out = [(3, 3), (3, 3), (3, 1)]
+        desc = "%d^3 fully unbounded vortex ring (R 0.4, core 0.05, mode-5 1%%): streamfunction, velocity by curl, div psi" % n
+    else:
+        n = a.n if a.n != 512 else 1024
+        rng = np.random.default_rng(5)
+        nb = 256
+        cen = rng.uniform(-0.8, 0.8, (nb, 3))
+        wid = rng.uniform(0.01, 0.03, nb)
+        amp = rng.standard_normal(nb)
+        src, h = separable_blobs(n, -1.0, 1.0, cen, wid, amp)
+        ncell, bc, reg, dx = (n, n, n), (0, 0, 0), 10, (h, h, h)
+        solves = [("A", "plain", [src]), ("gradA", "grad", [src, src, src])]
+        cin_out = [(1, 1), (3, 3)]
+        desc = "%d^3 fully unbounded, reg 10, 256 Gaussian blobs: A and grad A" % n
+    sub = (slice(None, None, 4),) * 3 if a.workload == "large1024" else (slice(None),) * 3
+    results = {}
+    outs = {}
+    for prec in precs + ["fp64"]:
+        per = {}
+        for name, op, rhs in solves:
+            s = P.poisson_solver(codes[prec])
+            s.setup3d(ncell, bc, dx, reg)
+            s.set_return_grad(op == "grad")
+            s.set_return_div(op == "div")
+            s.set_return_curl(op == "curl")
+            s.push(*rhs)
+            if prec != "fp64":
+                prof = _profile(s, a.steps, a.warmup)
+                per[name] = {"ms": prof["solve"], "pass_ms": {k: prof[k] for k in ("x", "y", "z", "y_inv", "x_inv")},
+                             "z_kernel": s.zpass_variant()}
+            else:
+                s.solve3d()
+            nout = 1 if op == "div" else len(rhs) if op != "plain" else len(rhs)
+            lhs = [np.empty_like(rhs[0]) for _ in range(nout)]
+            s.pull(*lhs)
+            outs[(prec, name)] = [x[sub].copy() for x in lhs]
+            del lhs
+            s.close()
+        results[prec] = per
+    line_prec = precs[0]
+    step_ms = sum(v["ms"] for v in results[line_prec].values())
+    out = {
+        "metric": METRIC, "value": 1000.0 / step_ms, "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": line_prec, "data": "synthetic",
+        "config": {"workload": a.workload + ": " + desc, "ncell": list(ncell), "bc": list(bc), "reg_order": reg,
+                   "step": [nm for nm, _, _ in solves]},
+        "solves": {}, "rel_l2_vs_fp64_gpu": {},
+    }
+    for prec in precs:
+        for name, op, _ in solves:
+            r = results[prec][name]
+            key = "%s/%s" % (prec, name)
+            out["solves"][key] = r
+            if name == "divpsi":
+                # div psi is ~0: its error is measured against the velocity scale
+                out["rel_l2_vs_fp64_gpu"][key + " (vs |u|)"] = _rel_to(outs[(prec, name)], outs[("fp64", name)],
+                                                                     outs[("fp64", "u")])
+            else:
+                out["rel_l2_vs_fp64_gpu"][key] = _rel(outs[(prec, name)], outs[("fp64", name)])
+    if len(precs) == 2:
+        out["speedup_vs_fp32"] = sum(v["ms"] for v in results["fp32"].values()) / step_ms
+    if a.workload == "vortex512":
+        # div psi relative to the velocity magnitude scale (L2 ratio, FP64)
+        dv = outs[("fp64", "divpsi")]
+        uu = outs[("fp64", "u")]
+        out["div_psi_over_u_l2"] = math.sqrt(float(np.sum(dv[0] ** 2)) / sum(float(np.sum(x ** 2)) for x in uu))
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_batch(a):
     import poissonsolver_amd as P
 
@@ -268,7 +446,7 @@ def main(argv=None):
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes that fill roofline.traffic")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", default="single", choices=["single", "batch"])
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "ppu256", "vortex512", "large1024"])
     ap.add_argument("--nprob", type=int, default=64)
     ap.add_argument("--autotune", action="store_true", help="batch: autotune each distinct size first")
     a = ap.parse_args(argv)
@@ -282,6 +460,8 @@ def main(argv=None):
             return 0
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload in ("ppu256", "vortex512", "large1024"):
+        return bench_config(a)
 
     import poissonsolver_amd as P
 
