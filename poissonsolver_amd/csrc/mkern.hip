@@ -78,6 +78,30 @@ template <int OPK> static hipError_t launch_zmfma_op(PencilArgs<float> a, int pe
     return hipGetLastError();
 }
 
+template <int OPK> static hipError_t launch_zmfma3_op(PencilArgs<float> a, hipStream_t s)
+{
+    const size_t shm = MfmaZ3::lds_bytes();
+    hipError_t e = allow_lds(k_zmfma1024v<OPK>, shm);
+    if (e != hipSuccess) return e;
+    a.ncoltile = (a.ncol + MfmaZ3::P - 1) / MfmaZ3::P;
+    const long nwork = (long)a.ncoltile * a.nslab;
+    hipLaunchKernelGGL((k_zmfma1024v<OPK>), dim3((unsigned)nwork), dim3(64 * MfmaZ3::NW), shm, s, a);
+    return hipGetLastError();
+}
+
+bool zmfma3_supported(int L, int half, int op)
+{
+    return L == MfmaZ3::L && half && (op == OP_DIV || op == OP_CURL);
+}
+
+hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s)
+{
+    if (a.plan.L != MfmaZ3::L || a.n_in != MfmaZ3::L / 2 || a.n_out != MfmaZ3::L / 2) return hipErrorInvalidValue;
+    if (a.op == OP_DIV) return launch_zmfma3_op<OP_DIV>(a, s);
+    if (a.op == OP_CURL) return launch_zmfma3_op<OP_CURL>(a, s);
+    return hipErrorInvalidValue;
+}
+
 bool zmfma_supported(int L, int half, int op)
 {
     return L == MfmaZ::L && half && (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF);

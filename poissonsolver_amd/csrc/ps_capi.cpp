@@ -357,6 +357,9 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, op)) {
             CK(launch_zmfma(za, c->tune.mfma == 2, s));
             mfma_done = true;
+        } else if (c->zmfma && nc == 3 && zmfma3_supported(zp.p.L, c->bc[2] == 0, op)) {
+            CK(launch_zmfma3(za, s));
+            mfma_done = true;
         }
     }
     if (mfma_done) {

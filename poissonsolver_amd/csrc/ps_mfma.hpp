@@ -71,39 +71,31 @@ struct Cos32 {
 };
 
 // ---------------------------------------------------------------------------
-// fp16 hi/lo split of scaled fp32 values with the mixed-precision FMA:
+// fp16 hi/lo split of scaled fp32 values:
 //   hi = f16(x s),  lo = f16(x s - hi)   (x s - hi is exact before rounding)
+// Written as ordinary arithmetic (the compiler forms v_cvt_pk_f16_f32 and
+// v_fma_mix_f32 from it) rather than inline assembly: the compiler tracks
+// the wait states between matrix-core instructions and the vector
+// instructions that read their results or overwrite their accumulator
+// inputs only for instructions it generates itself.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ unsigned mf_hi_pair(float x, float y, float sx, float sy)
+typedef _Float16 mf_h2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void mf_split_pair(float x, float y, float sx, float sy, unsigned& hi, unsigned& lo)
 {
-    unsigned h;
-    asm("v_fma_mixlo_f16 %0, %1, %2, 0 op_sel_hi:[0,0,0]" : "=v"(h) : "v"(x), "v"(sx));
-    asm("v_fma_mixhi_f16 %0, %1, %2, 0 op_sel_hi:[0,0,0]" : "+v"(h) : "v"(y), "v"(sy));
-    return h;
+    const float a = x * sx, b = y * sy;
+    const mf_h2 h = mf_h2{(_Float16)a, (_Float16)b};
+    const mf_h2 r = mf_h2{(_Float16)__builtin_fmaf(x, sx, -(float)h.x), (_Float16)__builtin_fmaf(y, sy, -(float)h.y)};
+    hi = __builtin_bit_cast(unsigned, h);
+    lo = __builtin_bit_cast(unsigned, r);
 }
 
-__device__ __forceinline__ unsigned mf_lo_pair(float x, float y, float sx, float sy, unsigned h)
+__device__ __forceinline__ void mf_split_pair1(float x, float y, unsigned& hi, unsigned& lo)
 {
-    unsigned r;
-    asm("v_fma_mixlo_f16 %0, %1, %2, -%3 op_sel:[0,0,0] op_sel_hi:[0,0,1]" : "=v"(r) : "v"(x), "v"(sx), "v"(h));
-    asm("v_fma_mixhi_f16 %0, %1, %2, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(r) : "v"(y), "v"(sy), "v"(h));
-    return r;
-}
-
-// unscaled pair: hi = f16(x), f16(y) (one packed conversion), lo as above
-__device__ __forceinline__ unsigned mf_hi_pair1(float x, float y)
-{
-    unsigned h;
-    asm("v_cvt_pk_f16_f32 %0, %1, %2" : "=v"(h) : "v"(x), "v"(y));
-    return h;
-}
-
-__device__ __forceinline__ unsigned mf_lo_pair1(float x, float y, unsigned h)
-{
-    unsigned r;
-    asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2 op_sel:[0,0,0] op_sel_hi:[0,0,1]" : "=v"(r) : "v"(x), "v"(h));
-    asm("v_fma_mixhi_f16 %0, %1, 1.0, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(r) : "v"(y), "v"(h));
-    return r;
+    const mf_h2 h = mf_h2{(_Float16)x, (_Float16)y};
+    const mf_h2 r = mf_h2{(_Float16)(x - (float)h.x), (_Float16)(y - (float)h.y)};
+    hi = __builtin_bit_cast(unsigned, h);
+    lo = __builtin_bit_cast(unsigned, r);
 }
 
 struct MfFrag {
@@ -116,10 +108,7 @@ __device__ __forceinline__ MfFrag mf_split4(const float2 (&v)[4], float s, float
 {
     unsigned H[4], Lo[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        H[e] = mf_hi_pair(v[e].x, v[e].y, s, sim);
-        Lo[e] = mf_lo_pair(v[e].x, v[e].y, s, sim, H[e]);
-    }
+    for (int e = 0; e < 4; ++e) mf_split_pair(v[e].x, v[e].y, s, sim, H[e], Lo[e]);
     MfFrag f;
     f.hi = __builtin_bit_cast(mf_h8, H);
     f.lo = __builtin_bit_cast(mf_h8, Lo);
@@ -131,10 +120,7 @@ __device__ __forceinline__ MfFrag mf_split4u(const float2 (&v)[4])
 {
     unsigned H[4], Lo[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-        H[e] = mf_hi_pair1(v[e].x, v[e].y);
-        Lo[e] = mf_lo_pair1(v[e].x, v[e].y, H[e]);
-    }
+    for (int e = 0; e < 4; ++e) mf_split_pair1(v[e].x, v[e].y, H[e], Lo[e]);
     MfFrag f;
     f.hi = __builtin_bit_cast(mf_h8, H);
     f.lo = __builtin_bit_cast(mf_h8, Lo);
@@ -749,6 +735,230 @@ __global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
             }
         }
         T = T1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Three-component operators (div, curl) of the fused z pass on the matrix
+// cores, L = 1024.  Workgroup: P3 = 4 pencils, 4 wavefronts; an MFMA column
+// is (pencil p = j & 3, butterfly b = 4 blk + (j >> 2)).  Each input
+// component runs forward stages 1 and 2 through one LDS exchange buffer and
+// its spectrum stays in registers (three per lane); each output's k-space
+// combination then runs inverse stages 1 and 2 through the same buffer.
+// Per-pencil scales per component (forward) and per output (inverse stage
+// 2), per-column scales for inverse stage 1, as in the one-component kernel.
+// ---------------------------------------------------------------------------
+struct MfmaZ3 {
+    static constexpr int L = 1024;
+    static constexpr int P = 4;          // pencils per workgroup
+    static constexpr int NW = 4;         // wavefronts per workgroup
+    static constexpr int NB = 16 / P;    // butterflies per MFMA block
+    static constexpr int RS = 33;        // LDS row stride: 1 (mod 16)
+    static constexpr int PS = 32 * 33 + 4;   // LDS pencil stride: 4 (mod 16), so 4p + bsub is distinct
+    static constexpr size_t lds_bytes()
+    {
+        return (size_t)P * PS * 8 + (size_t)L * 8 + 6 * P * 4;
+    }
+};
+
+template <int OPK>
+__global__ void __launch_bounds__(256, 2) k_zmfma1024v(PencilArgs<float> a)
+{
+    using M = MfmaZ3;
+    constexpr int P = M::P, RS = M::RS, PSTR = M::PS, L = M::L, NB = M::NB, W = M::NW;
+    constexpr int NOUT = (OPK == OP_DIV) ? 1 : 3;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* sd = reinterpret_cast<float2*>(smem_raw);
+    float2* stw = sd + P * PSTR;
+    unsigned* smax = reinterpret_cast<unsigned*>(stw + L);   // [3][P] forward, [3][P] inverse
+
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, l = tid & 63, q = l >> 4, j = l & 15;
+    const int p = j % P, bsub = j / P;
+    const int bp32 = (l ^ 32) << 2;
+
+    const long nblk = gridDim.x;
+    const long lin = a.xcd_remap ? xcd_remap(blockIdx.x, nblk) : (long)blockIdx.x;
+    const int slab0 = (int)(lin / a.ncoltile);
+    const int ct = (int)(lin - (long)slab0 * a.ncoltile);
+    const int slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    const int col = ct * P + p;
+    const bool cv = col < a.ncol;
+    const int colr = cv ? col : a.ncol - 1;
+
+    for (int i = tid; i < L; i += 64 * W) stw[i] = a.plan.tw[i];
+    if (tid < 6 * P) smax[tid] = 0u;
+    const MfDft32 F = mf_dft32_frags(a.mfrag, l);
+
+    // uniform bases and 32-bit byte offsets (host-checked below 4 GB)
+    const char* ibase = reinterpret_cast<const char*>(a.in);
+    char* obase = reinterpret_cast<char*>(a.out);
+    const unsigned icol = (unsigned)(colr * a.ia.col + boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi)) * 8u;
+    const unsigned ocol = (unsigned)(col * a.oa.col + boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi)) * 8u;
+    const unsigned is32 = (unsigned)mf_pos_stride32(a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) * 8u;
+    const unsigned os32 = (unsigned)mf_pos_stride32(a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) * 8u;
+    float2* myp = sd + p * PSTR;
+    int bb[2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) bb[bi] = NB * (w + W * bi) + bsub;
+
+    // Green's octant values and wavenumbers (Hermitian parts: Nyquist 0)
+    const TileK<float> tk = tile_k(a, col, slab, cv);
+    const int gkz = (int)a.g_kz;
+    float gk[2][8];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+        mf_kfactors<OP_PLAIN>(gk[bi], a, tk, 0, bb[bi], q, (bb[bi] + 128 * q) * gkz * 4,
+                              (M::L / 2 - bb[bi] - 128 * q) * gkz * 4, 32 * gkz * 4);
+    const float kzc = (float)(2.0 * kPi * a.dk[2]);
+
+    // ---- forward: each input component through stages 1 and 2 ----
+    float2 X[3][2][8];
+    float xinv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float2 x[2][4];
+        float mx = 0.0f;
+        const char* cb = ibase + (long)c * a.ia.comp * 8;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const unsigned o0 = icol + (unsigned)boff(bb[bi], a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) * 8u +
+                                (unsigned)(4 * q) * is32;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                x[bi][e] = *reinterpret_cast<const float2*>(cb + (o0 + (unsigned)e * is32));
+                mx = mf_absmax(x[bi][e], mx);
+            }
+        }
+        __syncthreads();      // tables and slots ready (c = 0); previous stage-2 reads done (c > 0)
+        atomicMax(&smax[c * P + p], __float_as_uint(mx));
+        __syncthreads();
+        float s0, inv0;
+        mf_scale(__uint_as_float(smax[c * P + p]), s0, inv0);
+        xinv[c] = inv0;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int b = bb[bi];
+            MfFrag B[2];
+            B[0] = mf_split4(x[bi], s0, s0);
+            const MfAcc d = mf_dft32_rows32i<1>(F, B);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int k1 = 4 * q + e + 16 * h;
+                    const float2 y = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                    myp[k1 * RS + b] = cmul(y, stw[(b * k1) & (L - 1)]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const float2* row = myp + bb[bi] * RS;
+            float2 v0[4], v1[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v0[e] = row[4 * q + e];
+                v1[e] = row[16 + 4 * q + e];
+            }
+            MfFrag B[2];
+            B[0] = mf_split4u(v0);
+            B[1] = mf_split4u(v1);
+            const MfAcc d = mf_dft32_rows32i<2>(F, B);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                X[c][bi][e] = float2{d.r0[e], d.i0[e]};
+                X[c][bi][4 + e] = float2{d.r1[e], d.i1[e]};
+            }
+        }
+    }
+
+    // ---- each output: k-space combination, inverse stages 1 and 2 ----
+#pragma unroll
+    for (int o = 0; o < NOUT; ++o) {
+        __syncthreads();      // previous output's stage-2 reads (or forward stage-2 reads) done
+        float m3 = 0.0f;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int bp = bb[bi];
+            float2 z[2][4];
+            float mz = 0.0f;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int kz = bp + 32 * (4 * q + e + 16 * h);
+                    const int kzs = (kz < M::L / 2) ? kz : kz - M::L;
+                    const float kzh = (kz == M::L / 2) ? 0.0f : (float)kzs * kzc;
+                    const float g = gk[bi][4 * h + e];
+                    const float2 sx = X[0][bi][4 * h + e], sy = X[1][bi][4 * h + e], sz = X[2][bi][4 * h + e];
+                    const float fx = xinv[0], fy = xinv[1], fz = xinv[2];
+                    float2 t;   // k-weighted sum before g * i
+                    if constexpr (OPK == OP_DIV) {
+                        const float ax = tk.kxh * fx, ay = tk.kyh * fy, az = kzh * fz;
+                        t = float2{ax * sx.x + ay * sy.x + az * sz.x, ax * sx.y + ay * sy.y + az * sz.y};
+                    } else if (o == 0) {
+                        const float ay = tk.kyh * fz, az = kzh * fy;
+                        t = float2{ay * sz.x - az * sy.x, ay * sz.y - az * sy.y};
+                    } else if (o == 1) {
+                        const float az = kzh * fx, ax = tk.kxh * fz;
+                        t = float2{az * sx.x - ax * sz.x, az * sx.y - ax * sz.y};
+                    } else {
+                        const float ax = tk.kxh * fy, ay = tk.kyh * fx;
+                        t = float2{ax * sy.x - ay * sx.x, ax * sy.y - ay * sx.y};
+                    }
+                    z[h][e] = float2{-g * t.y, g * t.x};   // g * i * t
+                    mz = mf_absmax(z[h][e], mz);
+                }
+            }
+            float s2, inv2;
+            mf_scale(mf_colmax(mz, bp32), s2, inv2);
+            MfFrag B[2];
+            B[0] = mf_split4(z[0], s2, -s2);
+            B[1] = mf_split4(z[1], s2, -s2);
+            const MfAcc d = mf_dft32_rows32i<2>(F, B);
+            float2* row = myp + bp * RS;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int n2 = 4 * q + e + 16 * h;
+                    const float2 u = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                    const float2 t = cmul(u, stw[(n2 * bp) & (L - 1)]);
+                    const float2 r = float2{t.x * inv2, -t.y * inv2};
+                    row[n2] = r;
+                    m3 = mf_absmax(r, m3);
+                }
+            }
+        }
+        atomicMax(&smax[3 * P + o * P + p], __float_as_uint(m3));
+        __syncthreads();
+        float s3, inv3;
+        mf_scale(__uint_as_float(smax[3 * P + o * P + p]), s3, inv3);
+        char* ob = obase + (long)o * a.oa.comp * 8;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int n2 = bb[bi];
+            float2 v0[4], v1[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v0[e] = myp[(4 * q + e) * RS + n2];
+                v1[e] = myp[(16 + 4 * q + e) * RS + n2];
+            }
+            MfFrag B[2];
+            B[0] = mf_split4(v0, s3, -s3);
+            B[1] = mf_split4(v1, s3, -s3);
+            const MfAcc d = mf_dft32_rows16(F, B);
+            if (cv) {
+                const unsigned o0 = ocol + (unsigned)boff(n2, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) * 8u +
+                                    (unsigned)(4 * q) * os32;
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    *reinterpret_cast<float2*>(ob + (o0 + (unsigned)e * os32)) =
+                        float2{d.r0[e] * inv3, -d.i0[e] * inv3};
+            }
+        }
     }
 }
 

@@ -17,6 +17,11 @@ bool zmfma_supported(int L, int half, int op);
 // persistent workgroup per CU that prefetches its next tile
 hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t s);
 
+// three-component operators (div, curl) of the fused z pass on the matrix
+// cores: inputs are components 0..2 of `a.in`, outputs components of `a.out`
+bool zmfma3_supported(int L, int half, int op);
+hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s);
+
 // per-lane fp16 hi/lo fragments of the 32-point DFT matrix that the
 // matrix-core kernels multiply by (binary16 bit patterns, kZmfmaFragHalves
 // entries; PencilArgs::mfrag points at a device copy)

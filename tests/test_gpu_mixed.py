@@ -107,3 +107,25 @@ def test_mixed_other_lengths_use_fp32_kernels():
     assert _variant(n, bc, dx, 2) != "matrix-core"
     b = random_rhs(n, 37)
     _compare(n, bc, dx, 2, "curl", b, tol=2e-6)
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("op", ["curl", "div"])
+def test_mixed_three_component_operators(case, op):
+    # curl and div: the fused three-component matrix-core z kernel
+    n, bc, dx = case
+    announce("mixed", op, n, bc)
+    b = random_rhs(n, 41)
+    _compare(n, bc, dx, 2, op, b)
+
+
+def test_mixed_curl_with_reprojection_and_filtered_rhs():
+    # the reprojected rhs filter (three-component, reg > 0) runs the FP32
+    # kernels; the curl itself runs on the matrix cores
+    n, bc, dx = (8, 6, 512), (0, 0, 0), (0.1, 0.12, 2 / 512)
+    announce("mixed curl + reprojection", n)
+    b = random_rhs(n, 43)
+    lo, ro = O.solve(n, bc, dx, 2, b, op="curl", reproject=True)
+    lg, rg = gpu_solve(n, bc, dx, 2, b, op="curl", reproject=True, prec="mixed", want_rhs=True)
+    assert O.rel_l2(lg, lo) <= MIXED_TOL
+    assert O.rel_l2(rg, ro) <= 2e-6
