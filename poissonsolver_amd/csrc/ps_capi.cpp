@@ -291,6 +291,8 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
     a.reg = regm > 0 ? regm : 0;
     a.invN = 1.0 / ((double)c->M[0] * c->M[1] * c->M[2]);
     a.mfrag = c->mfrag;
+    a.fone = 1.0f;
+    a.fzero = 0.0f;
 }
 
 // The five passes.  `src` holds nin real components starting at in_first

@@ -68,6 +68,9 @@ template <typename T> struct PencilArgs {
     // matrix-core z pass: per-lane fp16 hi/lo fragments of the 32-point DFT
     // matrix (64 lanes x 8 fragments x 8 halves), built once at setup
     const void* mfrag;
+    // 1 and 0 supplied at run time: the compiler cannot fold them, so the
+    // FP16 split forms each conversion as one mixed-precision FMA
+    float fone, fzero;
 };
 
 // x pass: two real rows (a, b) per complex pencil.

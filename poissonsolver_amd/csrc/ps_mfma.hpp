@@ -81,19 +81,21 @@ struct Cos32 {
 // ---------------------------------------------------------------------------
 typedef _Float16 mf_h2 __attribute__((ext_vector_type(2)));
 
-__device__ __forceinline__ void mf_split_pair(float x, float y, float sx, float sy, unsigned& hi, unsigned& lo)
+// z: a run-time zero (see PencilArgs::fzero)
+__device__ __forceinline__ void mf_split_pair(float x, float y, float sx, float sy, float z, unsigned& hi,
+                                              unsigned& lo)
 {
-    const float a = x * sx, b = y * sy;
-    const mf_h2 h = mf_h2{(_Float16)a, (_Float16)b};
+    const mf_h2 h = mf_h2{(_Float16)__builtin_fmaf(x, sx, z), (_Float16)__builtin_fmaf(y, sy, z)};
     const mf_h2 r = mf_h2{(_Float16)__builtin_fmaf(x, sx, -(float)h.x), (_Float16)__builtin_fmaf(y, sy, -(float)h.y)};
     hi = __builtin_bit_cast(unsigned, h);
     lo = __builtin_bit_cast(unsigned, r);
 }
 
-__device__ __forceinline__ void mf_split_pair1(float x, float y, unsigned& hi, unsigned& lo)
+// one: a run-time 1 (see PencilArgs::fone)
+__device__ __forceinline__ void mf_split_pair1(float x, float y, float one, unsigned& hi, unsigned& lo)
 {
     const mf_h2 h = mf_h2{(_Float16)x, (_Float16)y};
-    const mf_h2 r = mf_h2{(_Float16)(x - (float)h.x), (_Float16)(y - (float)h.y)};
+    const mf_h2 r = mf_h2{(_Float16)__builtin_fmaf(x, one, -(float)h.x), (_Float16)__builtin_fmaf(y, one, -(float)h.y)};
     hi = __builtin_bit_cast(unsigned, h);
     lo = __builtin_bit_cast(unsigned, r);
 }
@@ -104,11 +106,11 @@ struct MfFrag {
 
 // B fragment of four complex points (re, im interleaved) scaled by s; the
 // imaginary parts are scaled by sim (= -s for the conjugate)
-__device__ __forceinline__ MfFrag mf_split4(const float2 (&v)[4], float s, float sim)
+__device__ __forceinline__ MfFrag mf_split4(const float2 (&v)[4], float s, float sim, float z)
 {
     unsigned H[4], Lo[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) mf_split_pair(v[e].x, v[e].y, s, sim, H[e], Lo[e]);
+    for (int e = 0; e < 4; ++e) mf_split_pair(v[e].x, v[e].y, s, sim, z, H[e], Lo[e]);
     MfFrag f;
     f.hi = __builtin_bit_cast(mf_h8, H);
     f.lo = __builtin_bit_cast(mf_h8, Lo);
@@ -116,11 +118,11 @@ __device__ __forceinline__ MfFrag mf_split4(const float2 (&v)[4], float s, float
 }
 
 // B fragment of four complex points already inside the fp16 range
-__device__ __forceinline__ MfFrag mf_split4u(const float2 (&v)[4])
+__device__ __forceinline__ MfFrag mf_split4u(const float2 (&v)[4], float one)
 {
     unsigned H[4], Lo[4];
 #pragma unroll
-    for (int e = 0; e < 4; ++e) mf_split_pair1(v[e].x, v[e].y, H[e], Lo[e]);
+    for (int e = 0; e < 4; ++e) mf_split_pair1(v[e].x, v[e].y, one, H[e], Lo[e]);
     MfFrag f;
     f.hi = __builtin_bit_cast(mf_h8, H);
     f.lo = __builtin_bit_cast(mf_h8, Lo);
@@ -433,7 +435,7 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
     for (int bi = 0; bi < 2; ++bi) {
         const int b = 2 * (w + 8 * bi) + bsub;
         MfFrag B[2];
-        B[0] = mf_split4(x[bi], s0, s0);
+        B[0] = mf_split4(x[bi], s0, s0, a.fzero);
         const MfAcc d = mf_dft32_rows32i<1>(F, B);
         // twiddle W1024^{b k1}; stored transposed as row k1, column b
 #pragma unroll
@@ -461,8 +463,8 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
             v1[e] = row[16 + 4 * q + e];
         }
         MfFrag B[2];
-        B[0] = mf_split4u(v0);
-        B[1] = mf_split4u(v1);
+        B[0] = mf_split4u(v0, a.fone);
+        B[1] = mf_split4u(v1, a.fone);
         MfAcc d = mf_dft32_rows32i<2>(F, B);
         // s0 X[bp + 32 q'], q' = 4q + e (r0/i0) and 16 + 4q + e (r1/i1);
         // the k-space operator keeps the factor s0
@@ -484,8 +486,8 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
         float s2, inv2;
         mf_scale(mf_colmax(mz, bp32), s2, inv2);
         // inverse: conj(F) z = conj(F conj(z))
-        B[0] = mf_split4(z0, s2, -s2);
-        B[1] = mf_split4(z1, s2, -s2);
+        B[0] = mf_split4(z0, s2, -s2, a.fzero);
+        B[1] = mf_split4(z1, s2, -s2, a.fzero);
         d = mf_dft32_rows32i<2>(F, B);
         // U[n2] = conj of the inverse DFT output (times s0 s2); apply
         // W1024^{-n2 bp} and undo both scales: conj(U W) inv0 inv2, written
@@ -520,8 +522,8 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
             v1[e] = myp[(16 + 4 * q + e) * RS + n2];
         }
         MfFrag B[2];
-        B[0] = mf_split4(v0, s3, -s3);
-        B[1] = mf_split4(v1, s3, -s3);
+        B[0] = mf_split4(v0, s3, -s3, a.fzero);
+        B[1] = mf_split4(v1, s3, -s3, a.fzero);
         const MfAcc d = mf_dft32_rows16(F, B);
         if (cv) {
             const unsigned o0 = ocol + (unsigned)boff(n2, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) * 8u + (unsigned)(4 * q) * os32;
@@ -625,7 +627,7 @@ __global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
         for (int bi = 0; bi < 2; ++bi) {
             const int b = bb[bi];
             MfFrag B[2];
-            B[0] = mf_split4(x[bi], s0, s0);
+            B[0] = mf_split4(x[bi], s0, s0, a.fzero);
             const MfAcc d = mf_dft32_rows32i<1>(F, B);
             // twiddle W1024^{b k1}; stored transposed as row k1, column b
 #pragma unroll
@@ -665,8 +667,8 @@ __global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
                 v1[e] = row[16 + 4 * q + e];
             }
             MfFrag B[2];
-            B[0] = mf_split4u(v0);
-            B[1] = mf_split4u(v1);
+            B[0] = mf_split4u(v0, a.fone);
+            B[1] = mf_split4u(v1, a.fone);
             MfAcc d = mf_dft32_rows32i<2>(F, B);
             // s0 X[bp + 32 q'], q' = 4q + e (r0/i0) and 16 + 4q + e (r1/i1);
             // the k-space operator keeps the factor s0
@@ -690,8 +692,8 @@ __global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
             float s2, inv2;
             mf_scale(mf_colmax(mz, bp32), s2, inv2);
             // inverse: conj(F) z = conj(F conj(z))
-            B[0] = mf_split4(z0, s2, -s2);
-            B[1] = mf_split4(z1, s2, -s2);
+            B[0] = mf_split4(z0, s2, -s2, a.fzero);
+            B[1] = mf_split4(z1, s2, -s2, a.fzero);
             d = mf_dft32_rows32i<2>(F, B);
             // U[n2] = conj of the inverse DFT output (times s0 s2); apply
             // W1024^{-n2 bp} and undo both scales: conj(U W) inv0 inv2, written
@@ -726,8 +728,8 @@ __global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
                 v1[e] = myp[(16 + 4 * q + e) * RS + n2];
             }
             MfFrag B[2];
-            B[0] = mf_split4(v0, s3, -s3);
-            B[1] = mf_split4(v1, s3, -s3);
+            B[0] = mf_split4(v0, s3, -s3, a.fzero);
+            B[1] = mf_split4(v1, s3, -s3, a.fzero);
             const MfAcc d = mf_dft32_rows16(F, B);
             if (T.cv) {
 #pragma unroll
@@ -840,7 +842,7 @@ __global__ void __launch_bounds__(256, 2) k_zmfma1024v(PencilArgs<float> a)
         for (int bi = 0; bi < 2; ++bi) {
             const int b = bb[bi];
             MfFrag B[2];
-            B[0] = mf_split4(x[bi], s0, s0);
+            B[0] = mf_split4(x[bi], s0, s0, a.fzero);
             const MfAcc d = mf_dft32_rows32i<1>(F, B);
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -863,8 +865,8 @@ __global__ void __launch_bounds__(256, 2) k_zmfma1024v(PencilArgs<float> a)
                 v1[e] = row[16 + 4 * q + e];
             }
             MfFrag B[2];
-            B[0] = mf_split4u(v0);
-            B[1] = mf_split4u(v1);
+            B[0] = mf_split4u(v0, a.fone);
+            B[1] = mf_split4u(v1, a.fone);
             const MfAcc d = mf_dft32_rows32i<2>(F, B);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
@@ -915,8 +917,8 @@ __global__ void __launch_bounds__(256, 2) k_zmfma1024v(PencilArgs<float> a)
             float s2, inv2;
             mf_scale(mf_colmax(mz, bp32), s2, inv2);
             MfFrag B[2];
-            B[0] = mf_split4(z[0], s2, -s2);
-            B[1] = mf_split4(z[1], s2, -s2);
+            B[0] = mf_split4(z[0], s2, -s2, a.fzero);
+            B[1] = mf_split4(z[1], s2, -s2, a.fzero);
             const MfAcc d = mf_dft32_rows32i<2>(F, B);
             float2* row = myp + bp * RS;
 #pragma unroll
@@ -947,8 +949,8 @@ __global__ void __launch_bounds__(256, 2) k_zmfma1024v(PencilArgs<float> a)
                 v1[e] = myp[(16 + 4 * q + e) * RS + n2];
             }
             MfFrag B[2];
-            B[0] = mf_split4(v0, s3, -s3);
-            B[1] = mf_split4(v1, s3, -s3);
+            B[0] = mf_split4(v0, s3, -s3, a.fzero);
+            B[1] = mf_split4(v1, s3, -s3, a.fzero);
             const MfAcc d = mf_dft32_rows16(F, B);
             if (cv) {
                 const unsigned o0 = ocol + (unsigned)boff(n2, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) * 8u +
