@@ -100,6 +100,10 @@ int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int
 int ps_eval_special(const double* x, double* y, long n, int which);
 /* enqueue ps_solve3d on the context's stream and return without waiting */
 int ps_solve3d_async(ps_ctx* ctx);
+/* the FP16 hi/lo fragments of the 32-point DFT matrix that the matrix-core
+   kernels multiply by (binary16 bit patterns; n >= 4096 entries, layout
+   [lane 64][K chunk 2][Yr hi, Yr lo, Yi hi, Yi lo][8]) */
+int ps_dft32_fragments(unsigned short* out, int n);
 /* kernel family of the one-component z pass chosen at setup: 0 runtime-plan,
    1 compile-time specialised, 2 pencil-per-wavefront, 3 matrix-core (PS_MIXED);
    negative: not set up */

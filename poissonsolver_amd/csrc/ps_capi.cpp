@@ -477,7 +477,11 @@ template <typename T> int setup_impl(ps_ctx* c)
 
     // pencil-per-wavefront z kernel: specialised one-component plan, pencil
     // inside one wavefront, two or more stages; it reads Gk along kz
-    c->zwave = tn.zwave && c->pz1.stat && (64 % c->pz1.p.TP == 0) && c->pz1.p.nstage >= 2;
+    // (its elements per thread: 16, or 32 for FP32 pencils of 2048 (zwave_E))
+    const int zwE = (c->pz1.p.L > 1024 && c->prec != PS_FP64) ? 32 : 16;
+    c->zwave = tn.zwave && c->pz1.stat && c->pz1.p.L % zwE == 0 && (64 % (c->pz1.p.L / zwE) == 0) &&
+               c->pz1.p.nstage >= 2 && (c->zb <= 0 || (c->pz1.p.L / zwE) % (1 << c->zb) == 0);
+    if (c->zwave && zwE == 32) c->pz1.tile = tn.halfb_z ? kZwaveB32 / 2 : kZwaveB32;
     // matrix-core z pass (PS_MIXED): its lanes read Gk along kx; positions
     // z0 + 32 u must be linear in u (z blocks of at most 32 planes, or plain)
     // 32-bit byte offsets: one component of A2 and the Gk octant below 4 GB
@@ -717,6 +721,13 @@ int ps_solve3d(ps_ctx* c)
     r = (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
     if (r) return r;
     CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+int ps_dft32_fragments(unsigned short* out, int n)
+{
+    if (!out || n < kZmfmaFragHalves) return PS_EINVAL;
+    zmfma_fragments(out);
     return PS_OK;
 }
 

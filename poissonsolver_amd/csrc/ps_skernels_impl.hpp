@@ -239,21 +239,33 @@ __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
 // ordering only; the last exchange (second barrier) returns the coalesced
 // mapping for the final inverse stage and the store.
 // ---------------------------------------------------------------------------
-template <int L, int E, int DIR, int S, int SEND, typename T>
-__device__ __forceinline__ void wstages(cplx<T> (&v)[E], const STw<L, E, T>& W, cplx<T>* reg, int t)
+// TWREG = 1: stage twiddles from the register array W; 0: from the table tw
+template <int L, int E, int DIR, int S, int SEND, int TWREG, typename T>
+__device__ __forceinline__ void wstages(cplx<T> (&v)[E], const STw<L, E, T>& W, const cplx<T>* __restrict__ tw,
+                                        cplx<T>* reg, int t)
 {
     if constexpr (S < SEND) {
-        sstage<L, E, DIR, S, 1, T>(v, W, nullptr, t);
+        sstage<L, E, DIR, S, TWREG, T>(v, W, tw, t);
         if constexpr (S + 1 < SEND) sxchg<L, E, S, 1, T>(v, reg, t, reg, t);
-        wstages<L, E, DIR, S + 1, SEND, T>(v, W, reg, t);
+        wstages<L, E, DIR, S + 1, SEND, TWREG, T>(v, W, tw, reg, t);
     }
+}
+
+// elements per thread of the pencil-per-wavefront z kernel: 16 (stage
+// twiddles in registers); FP32 pencils of 2048 use 32 so that a pencil still
+// fits one wavefront, with stage twiddles read from the table (58 twiddles
+// would not fit in registers next to 32 points)
+template <typename T, int L> constexpr int zwave_E()
+{
+    return (L > 1024 && sizeof(T) == 4) ? 32 : 16;
 }
 
 template <typename T, int L, int B, int HIN, int HOUT>
 __global__ void __launch_bounds__(kMaxThreads, 2) k_zfused_w(PencilArgs<T> a)
 {
     using C = cplx<T>;
-    constexpr int E = 16, TP = L / E;
+    constexpr int E = zwave_E<T, L>(), TP = L / E;
+    constexpr int TWR = (E == 16) ? 1 : 0;
     constexpr int NS = SFft<L, E>::cp.n;
     static_assert(NS >= 2, "two or more stages");
     static_assert(64 % TP == 0, "each pencil inside one wavefront");
@@ -277,7 +289,7 @@ __global__ void __launch_bounds__(kMaxThreads, 2) k_zfused_w(PencilArgs<T> a)
     const bool cvl = col0 + pl < a.ncol, cvc = col0 + pc < a.ncol;
 
     STw<L, E, T> W;
-    load_stwiddles<L, E, T>(W, tc, a.plan.tw);
+    if constexpr (TWR) load_stwiddles<L, E, T>(W, tc, a.plan.tw);
 
     const long iu = (a.ia.pos_sh >= 30) ? TP * a.ia.pos_lo : (long)(TP >> a.ia.pos_sh) * a.ia.pos_hi;
     const long ou = (a.oa.pos_sh >= 30) ? TP * a.oa.pos_lo : (long)(TP >> a.oa.pos_sh) * a.oa.pos_hi;
@@ -295,15 +307,15 @@ __global__ void __launch_bounds__(kMaxThreads, 2) k_zfused_w(PencilArgs<T> a)
             v[u] = x;
         }
     }
-    sstage<L, E, -1, 0, 1, T>(v, W, nullptr, tl);
+    sstage<L, E, -1, 0, TWR, T>(v, W, a.plan.tw, tl);
     sxchg<L, E, 0, 0, T>(v, lds + pl * LP, tl, lds + pc * LP, tc);
     C* reg = lds + pc * LP;
-    wstages<L, E, -1, 1, NS, T>(v, W, reg, tc);
+    wstages<L, E, -1, 1, NS, TWR, T>(v, W, a.plan.tw, reg, tc);
 
     const TileK<T> tk = tile_k(a, col0 + pc, slab, cvc);
     kspace_op_inplace<T, E>(v, a, tk, comp, tc, TP);
 
-    wstages<L, E, +1, 0, NS - 1, T>(v, W, reg, tc);
+    wstages<L, E, +1, 0, NS - 1, TWR, T>(v, W, a.plan.tw, reg, tc);
     sxchg<L, E, NS - 2, 0, T>(v, reg, tc, lds + pl * LP, tl);
     sstage<L, E, +1, NS - 1, 0, T>(v, W, a.plan.tw, tl);
 
@@ -321,7 +333,7 @@ hipError_t s_launch_zwave_b(const PencilArgs<T>& a, int half, hipStream_t s)
 {
     const size_t shm = (size_t)B * lds_len(L) * sizeof(cplx<T>);
     const long nwork = (long)a.ncoltile * a.nslab * a.ncomp;
-    constexpr int TP = L / 16;
+    constexpr int TP = L / zwave_E<T, L>();
     hipError_t e;
     if (half) {
         if ((e = allow_lds(k_zfused_w<T, L, B, 1, 1>, shm)) != hipSuccess) return e;
@@ -336,7 +348,15 @@ hipError_t s_launch_zwave_b(const PencilArgs<T>& a, int half, hipStream_t s)
 template <typename T, int L> hipError_t s_launch_zwave(const PencilArgs<T>& a, int half, hipStream_t s)
 {
     using Pol = SPolicy<T, L>;
-    if constexpr (64 % (L / 16) == 0 && SFft<L, 16>::cp.n >= 2) {
+    constexpr int EW = zwave_E<T, L>();
+    if constexpr (EW == 32) {
+        // 2 wavefronts per SIMD (registers): one workgroup of 8 pencils per CU
+        // (8 x 16.9 KB LDS) reads 64-byte row segments
+        if constexpr (64 % (L / EW) == 0 && SFft<L, EW>::cp.n >= 2) {
+            if (a.B == kZwaveB32) return s_launch_zwave_b<T, L, kZwaveB32>(a, half, s);
+            if (a.B == kZwaveB32 / 2) return s_launch_zwave_b<T, L, kZwaveB32 / 2>(a, half, s);
+        }
+    } else if constexpr (64 % (L / EW) == 0 && SFft<L, EW>::cp.n >= 2) {
         if (a.B == Pol::B) return s_launch_zwave_b<T, L, Pol::B>(a, half, s);
         if constexpr (Pol::B >= 2) {
             if (a.B == Pol::B / 2) return s_launch_zwave_b<T, L, Pol::B / 2>(a, half, s);

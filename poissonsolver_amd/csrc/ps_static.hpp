@@ -13,6 +13,10 @@ enum StaticMode : int { SM_FWD = 0, SM_INV = 1, SM_FUSED = 2 };
 
 constexpr int s_lds_len(int L) { return L + (L >> 5) + 1; }
 
+// pencils per workgroup of the pencil-per-wavefront z kernel with 32
+// elements per thread (FP32 pencils of 2048)
+constexpr int kZwaveB32 = 8;
+
 // columns per strided tile: 128-byte row segments when 512 threads and an
 // 80 KB LDS budget (two workgroups per CU) allow it
 constexpr int s_pick_B(int TP, int L, int cb)

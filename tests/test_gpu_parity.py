@@ -99,3 +99,24 @@ ODD_CASES = [
 def test_fp64_parity_odd_radices(case):
     n, bc, dx, op = case
     _compare(n, bc, dx, 4, op, True, "fp64", FP64_TOL, 19)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("mixed", 2e-6)])
+@pytest.mark.parametrize("op", ["plain", "grad"])
+def test_long_z_pencil_per_wavefront(prec, tol, op):
+    # nz = 1024 unbounded: z transforms of 2048, the pencil-per-wavefront z
+    # kernel with 32 elements per thread (FP32 storage)
+    import poissonsolver_amd as P
+    n, bc, dx = (8, 6, 1024), (0, 0, 0), (0.1, 0.12, 1 / 512)
+    announce("long z", n, prec, op)
+    s = P.poisson_solver(P.PS_FP32 if prec == "fp32" else P.PS_MIXED)
+    s.setup3d(n, bc, dx, 2)
+    assert s.zpass_variant() == "pencil-per-wavefront"
+    s.close()
+    b = random_rhs(n, 47)
+    rhs = [b[0], b[1], b[2]] if op == "grad" else [b[0]]
+    lo, _ = O.solve(n, bc, dx, 2, rhs, op=op)
+    lg, _ = gpu_solve(n, bc, dx, 2, rhs, op=op, prec=prec)
+    got = [lg[d] for d in range(3) if lo[d] is not None]
+    ref = [lo[d] for d in range(3) if lo[d] is not None]
+    assert O.rel_l2(got, ref) <= tol
