@@ -1,7 +1,8 @@
 """Oracle -- TEST INFRASTRUCTURE ONLY.
 
 An FP64 CPU restatement of the reference's 3D solve path
-(``setup3d`` -> ``push`` -> ``solve3d`` -> ``pull``), written in numpy, with the
+(``setup3d`` -> ``push`` -> ``solve3d`` -> ``pull``) and of its 2D solve
+(``setup2d`` -> ``push`` -> ``solve2d`` -> ``pull``), written in numpy, with the
 two special functions in plain C (``oracle/special.c``).  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
 this module, and only as the checker: the product path never calls it.
@@ -28,6 +29,10 @@ How each part is pinned (see the tests named in brackets):
     [test_oracle_pins.py].
   * grad/div/curl/reprojection: spectral invariants and the reference
     tests' analytic fields [test_oracle_pins.py].
+  * 2D UU / PP / PU / UP kernels and the 2D grad/div/curl: the analytic
+    fields of the reference's unbounded2d, periodic2d and mixed2d tests (and
+    the transposed mixed case for UP), a 2D Gaussian closed form and a direct
+    lattice sum [test_oracle_pins2d.py].
 """
 from __future__ import annotations
 
@@ -233,6 +238,48 @@ def greens_2d_uu(n2, dx2, use_glassman=False):
     return g
 
 
+def greens2d(ncell2, bc2, dx2, reg_order, use_glassman=False):
+    """Restates poisson_solver::greens2d (greens2d.cpp:8-441) on one rank for
+    the 2D solver.  Returns (ikx, iky, zeta, G2D) with zeta and G2D of numpy
+    shape (My, Mx) on the fully extended grid (greens2d.cpp:58-59).
+
+    Branches: UU (:157-264) the Ji0 kernel; PP (:268-303) 1/k^2 with G(0) = 0;
+    PU (:307-366) the 1D Si kernel along y on the kx = 0 column; UP (:370-429)
+    the 1D kernel along x on the ky = 0 row.  The UP branch of the reference
+    stores every sample at a stale index j (:400); the intended index i is
+    used here."""
+    bc = tuple(int(b) for b in bc2)
+    h = max(dx2[0], dx2[1])                                            # :56
+    M = [2 * ncell2[d] if bc[d] == 0 else ncell2[d] for d in range(2)]
+    ikx = ik_vector(M[0], dx2[0])                                      # :75-91
+    iky = ik_vector(M[1], dx2[1])                                      # :93-110
+    zeta = None
+    if reg_order > 0:                                                  # :114-150
+        zeta = zeta_filter(ikx.imag, iky.imag, np.zeros(1), h, reg_order)[0]
+
+    def inv_k2():
+        k2 = ikx[None, :] ** 2 + iky[:, None] ** 2
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return -1.0 / k2
+
+    if bc == (0, 0):
+        g = greens_2d_uu((ncell2[0], ncell2[1]), (dx2[0], dx2[1]), use_glassman)
+    elif bc == (1, 1):
+        g = inv_k2()
+        g[0, 0] = 0.0
+    elif bc == (1, 0):
+        g = inv_k2()
+        g1 = greens_1d_samples(M[1], dx2[1]).astype(np.complex128)    # :319-338
+        g[:, 0] = _fft(fft_shift(g1, -1), -1, False, use_glassman)
+    elif bc == (0, 1):
+        g = inv_k2()
+        g1 = greens_1d_samples(M[0], dx2[0]).astype(np.complex128)    # :389-401, index i
+        g[0, :] = _fft(fft_shift(g1, -1), -1, False, use_glassman)
+    else:                                                              # :430-434
+        raise ValueError("greens2d: unsupported boundary condition pair %r" % (bc,))
+    return ikx, iky, zeta, g
+
+
 class Greens:
     """Output of greens3d: ik vectors, zeta and G3D on the fully extended
     y-pencil grid, numpy shape (Mz, My, Mx)."""
@@ -303,6 +350,75 @@ class OracleSolver:
         self.greens = greens3d(self.ncell, self.bc, self.dx, self.reg, self.use_glassman)
         self.Mx = 2 * self.ncell[0] if self.bc[0] == 0 else self.ncell[0]
         self.My = 2 * self.ncell[1] if self.bc[1] == 0 else self.ncell[1]
+
+    # setup2d.cpp:8-72: domain (nx, ny, 1); the y pencils are not y-extended,
+    # the y pass zero-pads to 2 ny itself (solve2d.cpp:181-191)
+    def setup2d(self, ncell, bc, dx, reg_order=0):
+        self.ncell = [int(ncell[0]), int(ncell[1]), 1]
+        self.bc = [int(bc[0]), int(bc[1]), -1]
+        self.dx = [float(dx[0]), float(dx[1]), 0.0]
+        self.reg = int(reg_order)
+        self.partition = partition_setup(0, self.ncell, self.bc, self.dx, False, False, False)
+        self.g2 = greens2d(self.ncell[:2], self.bc[:2], self.dx[:2], self.reg, self.use_glassman)
+        self.Mx = 2 * self.ncell[0] if self.bc[0] == 0 else self.ncell[0]
+        self.My = 2 * self.ncell[1] if self.bc[1] == 0 else self.ncell[1]
+
+    # solve2d.cpp:8-475
+    def solve2d(self):
+        ikx, iky, zeta, G = self.g2
+        nx, ny = self.ncell[0], self.ncell[1]
+        Mx, My = self.Mx, self.My
+        rX, rY, rZ = (r is not None for r in self.rhs)
+        if self.lhs_grad:                                          # :44-58
+            if not rX:
+                raise ValueError("grad: rhs fields have not been pushed")
+            lflag = (True, True, False)
+        elif self.lhs_div:                                         # :59-72
+            if not (rX and rY):
+                raise ValueError("div: rhs fields have not been pushed")
+            lflag = (True, False, False)
+        elif self.lhs_curl:                                        # :73-87
+            if not rZ:
+                raise ValueError("curl: rhs fields have not been pushed")
+            lflag = (True, True, False)
+        else:                                                      # :88-102
+            lflag = (rX, rY, rZ)
+        rflag = (rX, rY, rZ)
+        ug = self.use_glassman
+
+        r = [None] * 3
+        for d in range(3):
+            if not rflag[d]:
+                continue
+            a = _fft(self.rhs[d][0], 1, False, ug)                 # x-pass :117-158
+            b = np.zeros((My, Mx), dtype=np.complex128)            # y zero-padding :226-243
+            b[:ny] = a
+            r[d] = _fft(b, 0, False, ug)                           # y fft :261
+        if self.reg > 0:                                           # :273-287
+            r = [None if v is None else v * zeta for v in r]
+        kx = ikx[None, :]
+        ky = iky[:, None]
+        lk = [None] * 3
+        if self.lhs_grad:                                          # :292-300
+            lk[0] = kx * G * r[0]
+            lk[1] = ky * G * r[0]
+        elif self.lhs_div:                                         # :301-305
+            lk[0] = G * (kx * r[0] + ky * r[1])
+        elif self.lhs_curl:                                        # :306-310
+            lk[0] = ky * G * r[2]
+            lk[1] = -kx * G * r[2]
+        else:                                                      # :311-326
+            lk = [G * r[d] if lflag[d] else None for d in range(3)]
+
+        def inverse(f):
+            f = _fft(f, 0, True, ug)[:ny] / float(My)              # :331-367
+            return (_fft(f, 1, True, ug) / float(Mx))[None]        # x-inverse :381-471
+
+        for d in range(3):
+            if rflag[d]:
+                self.rhs[d] = inverse(r[d])
+            if lflag[d]:
+                self.lhs[d] = inverse(lk[d])
 
     # settings.cpp:9-30
     def set_return_grad(self, t): self.lhs_grad = bool(t)
@@ -431,6 +547,24 @@ def solve(ncell, bc, dx, reg_order, rhs, op="plain", reproject=False, use_glassm
     nx, ny, nz = s.ncell
     lhs = [None if s.lhs[d] is None else s.lhs[d][:, :, :nx].real.copy() for d in range(3)]
     rb = [None if s.rhs[d] is None else s.rhs[d][:, :, :nx].real.copy() for d in range(3)]
+    return lhs, rb
+
+
+def solve2d(ncell, bc, dx, reg_order, rhs, op="plain", use_glassman=False):
+    """One-shot 2D helper: setup2d, push ``rhs`` (list of up to 3 arrays of
+    shape (ny, nx) or None), solve2d with ``op`` in {plain, grad, div, curl},
+    return (lhs list, rhs list) of (ny, nx) arrays."""
+    s = OracleSolver(use_glassman)
+    s.setup2d(ncell, bc, dx, reg_order)
+    s.set_return_grad(op == "grad")
+    s.set_return_div(op == "div")
+    s.set_return_curl(op == "curl")
+    rhs = list(rhs) + [None] * (3 - len(rhs))
+    s.push(*rhs)
+    s.solve2d()
+    nx = s.ncell[0]
+    lhs = [None if s.lhs[d] is None else s.lhs[d][0, :, :nx].real.copy() for d in range(3)]
+    rb = [None if s.rhs[d] is None else s.rhs[d][0, :, :nx].real.copy() for d in range(3)]
     return lhs, rb
 
 
