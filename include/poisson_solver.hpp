@@ -21,6 +21,7 @@ private:
     int status_ = PS_OK;
     int precision_ = PS_FP64;
     bool report(int status, const char* where);
+    void refresh_partition();
 
 public:
     std::vector<class_partition> partition;

@@ -51,7 +51,10 @@ int ps_set_precision(ps_ctx* ctx, int precision);
    (poisson_solver.hpp:111, setup3d.cpp:8-71): ncell, bc (0 unbounded,
    1 periodic), cell size, regularisation order */
 int ps_setup3d(ps_ctx* ctx, const int ncell[3], const int bc[3], const double dx[3], int reg_order);
-/* replaces poisson_solver::setup2d (poisson_solver.hpp:110); not built: PS_EUNSUPPORTED */
+/* replaces poisson_solver::setup2d(int[2], int[2], double[2], int)
+   (poisson_solver.hpp:110, setup2d.cpp:8-72, greens2d.cpp:8-441): ncell,
+   bc (0 unbounded, 1 periodic; all four pairs), cell size, regularisation
+   order.  The UP kernel uses the intended sample index (greens2d.cpp:400). */
 int ps_setup2d(ps_ctx* ctx, const int ncell[2], const int bc[2], const double dx[2], int reg_order);
 
 /* replace set_return_grad/div/curl and set_reprojection (settings.cpp:9-30) */
@@ -65,7 +68,9 @@ int ps_set_reprojection(ps_ctx* ctx, int toggle);
 int ps_push(ps_ctx* ctx, const double* bx, const double* by, const double* bz);
 /* replaces poisson_solver::solve3d() (solve3d.cpp:8-635) */
 int ps_solve3d(ps_ctx* ctx);
-/* replaces poisson_solver::solve2d() (poisson_solver.hpp:118); not built: PS_EUNSUPPORTED */
+/* replaces poisson_solver::solve2d() (poisson_solver.hpp:118, solve2d.cpp:8-475):
+   plain (every pushed component), grad (of X, two components), div (X, Y ->
+   one component), curl (of the scalar Z, two components); no reprojection */
 int ps_solve2d(ps_ctx* ctx);
 /* replaces poisson_solver::pull(double* x6) (pull.cpp:9-469): lhs X,Y,Z then
    rhs X,Y,Z; a component is written only when the pointer is non-NULL and the
@@ -100,6 +105,11 @@ int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int
 int ps_eval_special(const double* x, double* y, long n, int which);
 /* enqueue ps_solve3d on the context's stream and return without waiting */
 int ps_solve3d_async(ps_ctx* ctx);
+/* enqueue ps_solve2d on the context's stream and return without waiting */
+int ps_solve2d_async(ps_ctx* ctx);
+/* 1 if the context was set up by ps_setup2d (its passes: x, fused y, x^-1;
+   ps_profile_solve reports the fused y pass as the z entry), else 0 */
+int ps_is_2d(const ps_ctx* ctx);
 /* the FP16 hi/lo fragments of the 32-point DFT matrix that the matrix-core
    kernels multiply by (binary16 bit patterns; n >= 4096 entries, layout
    [lane 64][K chunk 2][Yr hi, Yr lo, Yi hi, Yi lo][8]) */

@@ -60,6 +60,8 @@ SIGNATURES = {
                                         _c_int_p, _c_int_p]),
     "ps_eval_special": (ctypes.c_int, [_c_dbl_p, _c_dbl_p, ctypes.c_long, ctypes.c_int]),
     "ps_solve3d_async": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_solve2d_async": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_is_2d": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_zpass_variant": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_dft32_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
     "ps_tuning_default": (ctypes.c_int, [_tun_p]),

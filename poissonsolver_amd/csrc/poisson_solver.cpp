@@ -30,12 +30,20 @@ void poisson_solver::set_precision(int precision)
 
 void poisson_solver::setup2d(int ncell[2], int bc[2], double dx[2], int reg_order)
 {
-    report(ps_setup2d(ctx_, ncell, bc, dx, reg_order), "setup2d");
+    if (!report(ps_setup2d(ctx_, ncell, bc, dx, reg_order), "setup2d")) return;
+    refresh_partition();
 }
 
 void poisson_solver::setup3d(int ncell[3], int bc[3], double dx[3], int reg_order)
 {
     if (!report(ps_setup3d(ctx_, ncell, bc, dx, reg_order), "setup3d")) return;
+    refresh_partition();
+}
+
+// partition[rank] of the reference (setup3d.cpp:46, setup2d.cpp:46): one rank
+// holding the whole grid; (nx, ny, 1) and dx (dx, dy, 0) after setup2d
+void poisson_solver::refresh_partition()
+{
     partition.clear();
     const int n = ps_partition_count(ctx_);
     for (int r = 0; r < n; ++r) {

@@ -9,6 +9,10 @@
 //   x^-1     C2R, keep nx, real part                              (solve3d.cpp:541-629, pull.cpp:101-131)
 // The two MPI transposes of the reference (map.cpp) are the strides of the
 // strided passes: nothing is moved between passes except by the passes.
+//
+// A 2D context (setup2d) holds its grid (nx, ny) as (nx, 1, ny): the x passes
+// are the same, there is no y pass, and the fused pass along the third axis
+// is the y transform of solve2d.cpp:166-367 with the 2D operators.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -75,6 +79,7 @@ struct ps_ctx {
     int prec = PS_FP64;
     bool grad = false, div = false, curl = false, reproj = false;
     bool ready = false;
+    bool dim2 = false;      // set up by setup2d: grid (nx, ny) held as (nx, 1, ny)
     int ncell[3] = {0, 0, 0}, bc[3] = {0, 0, 0};
     double dx[3] = {0, 0, 0};
     int reg = 0;
@@ -88,7 +93,7 @@ struct ps_ctx {
     void* lhs = nullptr;    // 3 x n3 real, solutions
     void* rhsf = nullptr;   // 3 x n3 real, filtered sources (computed on demand)
     void* A1 = nullptr;     // ncomp_alloc x nz x ny x Hp complex
-    void* A2 = nullptr;     // ncomp_alloc x nz x My x Hp complex
+    void* A2 = nullptr;     // ncomp_alloc x nz x My x Hp complex (not used by 2D contexts)
     long A1c = 0, A2c = 0;  // complex elements per component
     int ncomp_alloc = 0;
     void* G = nullptr;      // octant of Ghat*zeta/N, real
@@ -188,7 +193,7 @@ int ensure_comp_buffers(ps_ctx* c, int ncomp)
     free_ptr(c->A2);
     const size_t cb = 2 * (size_t)real_bytes(c);
     CK(hipMalloc(&c->A1, (size_t)ncomp * c->A1c * cb));
-    CK(hipMalloc(&c->A2, (size_t)ncomp * c->A2c * cb));
+    if (c->A2c > 0) CK(hipMalloc(&c->A2, (size_t)ncomp * c->A2c * cb));
     c->ncomp_alloc = ncomp;
     return PS_OK;
 }
@@ -200,12 +205,14 @@ int ensure_real(ps_ctx* c, void** p)
     return PS_OK;
 }
 
-// Operator selection of solve3d.cpp:43-107 (grad > div > curl > plain).
+// Operator selection of solve3d.cpp:43-107 and solve2d.cpp:44-102
+// (grad > div > curl > plain).
 struct OpSel {
     int op = OP_PLAIN;
     int nc = 1;                 // 1: component-wise kernel, 3: mixes components
     int in_first = 0, in_step = 1, nin = 0;
     int out_first = 0, out_step = 1, nout = 0;
+    int kmap = KMAP_3D;         // axis of each vector component (KMap)
 };
 
 bool arith(const bool present[3], int* first, int* step, int* n)
@@ -220,8 +227,52 @@ bool arith(const bool present[3], int* first, int* step, int* n)
     return true;
 }
 
+// 2D operators (solve2d.cpp:292-326).  grad and curl differentiate one
+// scalar in two directions: the x pass transforms it twice (input step 0),
+// so each output component is an in-place component-wise z pass.
+int select_op2d(ps_ctx* c, OpSel* s)
+{
+    const bool rX = c->rhs_present[0], rY = c->rhs_present[1], rZ = c->rhs_present[2];
+    if (!arith(c->rhs_present, &s->in_first, &s->in_step, &s->nin))
+        return fail(c, PS_ESTATE, "[poisson_solver.solve2d]: no rhs field has been pushed");
+    if (c->grad) {
+        if (!rX)
+            return fail(c, PS_ESTATE,
+                        "[poisson_solver.solve2d]: Error rhs fields have not been pushed correctly for grad option");
+        s->op = OP_GRAD;
+        s->nc = 1;
+        s->kmap = KMAP_2D_GRAD;
+        s->in_first = 0; s->in_step = 0; s->nin = 2;
+        s->out_first = 0; s->out_step = 1; s->nout = 2;
+    } else if (c->div) {
+        if (!(rX && rY))
+            return fail(c, PS_ESTATE,
+                        "[poisson_solver.solve2d]: Error rhs fields have not been pushed correctly for div option");
+        s->op = OP_DIV;
+        s->nc = 3;
+        s->kmap = KMAP_2D_DIV;
+        s->in_first = 0; s->in_step = 1; s->nin = 2;
+        s->out_first = 0; s->out_step = 1; s->nout = 1;
+    } else if (c->curl) {
+        if (!rZ)
+            return fail(c, PS_ESTATE,
+                        "[poisson_solver.solve2d]: Error rhs fields have not been pushed correctly for curl option");
+        s->op = OP_GRAD;
+        s->nc = 1;
+        s->kmap = KMAP_2D_CURL;
+        s->in_first = 2; s->in_step = 0; s->nin = 2;
+        s->out_first = 0; s->out_step = 1; s->nout = 2;
+    } else {
+        s->op = OP_PLAIN;
+        s->nc = 1;
+        s->out_first = s->in_first; s->out_step = s->in_step; s->nout = s->nin;
+    }
+    return PS_OK;
+}
+
 int select_op(ps_ctx* c, OpSel* s)
 {
+    if (c->dim2) return select_op2d(c, s);
     const bool rX = c->rhs_present[0], rY = c->rhs_present[1], rZ = c->rhs_present[2];
     if (!arith(c->rhs_present, &s->in_first, &s->in_step, &s->nin))
         return fail(c, PS_ESTATE, "[poisson_solver.solve3d]: no rhs field has been pushed");
@@ -261,7 +312,7 @@ template <typename T>
 void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
 {
     a.plan = device_plan<T>(d.p, reinterpret_cast<const cplx<T>*>(d.tw));
-    a.in = a.out = c->A2;
+    a.in = a.out = c->dim2 ? c->A1 : c->A2;           // 2D: the x pass output, in place
     a.ia = a.oa = c->a2_zpass;                        // slab = ky, pos = kz, col = kx
     a.xcd_remap = c->tune.xcd;
     a.slab_pair = 1;
@@ -295,11 +346,12 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
     a.fzero = 0.0f;
 }
 
-// The five passes.  `src` holds nin real components starting at in_first
-// with step in_step; `dst` receives nout components at out_first/out_step.
+// The five passes (three for a 2D context).  `src` holds nin real components
+// starting at in_first with step in_step (step 0: one component transformed
+// nin times); `dst` receives nout components at out_first/out_step.
 template <typename T>
 int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin, int op, int nc, void* dst,
-                 int out_first, int out_step, int nout, hipEvent_t* ev)
+                 int out_first, int out_step, int nout, hipEvent_t* ev, int kmap = KMAP_3D)
 {
     hipStream_t s = c->stream;
     const int nx = c->ncell[0], ny = c->ncell[1], nz = c->ncell[2];
@@ -326,6 +378,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     if (ev) CK(hipEventRecord(ev[1], s));
 
     PencilArgs<T> ya{};
+    if (!c->dim2) {
     ya.plan = device_plan<T>(c->py.p, reinterpret_cast<const cplx<T>*>(c->py.tw));
     ya.in = c->A1;
     ya.out = c->A2;
@@ -347,19 +400,21 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     ya.op = OP_NONE;
     if (c->py.stat) CK(static_launch_pencil<T>(c->py.p.L, ya, 1, SM_FWD, c->bc[1] == 0, s));
     else CK(launch_pencil<T>(c->py.p.E, 1, ya, s));
+    }
     if (ev) CK(hipEventRecord(ev[2], s));
 
     const DevPlan& zp = (nc == 3) ? c->pz3 : c->pz1;
     PencilArgs<T> za{};
     fill_zargs<T>(c, za, zp);
     za.op = op;
+    za.kmap = kmap;
     za.ncomp = (nc == 3) ? 1 : nin;
     bool mfma_done = false;
     if constexpr (sizeof(T) == 4) {
         if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, op)) {
             CK(launch_zmfma(za, c->tune.mfma == 2, s));
             mfma_done = true;
-        } else if (c->zmfma && nc == 3 && zmfma3_supported(zp.p.L, c->bc[2] == 0, op)) {
+        } else if (c->zmfma && nc == 3 && kmap == KMAP_3D && zmfma3_supported(zp.p.L, c->bc[2] == 0, op)) {
             CK(launch_zmfma3(za, s));
             mfma_done = true;
         }
@@ -369,6 +424,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     else CK(launch_pencil<T>(zp.p.E, nc, za, s));
     if (ev) CK(hipEventRecord(ev[3], s));
 
+    if (!c->dim2) {
     PencilArgs<T> yi = ya;
     yi.in = c->A2;
     yi.out = c->A1;
@@ -381,6 +437,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     yi.do_inv = 1;
     if (c->py.stat) CK(static_launch_pencil<T>(c->py.p.L, yi, 1, SM_INV, c->bc[1] == 0, s));
     else CK(launch_pencil<T>(c->py.p.E, 1, yi, s));
+    }
     if (ev) CK(hipEventRecord(ev[4], s));
 
     XArgs<T> xi = xa;
@@ -418,8 +475,14 @@ template <typename T> int setup_impl(ps_ctx* c)
     if (zb == -2) zb = (nz >= 32) ? 4 : 0;    // 16-plane blocks: measured best at 512^3
     const long My = c->M[1];
     if (zb > 0 && (1 << zb) >= nz) zb = -1;
+    if (c->dim2) zb = 0;
     c->zb = zb;
-    if (zb == 0) {
+    if (c->dim2) {
+        // no y pass: the fused pass reads the x pass output, rows of Hp
+        c->A2c = 0;
+        c->a2_ypass = addr_plain(0, 0, 0, 1);
+        c->a2_zpass = addr_plain(c->A1c, c->Hp, c->Hp, 1);
+    } else if (zb == 0) {
         c->A2c = (long)nz * My * c->Hp;
         c->a2_ypass = addr_plain(c->A2c, My * c->Hp, c->Hp, 1);
         c->a2_zpass = addr_plain(c->A2c, c->Hp, My * c->Hp, 1);
@@ -485,8 +548,9 @@ template <typename T> int setup_impl(ps_ctx* c)
     // matrix-core z pass (PS_MIXED): its lanes read Gk along kx; positions
     // z0 + 32 u must be linear in u (z blocks of at most 32 planes, or plain)
     // 32-bit byte offsets: one component of A2 and the Gk octant below 4 GB
+    const long zcomp = c->dim2 ? c->A1c : c->A2c;
     c->zmfma = c->prec == PS_MIXED && tn.mfma && zmfma_supported(c->M[2], c->bc[2] == 0, OP_PLAIN) &&
-               c->zb <= 5 && c->A2c * 8 < (1L << 32) && (long)c->O[0] * c->O[1] * c->O[2] * 4 < (1L << 32);
+               c->zb <= 5 && zcomp * 8 < (1L << 32) && (long)c->O[0] * c->O[1] * c->O[2] * 4 < (1L << 32);
     c->g_layout = (c->zwave && !c->zmfma) ? 1 : 0;
     if (c->zmfma) {
         std::vector<unsigned short> fr(kZmfmaFragHalves);
@@ -496,7 +560,7 @@ template <typename T> int setup_impl(ps_ctx* c)
     }
     CK(hipMalloc(&c->G, (size_t)c->O[0] * c->O[1] * c->O[2] * sizeof(T)));
     r = build_greens<T>(c->ncell, c->bc, c->dx, c->reg, reinterpret_cast<T*>(c->G), c->g_layout, c->stream,
-                        &c->err);
+                        &c->err, c->dim2 ? 1 : 0);
     if (r) return r;
     return PS_OK;
 }
@@ -509,7 +573,7 @@ void after_solve(ps_ctx* c, const OpSel& s)
     c->solved_after_push = true;
     c->rhsf_valid = false;
     const bool all3 = c->rhs_present[0] && c->rhs_present[1] && c->rhs_present[2];
-    c->solve_reproj = c->reproj && all3;                          // solve3d.cpp:388
+    c->solve_reproj = !c->dim2 && c->reproj && all3;              // solve3d.cpp:388; none in 2D
     c->solve_filter_active = (c->reg > 0) || c->solve_reproj;
 }
 
@@ -520,7 +584,7 @@ template <typename T> int solve_impl(ps_ctx* c, hipEvent_t* ev)
     if (r) return r;
     if ((r = ensure_real(c, &c->lhs))) return r;
     r = run_pipeline<T>(c, c->rhs, s.in_first, s.in_step, s.nin, s.op, s.nc, c->lhs, s.out_first, s.out_step,
-                        s.nout, ev);
+                        s.nout, ev, s.kmap);
     if (r) return r;
     after_solve(c, s);
     return PS_OK;
@@ -576,7 +640,34 @@ template <typename T> int pull_host(ps_ctx* c, double* dst, const void* src_base
 int check_ready(ps_ctx* c)
 {
     if (!c) return PS_EINVAL;
-    if (!c->ready) return fail(c, PS_ESTATE, "setup3d has not been called");
+    if (!c->ready) return fail(c, PS_ESTATE, "setup3d or setup2d has not been called");
+    return PS_OK;
+}
+
+// Set-up shared by setup3d and setup2d; ncell/bc/dx are the internal 3D form.
+int setup_common(ps_ctx* c, const int ncell[3], const int bc[3], const double dx[3], int reg_order, bool dim2)
+{
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(c, PS_EHIP, "no HIP device available");
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    release(c);
+    CK(hipGetDevice(&c->device));
+    if (!c->stream) CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (int d = 0; d < 3; ++d) {
+        c->ncell[d] = ncell[d];
+        c->bc[d] = bc[d];
+        c->dx[d] = dx[d];
+        c->rhs_present[d] = c->lhs_present[d] = false;
+    }
+    c->dim2 = dim2;
+    c->reg = reg_order;
+    c->solved_after_push = c->rhsf_valid = false;
+    int r = (c->prec == PS_FP64) ? setup_impl<double>(c) : setup_impl<float>(c);
+    if (r) {
+        release(c);
+        return r;
+    }
+    c->ready = true;
     return PS_OK;
 }
 
@@ -647,32 +738,26 @@ int ps_setup3d(ps_ctx* c, const int ncell[3], const int bc[3], const double dx[3
                                "z-direction must be the unbounded direction");
     }
     if (reg_order < 0) return fail(c, PS_EINVAL, "setup3d: reg_order must be >= 0");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(c, PS_EHIP, "no HIP device available");
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    release(c);
-    CK(hipGetDevice(&c->device));
-    if (!c->stream) CK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (int d = 0; d < 3; ++d) {
-        c->ncell[d] = ncell[d];
-        c->bc[d] = bc[d];
-        c->dx[d] = dx[d];
-        c->rhs_present[d] = c->lhs_present[d] = false;
-    }
-    c->reg = reg_order;
-    c->solved_after_push = c->rhsf_valid = false;
-    int r = (c->prec == PS_FP64) ? setup_impl<double>(c) : setup_impl<float>(c);
-    if (r) {
-        release(c);
-        return r;
-    }
-    c->ready = true;
-    return PS_OK;
+    return setup_common(c, ncell, bc, dx, reg_order, false);
 }
 
-int ps_setup2d(ps_ctx* c, const int*, const int*, const double*, int)
+int ps_setup2d(ps_ctx* c, const int ncell[2], const int bc[2], const double dx[2], int reg_order)
 {
-    return fail(c, PS_EUNSUPPORTED, "setup2d: the 2D solver is not part of this library");
+    if (!c) return PS_EINVAL;
+    if (!ncell || !bc || !dx) return fail(c, PS_EINVAL, "setup2d: NULL argument");
+    for (int d = 0; d < 2; ++d) {
+        if (ncell[d] < 1) return fail(c, PS_EINVAL, "setup2d: ncell must be positive");
+        if (!(dx[d] > 0.0)) return fail(c, PS_EINVAL, "setup2d: dx must be positive");
+        if (bc[d] != 0 && bc[d] != 1)
+            return fail(c, PS_EBC, "[poisson_solver.greens2d]: Boundary condition configuration unknown.");
+    }
+    if (reg_order < 0) return fail(c, PS_EINVAL, "setup2d: reg_order must be >= 0");
+    // (nx, ny) held as (nx, 1, ny): the middle axis is one periodic cell whose
+    // spacing dx does not change max(dx, dy)
+    const int n3[3] = {ncell[0], 1, ncell[1]};
+    const int b3[3] = {bc[0], 1, bc[1]};
+    const double d3[3] = {dx[0], dx[0], dx[1]};
+    return setup_common(c, n3, b3, d3, reg_order, true);
 }
 
 int ps_set_return_grad(ps_ctx* c, int t) { if (!c) return PS_EINVAL; c->grad = t != 0; return PS_OK; }
@@ -717,6 +802,7 @@ int ps_solve3d(ps_ctx* c)
 {
     int r = check_ready(c);
     if (r) return r;
+    if (c->dim2) return fail(c, PS_ESTATE, "solve3d: the solver was set up by setup2d");
     if (!c->rhs) return fail(c, PS_ESTATE, "[poisson_solver.solve3d]: no rhs field has been pushed");
     r = (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
     if (r) return r;
@@ -743,11 +829,33 @@ int ps_solve3d_async(ps_ctx* c)
 {
     int r = check_ready(c);
     if (r) return r;
+    if (c->dim2) return fail(c, PS_ESTATE, "solve3d: the solver was set up by setup2d");
     if (!c->rhs) return fail(c, PS_ESTATE, "[poisson_solver.solve3d]: no rhs field has been pushed");
     return (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
 }
 
-int ps_solve2d(ps_ctx* c) { return fail(c, PS_EUNSUPPORTED, "solve2d: the 2D solver is not part of this library"); }
+int ps_solve2d(ps_ctx* c)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->dim2) return fail(c, PS_ESTATE, "solve2d: the solver was set up by setup3d");
+    if (!c->rhs) return fail(c, PS_ESTATE, "[poisson_solver.solve2d]: no rhs field has been pushed");
+    r = (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
+    if (r) return r;
+    CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+int ps_solve2d_async(ps_ctx* c)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!c->dim2) return fail(c, PS_ESTATE, "solve2d: the solver was set up by setup3d");
+    if (!c->rhs) return fail(c, PS_ESTATE, "[poisson_solver.solve2d]: no rhs field has been pushed");
+    return (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
+}
+
+int ps_is_2d(const ps_ctx* c) { return (c && c->ready && c->dim2) ? 1 : 0; }
 
 int ps_pull(ps_ctx* c, double* ax, double* ay, double* az, double* bx, double* by, double* bz)
 {
@@ -809,6 +917,12 @@ int ps_partition(const ps_ctx* c, int rank, int ncell[3], int icell[3], double d
         ncell[d] = c->ncell[d];
         icell[d] = 0;
         dx[d] = c->dx[d];
+    }
+    if (c->dim2) {   // setup2d.cpp:31-46: domain (nx, ny, 1), dx (dx, dy, 0)
+        ncell[1] = c->ncell[2];
+        ncell[2] = 1;
+        dx[1] = c->dx[2];
+        dx[2] = 0.0;
     }
     return PS_OK;
 }
@@ -962,6 +1076,7 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
         dx[d] = c->dx[d];
     }
     const int reg = c->reg;
+    const bool dim2 = c->dim2;
     bool present[3] = {c->rhs_present[0], c->rhs_present[1], c->rhs_present[2]};
     if (c->grad || c->div || c->curl) present[0] = present[1] = present[2] = true;
     if (!present[0] && !present[1] && !present[2]) present[0] = true;
@@ -969,7 +1084,7 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
     auto trial = [&](const Tuning& t, double* ms) -> int {
         c->tune_req = t;
         c->tune_set = true;
-        int st = ps_setup3d(c, ncell, bc, dx, reg);
+        int st = setup_common(c, ncell, bc, dx, reg, dim2);
         if (st) return st;
         if ((st = ensure_real(c, &c->rhs))) return st;
         CK(hipMemset(c->rhs, 0, 3 * (size_t)c->n3 * real_bytes(c)));
@@ -1024,7 +1139,7 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
     }
     c->tune_req = best;
     c->tune_set = true;
-    if ((r = ps_setup3d(c, ncell, bc, dx, reg))) return r;
+    if ((r = setup_common(c, ncell, bc, dx, reg, dim2))) return r;
     if (best_out) *best_out = c->tune;
     if (best_ms_out) *best_ms_out = best_ms;
     return PS_OK;

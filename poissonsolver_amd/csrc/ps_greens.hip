@@ -94,6 +94,10 @@ __global__ void k_finalize(T* out, const double* gsp, int mode, const double* g2
             g = (i == 0 && j == 0) ? g1d[k] : 1.0 / k2;
         } else if (mode == 2) {     // UUP: 2D kernel on the kz = 0 plane
             g = (k == 0) ? g2d[(long)j * Ox + i] : 1.0 / k2;
+        } else if (mode == 4) {     // 2D UU: the 2D kernel transform on the (kx, kz) plane (Oy = 1)
+            g = gsp[id];
+        } else if (mode == 5) {     // 2D UP: 1D kernel along x on the kz = 0 row
+            g = (k == 0) ? g1d[i] : 1.0 / k2;
         } else {                    // PPP: zero mode removed
             g = (i == 0 && j == 0 && k == 0) ? 0.0 : 1.0 / k2;
         }
@@ -195,9 +199,17 @@ static unsigned grid_for(long n)
 
 }  // namespace
 
+// 2D grids (dim2) are held as (nx, 1, ny) with the y direction on the third
+// axis; their branches (greens2d.cpp:157-429) map onto the modes as
+//   UU (0,1,0) -> 4  Ji0 kernel sampled on the (x, z) quadrant, DCT-I along x and z
+//   PU (1,1,0) -> 1  1D Si kernel along z on the kx = 0 column (:307-366)
+//   UP (0,1,1) -> 5  1D Si kernel along x on the kz = 0 row; the reference
+//                    stores every sample at a stale index j (:400), the
+//                    intended index i is used
+//   PP (1,1,1) -> 3  1/k^2, Ghat(0) = 0 (:268-303)
 template <typename T>
 int build_greens(const int ncell[3], const int bc[3], const double dx[3], int reg, T* out, int layout,
-                 hipStream_t s, std::string* err)
+                 hipStream_t s, std::string* err, int dim2)
 {
     int M[3], O[3];
     for (int d = 0; d < 3; ++d) {
@@ -207,7 +219,13 @@ int build_greens(const int ncell[3], const int bc[3], const double dx[3], int re
     const double h = std::max(std::max(dx[0], dx[1]), dx[2]);    // greens3d.cpp:55
     const long n = (long)O[0] * O[1] * O[2];
     int mode;
-    if (bc[0] == 0 && bc[1] == 0 && bc[2] == 0) mode = 0;
+    if (dim2) {
+        if (ncell[1] != 1 || bc[1] != 1) {
+            if (err) *err = "2D kernel: the grid must be held as (nx, 1, ny) with a periodic middle axis";
+            return kGreensEBC;
+        }
+        mode = (bc[0] == 0) ? (bc[2] == 0 ? 4 : 5) : (bc[2] == 0 ? 1 : 3);
+    } else if (bc[0] == 0 && bc[1] == 0 && bc[2] == 0) mode = 0;
     else if (bc[0] == 1 && bc[1] == 1 && bc[2] == 0) mode = 1;
     else if (bc[0] == 0 && bc[1] == 0 && bc[2] == 1) mode = 2;
     else if (bc[0] == 1 && bc[1] == 1 && bc[2] == 1) mode = 3;
@@ -232,6 +250,25 @@ int build_greens(const int ncell[3], const int bc[3], const double dx[3], int re
         PS_HIP(hipGetLastError());
         const int O1[3] = {O[2], 1, 1};
         int r = dct_pass((double*)g1.p, O1, 0, M[2], s, err);
+        if (r) return r;
+    } else if (mode == 4) {
+        // greens2d.cpp:157-264 on (x, z): sigma = max(dx, dy)/pi, C2 = ln(2 sigma) - gamma
+        PS_HIP(g3.alloc(n * sizeof(double)));
+        const double sigma2 = h / kPi;
+        const double c2 = std::log(2.0 * sigma2) - kGamma;
+        hipLaunchKernelGGL(k_sample_2d, dim3(grid_for(n)), dim3(256), 0, s, (double*)g3.p, O[0], O[2], dx[0],
+                           dx[2], sigma2, c2);
+        PS_HIP(hipGetLastError());
+        int r = dct_pass((double*)g3.p, O, 0, M[0], s, err);
+        if (r) return r;
+        r = dct_pass((double*)g3.p, O, 2, M[2], s, err);
+        if (r) return r;
+    } else if (mode == 5) {
+        PS_HIP(g1.alloc(O[0] * sizeof(double)));
+        hipLaunchKernelGGL(k_sample_1d, dim3(grid_for(O[0])), dim3(256), 0, s, (double*)g1.p, O[0], dx[0]);
+        PS_HIP(hipGetLastError());
+        const int O1[3] = {O[0], 1, 1};
+        int r = dct_pass((double*)g1.p, O1, 0, M[0], s, err);
         if (r) return r;
     } else if (mode == 2) {
         // greens3d.cpp:413-425 calls greens2d with (nx, ny), dx[2] = 0, so h = max(dx, dy)
@@ -265,9 +302,9 @@ int build_greens(const int ncell[3], const int bc[3], const double dx[3], int re
 }
 
 template int build_greens<float>(const int[3], const int[3], const double[3], int, float*, int, hipStream_t,
-                                 std::string*);
+                                 std::string*, int);
 template int build_greens<double>(const int[3], const int[3], const double[3], int, double*, int, hipStream_t,
-                                  std::string*);
+                                  std::string*, int);
 
 // Si and Ji0 evaluated by the device code, for tests of the special functions.
 __global__ void k_eval_special(const double* x, double* y, long n, int which)

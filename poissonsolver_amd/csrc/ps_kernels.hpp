@@ -23,6 +23,16 @@ enum PencilOp : int {
     OP_RHSF_PROJ = 6, // r_d*zeta/N - Gk (kh_d kh.r + kn_d kn.r)  (filtered + reprojected)
 };
 
+// Wavenumber axis of each vector component for OP_GRAD and OP_DIV.  A 2D
+// solve runs as a 3D one of shape (nx, 1, ny): its y direction is the pass
+// axis z, and its operators pick their axes by this map.
+enum KMap : int {
+    KMAP_3D = 0,       // component c <-> axis c
+    KMAP_2D_GRAD = 1,  // component 0 <-> k_x, 1 <-> k_z (both read the same scalar)
+    KMAP_2D_CURL = 2,  // component 0 <-> +k_z, 1 <-> -k_x (curl of a scalar along z)
+    KMAP_2D_DIV = 3,   // component 0 <-> k_x, 1 <-> k_z, component 2 unused
+};
+
 enum LoadMode : int { LOAD_COMPLEX = 0, LOAD_REAL_EVEN = 1 };
 enum StoreMode : int { STORE_COMPLEX = 0, STORE_REAL = 1 };
 
@@ -57,6 +67,7 @@ template <typename T> struct PencilArgs {
     int occ4;        // specialised fused kernel: target 4 waves/SIMD (else 2)
     int zwave;       // specialised fused kernel: pencil-per-wavefront variant (k_zfused_w)
     int slab_pair;   // 1: visit slabs as 0, M/2, 1, M-1, 2, M-2, ... (z pass: ky and My-ky share a row of Gk)
+    int kmap;        // KMap of the operator's vector components
     // k-space operator (z pass): slab = ky, col = kx, pos = kz
     const T* G;
     long g_ky, g_kx, g_kz;

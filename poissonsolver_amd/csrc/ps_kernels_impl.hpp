@@ -73,6 +73,16 @@ __device__ __forceinline__ TileK<T> tile_k(const PencilArgs<T>& a, int col, int 
     return k;
 }
 
+// Signed wavenumber of vector component c under map `kmap` (KMap), from the
+// axis wavenumbers kx, ky, kz; 0 for a component the map does not use.
+template <typename T>
+__device__ __forceinline__ T kcomp(int kmap, int c, T kx, T ky, T kz)
+{
+    if (kmap == KMAP_3D) return (c == 0) ? kx : (c == 1) ? ky : kz;
+    if (kmap == KMAP_2D_CURL) return (c == 0) ? kz : (c == 1) ? -kx : T(0);
+    return (c == 0) ? kx : (c == 1) ? kz : T(0);
+}
+
 // k-space operator of the fused z pass on one thread's E spectrum values
 // (solve3d.cpp:329-396 in the Hermitian-part form of the half spectrum).
 // Output o of the NC inputs v goes to w; positions are kz = t + TP u.
@@ -102,14 +112,20 @@ __device__ __forceinline__ void kspace_op(cplx<T> (&w)[E], const cplx<T> (&v)[NC
         const T kzn = nyz ? (T)kzf : T(0);
         C r;
         if (op == OP_GRAD) {
-            const T kd = (comp == 0) ? tk.kxh : (comp == 1) ? tk.kyh : kzh;
+            const T kd = kcomp<T>(a.kmap, comp, tk.kxh, tk.kyh, kzh);
             const C s = v[0][u];
             r = C{-kd * g * s.y, kd * g * s.x};
         } else if (op == OP_DIV || op == OP_CURL) {
             const C sx = v[0][u], sy = v[vi][u], sz = v[wi][u];
             C q;
             if (op == OP_DIV) {
-                q = C{tk.kxh * sx.x + tk.kyh * sy.x + kzh * sz.x, tk.kxh * sx.y + tk.kyh * sy.y + kzh * sz.y};
+                const T k0 = kcomp<T>(a.kmap, 0, tk.kxh, tk.kyh, kzh);
+                const T k1 = kcomp<T>(a.kmap, 1, tk.kxh, tk.kyh, kzh);
+                const T k2 = kcomp<T>(a.kmap, 2, tk.kxh, tk.kyh, kzh);
+                if (a.kmap == KMAP_2D_DIV)   // the third component is not loaded data
+                    q = C{k0 * sx.x + k1 * sy.x, k0 * sx.y + k1 * sy.y};
+                else
+                    q = C{k0 * sx.x + k1 * sy.x + k2 * sz.x, k0 * sx.y + k1 * sy.y + k2 * sz.y};
             } else if (o == 0) {
                 q = C{tk.kyh * sz.x - kzh * sy.x, tk.kyh * sz.y - kzh * sy.y};
             } else if (o == 1) {
@@ -162,10 +178,8 @@ __device__ __forceinline__ void kspace_op_inplace(cplx<T> (&v)[E], const PencilA
         if (op == OP_PLAIN) {
             v[u] = cscale(v[u], g);
         } else if (op == OP_GRAD) {
-            T kd;
-            if (comp == 0) kd = tk.kxh;
-            else if (comp == 1) kd = tk.kyh;
-            else kd = (2 * kz == a.Mz) ? T(0) : (T)kappa_full(kz, a.Mz, a.dk[2]);
+            const T kzh = (2 * kz == a.Mz) ? T(0) : (T)kappa_full(kz, a.Mz, a.dk[2]);
+            const T kd = kcomp<T>(a.kmap, comp, tk.kxh, tk.kyh, kzh);
             const C s = v[u];
             v[u] = C{-kd * g * s.y, kd * g * s.x};
         } else {

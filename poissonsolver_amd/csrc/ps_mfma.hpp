@@ -326,11 +326,8 @@ __device__ __forceinline__ void mf_kfactors(float (&f)[8], const PencilArgs<floa
                 if constexpr (OPK == OP_PLAIN) {
                     v = g;
                 } else {
-                    float kd;
-                    if (comp == 0) kd = tk.kxh;
-                    else if (comp == 1) kd = tk.kyh;
-                    else kd = (2 * kz == a.Mz) ? 0.0f : (float)kappa_full(kz, a.Mz, a.dk[2]);
-                    v = kd * g;
+                    const float kzh = (2 * kz == a.Mz) ? 0.0f : (float)kappa_full(kz, a.Mz, a.dk[2]);
+                    v = kcomp<float>(a.kmap, comp, tk.kxh, tk.kyh, kzh) * g;
                 }
             } else {
                 const double kzf = kappa_full(kz, a.Mz, a.dk[2]);

@@ -1,10 +1,11 @@
 """Python mirror of the reference's `poisson_solver` class (poisson_solver.hpp:39-124).
 
-Same method names and argument meaning: ``setup3d(ncell, bc, dx, reg_order=0)``,
-``set_return_grad/div/curl(bool)``, ``set_reprojection(bool)``, ``push(bx, by,
-bz)`` with None for an absent component, ``solve3d()``, ``pull(ax, ay, az, bx,
-by, bz)`` filling caller-owned float64 arrays, and the public ``partition``
-list.  Arrays hold nx*ny*nz doubles with x fastest (numpy shape (nz, ny, nx)).
+Same method names and argument meaning: ``setup3d(ncell, bc, dx, reg_order=0)``
+and ``setup2d(ncell, bc, dx, reg_order=0)``, ``set_return_grad/div/curl(bool)``,
+``set_reprojection(bool)``, ``push(bx, by, bz)`` with None for an absent
+component, ``solve3d()`` / ``solve2d()``, ``pull(ax, ay, az, bx, by, bz)``
+filling caller-owned float64 arrays, and the public ``partition`` list.  Arrays
+hold nx*ny*nz doubles with x fastest (numpy shape (nz, ny, nx); (ny, nx) in 2D).
 Like the reference, a failing call prints a message to stderr and returns;
 ``last_status`` holds the status code.  The solve runs on the GPU through the
 C ABI; there is no CPU path.
@@ -73,6 +74,9 @@ class poisson_solver:
         if not self._report(self._lib.ps_setup3d(self._ctx, n, b, d, int(reg_order)), "setup3d"):
             return
         self._n = tuple(int(v) for v in ncell)
+        self._refresh_partition()
+
+    def _refresh_partition(self):
         self.partition = []
         for r in range(self._lib.ps_partition_count(self._ctx)):
             nc = (ctypes.c_int * 3)()
@@ -82,10 +86,14 @@ class poisson_solver:
             self.partition.append(class_partition(list(nc), list(ic), list(dd)))
 
     def setup2d(self, ncell, bound_cond, dx, reg_order: int = 0):
+        """setup2d.cpp:8-72 (kernels of greens2d.cpp:8-441)."""
         n = (ctypes.c_int * 2)(*[int(v) for v in ncell])
         b = (ctypes.c_int * 2)(*[int(v) for v in bound_cond])
         d = (ctypes.c_double * 2)(*[float(v) for v in dx])
-        self._report(self._lib.ps_setup2d(self._ctx, n, b, d, int(reg_order)), "setup2d")
+        if not self._report(self._lib.ps_setup2d(self._ctx, n, b, d, int(reg_order)), "setup2d"):
+            return
+        self._n = (int(ncell[0]), int(ncell[1]), 1)
+        self._refresh_partition()
 
     # -- options (settings.cpp:9-30) ---------------------------------------
     def set_return_grad(self, toggle: bool):
@@ -122,6 +130,7 @@ class poisson_solver:
         self._report(self._lib.ps_solve3d(self._ctx), "solve3d")
 
     def solve2d(self):
+        """solve2d.cpp:8-475."""
         self._report(self._lib.ps_solve2d(self._ctx), "solve2d")
 
     def pull(self, ax=None, ay=None, az=None, bx=None, by=None, bz=None):

@@ -40,3 +40,27 @@ def gpu_solve(n, bc, dx, reg, rhs, op="plain", reproject=False, prec="fp64", wan
 def random_rhs(n, seed, ncomp=3):
     rng = np.random.default_rng(seed)
     return [rng.standard_normal((n[2], n[1], n[0])) for _ in range(ncomp)]
+
+
+def gpu_solve2d(n, bc, dx, reg, rhs, op="plain", prec="fp64", want_rhs=False, solver=None):
+    """setup2d -> push -> solve2d -> pull on the GPU; arrays of shape (ny, nx)."""
+    s = solver or P.poisson_solver(PREC[prec])
+    if solver is None:
+        s.setup2d(n, bc, dx, reg)
+        assert s.last_status == 0, s.last_status
+    s.set_return_grad(op == "grad")
+    s.set_return_div(op == "div")
+    s.set_return_curl(op == "curl")
+    rhs = list(rhs) + [None] * (3 - len(rhs))
+    s.push(*[None if r is None else np.ascontiguousarray(r, np.float64) for r in rhs])
+    assert s.last_status == 0
+    s.solve2d()
+    assert s.last_status == 0, s.last_status
+    shape = (n[1], n[0])
+    lhs = [np.full(shape, np.nan) for _ in range(3)]
+    rb = [np.full(shape, np.nan) for _ in range(3)]
+    s.pull(*lhs, *(rb if want_rhs else [None] * 3))
+    assert s.last_status == 0
+    if solver is None:
+        s.close()
+    return lhs, rb

@@ -62,8 +62,16 @@ def test_host_validation_without_device():
     assert lib.ps_solve3d(ctx) == _capi.PS_ESTATE
     assert lib.ps_push(ctx, None, None, None) == _capi.PS_ESTATE
     assert lib.ps_set_precision(ctx, 7) == _capi.PS_EINVAL
-    assert lib.ps_setup2d(ctx, n, b, dx, 0) == _capi.PS_EUNSUPPORTED
-    assert lib.ps_solve2d(ctx) == _capi.PS_EUNSUPPORTED
+    # setup2d: argument checks run before any device call (greens2d.cpp:430-434)
+    b2 = (ctypes.c_int * 2)(0, 2)
+    assert lib.ps_setup2d(ctx, n, b2, dx, 0) == _capi.PS_EBC
+    assert "Boundary condition configuration unknown" in lib.ps_last_error(ctx).decode()
+    n2 = (ctypes.c_int * 2)(8, 0)
+    assert lib.ps_setup2d(ctx, n2, b, dx, 0) == _capi.PS_EINVAL
+    assert lib.ps_setup2d(ctx, n, b, dx, -2) == _capi.PS_EINVAL
+    assert lib.ps_setup2d(ctx, n, b, dx, 0) == _capi.PS_EHIP       # valid, but no device here
+    assert lib.ps_solve2d(ctx) == _capi.PS_ESTATE
+    assert lib.ps_is_2d(ctx) == 0
     assert lib.ps_partition_count(ctx) == 0
     assert lib.ps_destroy(ctx) == _capi.PS_OK
     assert lib.ps_destroy(None) == _capi.PS_EINVAL
