@@ -16,6 +16,9 @@ Prints one JSON line.
       64 independent unbounded scalar problems, n from {32,...,192} (seed 4),
       sources = sums of 1-4 Gaussian blobs; one step = one hipGraph launch
       solving the whole batch.
+  python bench.py --workload uu2d [--n 4096]
+      2D solver: the reference's unbounded2d case (bump source on [-1,1]^2,
+      grad) at n^2, FP64 with FP32 beside it; one step = one solve2d.
 """
 from __future__ import annotations
 
@@ -111,10 +114,12 @@ def calib_copy():
     del src, dst
 
 
-def measure_traffic(argv_child, kernel_key, timeout=240):
+def measure_traffic(argv_child, kernel_key, timeout=240, take_last=0):
     """HBM bytes per dispatch of the dominant kernel from two rocprofv3 --pmc
     runs (FETCH_SIZE, then WRITE_SIZE) of this script in a child process, each
     scaled by what the counter reads for the 1 GiB calibration copy.
+    take_last > 0: average only the last take_last matching dispatches (the
+    timed solves; set-up kernels of the same name run first).
     Returns (traffic_bytes, detail) or (None, reason)."""
     import csv
     import shutil
@@ -141,7 +146,10 @@ def measure_traffic(argv_child, kernel_key, timeout=240):
                     rows += list(csv.DictReader(open(os.path.join(root, f))))
         shutil.rmtree(d, ignore_errors=True)
         copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
-        kern = [float(r["Counter_Value"]) for r in rows if kernel_key(r["Kernel_Name"])]
+        krows = [r for r in rows if kernel_key(r["Kernel_Name"])]
+        if take_last and krows and "Dispatch_Id" in krows[0]:
+            krows = sorted(krows, key=lambda r: int(r["Dispatch_Id"]))[-take_last:]
+        kern = [float(r["Counter_Value"]) for r in krows]
         if not copy or not kern:
             return None, "%s: counters missing (copy %d, kernel %d rows)" % (ctr, len(copy), len(kern))
         factor = (CALIB_BYTES / 1024.0) / max(copy)      # KiB the counter should read / KiB it read
@@ -357,6 +365,122 @@ def bench_config(a):
     return 0
 
 
+def bump2d(n):
+    """Source and analytic grad of the reference's unbounded2d test
+    (c++/test/unbounded2d/sources/main.cpp:143-170, :230-249; c = 10, r0 = 1)
+    at cell centres of [-1,1]^2, arrays (ny, nx)."""
+    h = 2.0 / n
+    x = -1.0 + (np.arange(n) + 0.5) * h
+    X, Y = x[None, :], x[:, None]
+    r2 = X * X + Y * Y
+    inside = r2 < 1.0
+    d = np.where(inside, 1.0 - r2, 1.0)
+    e = np.where(inside, np.exp(-10.0 / d), 0.0)
+    B = np.where(inside, 40.0 * e * (1.0 - X ** 4 - Y ** 4 - 2 * X ** 2 * Y ** 2 - 10.0 * r2) / d ** 4, 0.0)
+    Ax = np.where(inside, -20.0 * X * e / d ** 2, 0.0)
+    Ay = np.where(inside, -20.0 * Y * e / d ** 2, 0.0)
+    return B, Ax, Ay, h
+
+
+def alg_bytes_2d(n, cin, cout, eb):
+    """Algorithmic HBM bytes of the 2D pipeline (n^2 unbounded): R2C x pass,
+    fused y pass (pad, FFT, Ghat*zeta*op, IFFT, truncate; real quadrant of the
+    kernel), C2R x pass.  cin spectra in, cout out."""
+    H = n + 1
+    er, ec = eb, 2 * eb
+    P1 = cin * (n * n * er + H * n * ec)
+    P3 = cin * H * n * ec + H * (n + 1) * er + cout * H * n * ec
+    P5 = cout * (H * n * ec + n * n * er)
+    return dict(x=P1, z=P3, x_inv=P5, solve=P1 + P3 + P5)
+
+
+def cpu_baseline_2d(n_target):
+    """The oracle (numpy FP64) 2D grad solve timed at 1024^2 and scaled to
+    n_target by n^2 log n."""
+    from oracle import oracle as O
+    ns = 1024
+    B, _, _, h = bump2d(ns)
+    reps, t = 0, 0.0
+    while reps < 3 and t < 20.0:
+        t0 = time.perf_counter()
+        O.solve2d((ns, ns), (0, 0), (h, h), 0, [B], op="grad")
+        t += time.perf_counter() - t0
+        reps += 1
+    per = t / reps
+    scale = (n_target / ns) ** 2 * math.log2(2 * n_target) / math.log2(2 * ns)
+    return {"value": 1.0 / (per * scale), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": "oracle (numpy FP64 C2C restatement of solve2d) setup2d+push+solve2d+pull, grad, at "
+                      "%d^2 UU reg0, %d reps, %.2f s/solve, scaled to %d^2 by n^2 log n (x%.0f)"
+                      % (ns, reps, per, n_target, scale)}
+
+
+def bench_2d(a):
+    import poissonsolver_amd as P
+    n = a.n if a.n != 512 else 4096
+    B, Ax, Ay, h = bump2d(n)
+    codes = {"fp32": P.PS_FP32, "mixed": P.PS_MIXED, "fp64": P.PS_FP64}
+    precs = ["fp64", "fp32"] if not a.pmc_child else [a.prec]
+    res = {}
+    for prec in precs:
+        s = P.poisson_solver(codes[prec])
+        s.setup2d((n, n), (0, 0), (h, h), 0)
+        s.set_return_grad(True)
+        s.push(B)
+        if s.last_status:
+            return 1
+        for _ in range(a.warmup):
+            s.solve2d()
+        s.synchronize()
+        t0 = time.perf_counter()
+        prof = s.profile_solve(a.steps)
+        wall = time.perf_counter() - t0
+        gx, gy = np.empty_like(B), np.empty_like(B)
+        s.pull(gx, gy)
+        err = _rel([gx, gy], [Ax, Ay])
+        res[prec] = (prof, wall, err, s.zpass_variant())
+        s.close()
+    prec = precs[0]
+    prof, wall, err, zvar = res[prec]
+    eb = 8 if prec == "fp64" else 4
+    Bb = alg_bytes_2d(n, 1, 2, eb)
+    achieved = Bb["z"] / (prof["z"] * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": 1000.0 / prof["solve"], "unit": "solves/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": prof["solve"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": {"fp64": "f64", "fp32": "f32", "mixed": "f32"}[prec], "data": "synthetic",
+        "config": {"workload": "uu2d: %d^2 fully unbounded 2D solve (solve2d), FFT domain doubled to %d^2, reg 0, "
+                               "the reference's unbounded2d bump source, grad (2 components)" % (n, 2 * n),
+                   "ncell": [n, n], "bc": [0, 0], "reg_order": 0, "op": "grad", "precision": prec},
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "fused y pass (%s kernel: pad, FFT, Ghat*op for 2 outputs, IFFT, truncate)" % zvar,
+            "alg_bytes_per_launch": Bb["z"], "avg_ms": prof["z"]},
+        "solve_hbm_gbs": Bb["solve"] / (prof["solve"] * 1e-3) / 1e9,
+        "alg_bytes_per_solve": Bb["solve"],
+        "pass_ms": {k: prof[k] for k in ("x", "z", "x_inv")},
+        "pass_gbs": {k: Bb[k] / (prof[k] * 1e-3) / 1e9 for k in ("x", "z", "x_inv")},
+        "rel_l2_vs_analytic_grad": err, "wall_s_timed": wall,
+    }
+    if len(precs) == 2:
+        p32, _, e32, zv32 = res["fp32"]
+        out["fp32_path"] = {"value": 1000.0 / p32["solve"], "ms_per_step": p32["solve"], "z_kernel": zv32,
+                            "pass_ms": {k: p32[k] for k in ("x", "z", "x_inv")},
+                            "rel_l2_vs_analytic_grad": e32}
+    if not (a.no_traffic or a.pmc_child):
+        child = ["--pmc-child", "--workload", "uu2d", "--no-traffic", "--no-cpu", "--steps", "2", "--warmup", "1",
+                 "--n", str(n), "--prec", prec]
+        traffic, detail = measure_traffic(child, lambda name: "k_pencil" in name or "k_zfused_w<" in name,
+                                          take_last=3)
+        out["roofline"]["traffic"] = traffic
+        out["roofline"]["traffic_detail"] = detail
+    if not (a.no_cpu or a.pmc_child):
+        out["cpu_baseline"] = cpu_baseline_2d(n)
+    if not a.pmc_child:
+        print(json.dumps(out), flush=True)
+    return 0
+
+
 def bench_batch(a):
     import poissonsolver_amd as P
 
@@ -446,7 +570,8 @@ def main(argv=None):
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the rocprofv3 FETCH_SIZE/WRITE_SIZE passes that fill roofline.traffic")
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "ppu256", "vortex512", "large1024"])
+    ap.add_argument("--workload", default="single",
+                    choices=["single", "batch", "ppu256", "vortex512", "large1024", "uu2d"])
     ap.add_argument("--nprob", type=int, default=64)
     ap.add_argument("--autotune", action="store_true", help="batch: autotune each distinct size first")
     a = ap.parse_args(argv)
@@ -460,6 +585,8 @@ def main(argv=None):
             return 0
     if a.workload == "batch":
         return bench_batch(a)
+    if a.workload == "uu2d":
+        return bench_2d(a)
     if a.workload in ("ppu256", "vortex512", "large1024"):
         return bench_config(a)
 

@@ -533,7 +533,9 @@ template <typename T> int setup_impl(ps_ctx* c)
         if (d->tile < 1 || d->p.TP * d->tile > kMaxThreads)
             return fail(c, PS_EINVAL, "tile shape exceeds the workgroup size bound");
         if (!d->stat && (size_t)d->tile * lds_len(d->p.L) * 2 * eb > 160 * 1024)
-            return fail(c, PS_EINVAL, "tile shape exceeds the LDS of a compute unit");
+            return d->tile == 1 ? fail(c, PS_EUNSUPPORTED, "transform length " + std::to_string(d->p.L) +
+                                                               " exceeds the pencil held in LDS at this precision")
+                                : fail(c, PS_EINVAL, "tile shape exceeds the LDS of a compute unit");
     }
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3})
         if ((r = upload_twiddles<T>(c, d))) return r;

@@ -177,6 +177,11 @@ static int dct_pass(double* g, const int O[3], int d, int M, hipStream_t s, std:
     a.n_out = M / 2 + 1;
     a.ncol = ncol;
     a.B = choose_B(p, 8, ncol);
+    if ((size_t)a.B * lds_len(M) * sizeof(double2) > 160 * 1024) {
+        // one FP64 pencil of the kernel transform is staged whole in LDS
+        if (err) *err = "kernel transform length " + std::to_string(M) + " exceeds the FP64 pencil held in LDS";
+        return kGreensEUNSUPPORTED;
+    }
     a.ncoltile = (ncol + a.B - 1) / a.B;
     a.nslab = nslab;
     a.ncomp = 1;
