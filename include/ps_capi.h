@@ -86,6 +86,31 @@ int ps_partition(const ps_ctx* ctx, int rank, int ncell[3], int icell[3], double
 const char* ps_strerror(int status);
 const char* ps_last_error(const ps_ctx* ctx);
 
+/* --- the reference's Fortran interface (fortran/source/poisson_solver/) ---
+   A Fortran binding keeps one ps_ctx per solver index of
+   poisson_solver_initialise(nsolvers) (poisson_solver_initialise.f:8) and
+   binds the routines below; the 3-array push/pull, setup2d/3d, the setters
+   and solve2d/3d bind the functions above.
+   Strided block: element (c, i, j, k) of component c < ncomp and physical
+   cell (i, j, k) is base[c*stride[0] + (first[0]+i)*stride[1] +
+   (first[1]+j)*stride[2] + (first[2]+k)*stride[3]] -- for a Fortran array
+   real_rhs(ncomp, lx:ux, ly:uy, lz:uz) passed as c_loc(real_rhs(1,lx,ly,lz)):
+   stride = (1, ncomp, ncomp*nx', ncomp*nx'*ny'), first = offset - (lx, ly, lz)
+   with nx' = ux-lx+1 etc.  ncomp 1, 2, 3 = X; X, Y; X, Y, Z.  2D contexts
+   take k = 0 (first[2] = 0). */
+/* replaces poisson_solver_push_1pointer3d (poisson_solver_push.f:8-419) */
+int ps_push_strided(ps_ctx* ctx, const double* base, int ncomp, const long stride[4], const long first[3]);
+/* replaces poisson_solver_pull_1pointer3d (poisson_solver_pull.f:8-493): lhs
+   components X..ncomp-1 the solver holds, then (rhs_base non-NULL) the rhs
+   ones; ghost cells and components the solver does not hold are left as they
+   are */
+int ps_pull_strided(ps_ctx* ctx, double* lhs_base, int lhs_ncomp, const long lhs_stride[4], double* rhs_base,
+                    int rhs_ncomp, const long rhs_stride[4], const long first[3]);
+/* replaces poisson_solver_smooth3d(ps, h) (poisson_solver_smooth3d.f:8-401):
+   every held rhs component becomes Re(IFFT(exp(-h^2 |k|^2 / 2) FFT(rhs))) on
+   the domain-doubled grid; 3D contexts only */
+int ps_smooth3d(ps_ctx* ctx, double h);
+
 /* --- extensions: device-resident data, streams, timing (no reference counterpart) --- */
 /* real element type of device arrays: double for PS_FP64, float otherwise */
 int ps_real_bytes(const ps_ctx* ctx);

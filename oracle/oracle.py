@@ -33,6 +33,8 @@ How each part is pinned (see the tests named in brackets):
     fields of the reference's unbounded2d, periodic2d and mixed2d tests (and
     the transposed mixed case for UP), a 2D Gaussian closed form and a direct
     lattice sum [test_oracle_pins2d.py].
+  * smooth3d (the Fortran interface's Gaussian filter): the Gaussian-of-
+    Gaussian closed form and a periodic single mode [test_oracle_pins.py].
 """
 from __future__ import annotations
 
@@ -419,6 +421,33 @@ class OracleSolver:
                 self.rhs[d] = inverse(r[d])
             if lflag[d]:
                 self.lhs[d] = inverse(lk[d])
+
+    # fortran/source/poisson_solver/poisson_solver_smooth3d.f:8-401: every
+    # held rhs component is replaced by Re(IFFT(exp(-h^2 |k|^2 / 2) FFT(rhs)))
+    # on the x-, y- and z-extended grid (y zero-filled by the map, z padded
+    # in the z pass), truncated back to the physical cells
+    def smooth3d(self, h):
+        g = self.greens
+        nx, ny, nz = self.ncell
+        Mx, My = self.Mx, self.My
+        ug = self.use_glassman
+        k2 = (g.ikx.imag[None, None, :] ** 2 + g.iky.imag[None, :, None] ** 2
+              + g.ikz.imag[:, None, None] ** 2)
+        filt = np.exp(-0.5 * h * h * k2)                           # :187, :236-240
+        for d in range(3):
+            if self.rhs[d] is None:
+                continue
+            a = _fft(self.rhs[d], 2, False, ug)                    # x :77-121
+            b = np.zeros((nz, My, Mx), dtype=np.complex128)        # map :123, y fft :124-177
+            b[:, :ny, :] = a
+            b = _fft(b, 1, False, ug)
+            nfft = 2 * nz if self.bc[2] == 0 else nz               # z pad + fft :179-224
+            c = np.zeros((nfft, My, Mx), dtype=np.complex128)
+            c[:nz] = b
+            c = _fft(c, 0, False, ug) * filt
+            c = _fft(c, 0, True, ug)[:nz] / float(nfft)            # :256-270
+            c = _fft(c, 1, True, ug)[:, :ny, :] / float(My)        # :272-332
+            self.rhs[d] = _fft(c, 2, True, ug) / float(Mx)         # :333-386
 
     # settings.cpp:9-30
     def set_return_grad(self, t): self.lhs_grad = bool(t)

@@ -16,6 +16,7 @@ PS_FP64, PS_FP32, PS_MIXED = 0, 1, 2
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_c_long_p = ctypes.POINTER(ctypes.c_long)
 
 TUNING_FIELDS = ("zblk", "tile_x", "tile_y", "tile_z", "radix_e", "lds_kb", "xcd", "specialised",
                  "specialised3", "twreg_z", "twreg_y", "occ4_z", "zwave", "halfb_z", "halfb_y", "mfma")
@@ -62,6 +63,10 @@ SIGNATURES = {
     "ps_solve3d_async": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_solve2d_async": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_is_2d": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_push_strided": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_long_p, _c_long_p]),
+    "ps_pull_strided": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_long_p, ctypes.c_void_p,
+                                       ctypes.c_int, _c_long_p, _c_long_p]),
+    "ps_smooth3d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "ps_zpass_variant": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_dft32_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
     "ps_tuning_default": (ctypes.c_int, [_tun_p]),

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -25,6 +26,7 @@
 
 #include "../../include/ps_capi.h"
 #include "ps_greens.hpp"
+#include "ps_io.hpp"
 #include "ps_launch.hpp"
 #include "ps_mfma_launch.hpp"
 #include "ps_plan.hpp"
@@ -349,9 +351,12 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
 // The five passes (three for a 2D context).  `src` holds nin real components
 // starting at in_first with step in_step (step 0: one component transformed
 // nin times); `dst` receives nout components at out_first/out_step.
+// zarg / reg >= 0 replace the regularisation filter's argument scale and
+// order (the Gaussian of smooth3d is order 2 with zarg = h^2 / 2).
 template <typename T>
 int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin, int op, int nc, void* dst,
-                 int out_first, int out_step, int nout, hipEvent_t* ev, int kmap = KMAP_3D)
+                 int out_first, int out_step, int nout, hipEvent_t* ev, int kmap = KMAP_3D, double zarg = -1.0,
+                 int reg = -1)
 {
     hipStream_t s = c->stream;
     const int nx = c->ncell[0], ny = c->ncell[1], nz = c->ncell[2];
@@ -408,6 +413,8 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     fill_zargs<T>(c, za, zp);
     za.op = op;
     za.kmap = kmap;
+    if (zarg >= 0.0) za.zarg = zarg;
+    if (reg >= 0) za.reg = reg;
     za.ncomp = (nc == 3) ? 1 : nin;
     bool mfma_done = false;
     if constexpr (sizeof(T) == 4) {
@@ -687,6 +694,35 @@ int rhs_source(ps_ctx* c, const void** src)
     return PS_OK;
 }
 
+// A strided block's cell (i, j, k) of the user grid in the context's 3D form:
+// a 2D context holds (nx, ny) as (nx, 1, ny), so user j is internal k.
+bool strided_args(const ps_ctx* c, const long stride[4], const long first[3], int ncomp, Strided* st, long* off,
+                  long* span)
+{
+    if (ncomp < 1 || ncomp > 3) return false;
+    long s[4] = {stride[0], stride[1], stride[2], stride[3]};
+    long f[3] = {first[0], first[1], first[2]};
+    const int n[3] = {c->ncell[0], c->dim2 ? c->ncell[2] : c->ncell[1], c->dim2 ? 1 : c->ncell[2]};
+    for (int d = 0; d < 4; ++d)
+        if (s[d] < 1) return false;
+    for (int d = 0; d < 3; ++d)
+        if (f[d] < 0) return false;
+    *off = f[0] * s[1] + f[1] * s[2] + f[2] * s[3];
+    *span = (long)(ncomp - 1) * s[0] + (f[0] + n[0] - 1) * s[1] + (f[1] + n[1] - 1) * s[2] +
+            (f[2] + n[2] - 1) * s[3] + 1;
+    if (c->dim2) *st = Strided{s[0], s[1], 0, s[2]};   // internal j = 0, internal k = user j
+    else *st = Strided{s[0], s[1], s[2], s[3]};
+    return true;
+}
+
+struct DevScratch {
+    void* p = nullptr;
+    ~DevScratch()
+    {
+        if (p) (void)hipFree(p);
+    }
+};
+
 }  // namespace
 
 extern "C" {
@@ -796,6 +832,84 @@ int ps_push_device(ps_ctx* c, const void* bx, const void* by, const void* bz)
     }
     CK(hipStreamSynchronize(c->stream));
     c->solved_after_push = false;
+    c->rhsf_valid = false;
+    return PS_OK;
+}
+
+int ps_push_strided(ps_ctx* c, const double* base, int ncomp, const long stride[4], const long first[3])
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!base || !stride || !first) return fail(c, PS_EINVAL, "push_strided: NULL argument");
+    Strided st;
+    long off, span;
+    if (!strided_args(c, stride, first, ncomp, &st, &off, &span))
+        return fail(c, PS_EINVAL, "push_strided: ncomp must be 1..3, strides positive and offsets non-negative");
+    if ((r = ensure_real(c, &c->rhs))) return r;
+    DevScratch stage;
+    CK(hipMalloc(&stage.p, (size_t)span * sizeof(double)));
+    CK(hipMemcpyAsync(stage.p, base, (size_t)span * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CK(launch_gather(static_cast<const double*>(stage.p) + off, st, c->rhs, real_bytes(c), ncomp, c->ncell[0],
+                     c->ncell[1], c->ncell[2], c->stream));
+    CK(hipStreamSynchronize(c->stream));
+    for (int d = 0; d < ncomp; ++d) c->rhs_present[d] = true;
+    c->solved_after_push = false;
+    c->rhsf_valid = false;
+    return PS_OK;
+}
+
+int ps_pull_strided(ps_ctx* c, double* lhs_base, int lhs_ncomp, const long lhs_stride[4], double* rhs_base,
+                    int rhs_ncomp, const long rhs_stride[4], const long first[3])
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!first || (lhs_base && !lhs_stride) || (rhs_base && !rhs_stride))
+        return fail(c, PS_EINVAL, "pull_strided: NULL argument");
+    for (int which = 0; which < 2; ++which) {
+        double* base = which ? rhs_base : lhs_base;
+        if (!base) continue;
+        const int ncomp = which ? rhs_ncomp : lhs_ncomp;
+        Strided st;
+        long off, span;
+        if (!strided_args(c, which ? rhs_stride : lhs_stride, first, ncomp, &st, &off, &span))
+            return fail(c, PS_EINVAL, "pull_strided: ncomp must be 1..3, strides positive and offsets non-negative");
+        const bool* held = which ? c->rhs_present : c->lhs_present;
+        int mask = 0;
+        for (int d = 0; d < ncomp; ++d)
+            if (held[d]) mask |= 1 << d;
+        if (!mask) continue;
+        const void* src = c->lhs;
+        if (which && (r = rhs_source(c, &src))) return r;
+        // the block's ghost cells and unheld components keep their values
+        DevScratch stage;
+        CK(hipMalloc(&stage.p, (size_t)span * sizeof(double)));
+        CK(hipMemcpyAsync(stage.p, base, (size_t)span * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        CK(launch_scatter(src, real_bytes(c), static_cast<double*>(stage.p) + off, st, mask, ncomp, c->ncell[0],
+                          c->ncell[1], c->ncell[2], c->stream));
+        CK(hipMemcpyAsync(base, stage.p, (size_t)span * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CK(hipStreamSynchronize(c->stream));
+    }
+    return PS_OK;
+}
+
+int ps_smooth3d(ps_ctx* c, double h)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->dim2) return fail(c, PS_ESTATE, "smooth3d: the solver was set up by setup2d");
+    if (!(h >= 0.0) || !std::isfinite(h)) return fail(c, PS_EINVAL, "smooth3d: h must be finite and >= 0");
+    int first, step, n;
+    if (!c->rhs || !arith(c->rhs_present, &first, &step, &n)) return PS_OK;   // nothing held: no-op
+    // exp(-h^2 |k|^2 / 2) / N is the order-2 filter with argument scale h^2 / 2
+    const double zarg = 0.5 * h * h;
+    r = (c->prec == PS_FP64)
+            ? run_pipeline<double>(c, c->rhs, first, step, n, OP_RHSF, 1, c->rhs, first, step, n, nullptr,
+                                   KMAP_3D, zarg, 2)
+            : run_pipeline<float>(c, c->rhs, first, step, n, OP_RHSF, 1, c->rhs, first, step, n, nullptr, KMAP_3D,
+                                  zarg, 2);
+    if (r) return r;
+    CK(hipStreamSynchronize(c->stream));
+    c->solved_after_push = false;      // the held rhs is the smoothed field
     c->rhsf_valid = false;
     return PS_OK;
 }

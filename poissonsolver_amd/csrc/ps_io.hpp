@@ -1,0 +1,23 @@
+// Strided field blocks at the push/pull boundary: the one-array form of the
+// reference's Fortran interface (poisson_solver_push.f:8, poisson_solver_pull.f:8),
+// an array real(ncomp, :, :, :) with the component index fastest and ghost
+// cells around the physical cells.  Element (c, i, j, k) of such a block is
+//   base[c*s0 + i*s1 + j*s2 + k*s3]   (base already offset to cell (0, 0, 0)).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace psamd {
+
+struct Strided {
+    long s0, s1, s2, s3;
+};
+
+// dst[c][k][j][i] (solve precision, elem_bytes 4 or 8) = src(c, i, j, k), c < ncomp
+hipError_t launch_gather(const double* src, Strided st, void* dst, int elem_bytes, int ncomp, int nx, int ny,
+                         int nz, hipStream_t s);
+
+// dst(c, i, j, k) = src[c][k][j][i] for the components whose bit is set in cmask
+hipError_t launch_scatter(const void* src, int elem_bytes, double* dst, Strided st, int cmask, int ncomp, int nx,
+                          int ny, int nz, hipStream_t s);
+
+}  // namespace psamd

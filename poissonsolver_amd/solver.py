@@ -138,6 +138,51 @@ class poisson_solver:
         arrs = [self._host(a, True) for a in (ax, ay, az, bx, by, bz)]
         self._report(self._lib.ps_pull(self._ctx, *[_ptr(a) for a in arrs]), "pull")
 
+    # -- the Fortran interface's one-array forms ---------------------------
+    @staticmethod
+    def _block(arr, first, writable):
+        """A numpy array of shape (nz', ny', nx', ncomp) (C order: the memory
+        layout of Fortran real(ncomp, nx', ny', nz')) or (ny', nx', ncomp) in
+        2D, holding the physical cells from index `first` (x, y[, z])."""
+        if not isinstance(arr, np.ndarray) or arr.dtype != np.float64:
+            raise TypeError("blocks must be float64 numpy arrays")
+        if writable and not arr.flags.writeable:
+            raise TypeError("output blocks must be writable")
+        if arr.ndim not in (3, 4):
+            raise ValueError("blocks have 4 axes (3 in 2D)")
+        st = [v // 8 for v in arr.strides]
+        if any(v % 8 for v in arr.strides) or min(st) < 1:
+            raise ValueError("blocks need positive element strides")
+        if arr.ndim == 4:
+            stride = (ctypes.c_long * 4)(st[3], st[2], st[1], st[0])
+        else:   # (y, x, c): the z stride is never stepped (k = 0)
+            stride = (ctypes.c_long * 4)(st[2], st[1], st[0], st[0] * arr.shape[0])
+        f = list(first) + [0] * (3 - len(first))
+        return arr.shape[-1], stride, (ctypes.c_long * 3)(*[int(v) for v in f])
+
+    def push_strided(self, block, first=(0, 0, 0)):
+        """poisson_solver_push_1pointer3d (poisson_solver_push.f:8): the
+        components of `block` (1: X, 2: X, Y, 3: X, Y, Z) from cell `first` on."""
+        nc, st, f = self._block(block, first, False)
+        self._report(self._lib.ps_push_strided(self._ctx, ctypes.c_void_p(block.ctypes.data), nc, st, f),
+                     "push_strided")
+
+    def pull_strided(self, lhs_block, rhs_block=None, first=(0, 0, 0)):
+        """poisson_solver_pull_1pointer3d (poisson_solver_pull.f:8)."""
+        nl, sl, f = self._block(lhs_block, first, True)
+        if rhs_block is not None:
+            nr, sr, _ = self._block(rhs_block, first, True)
+            rb = ctypes.c_void_p(rhs_block.ctypes.data)
+        else:
+            nr, sr, rb = 0, None, None
+        self._report(self._lib.ps_pull_strided(self._ctx, ctypes.c_void_p(lhs_block.ctypes.data), nl, sl, rb, nr,
+                                               sr, f), "pull_strided")
+
+    def smooth3d(self, h: float):
+        """poisson_solver_smooth3d (poisson_solver_smooth3d.f:8): Gaussian
+        smoothing exp(-h^2 k^2 / 2) of every held rhs component."""
+        self._report(self._lib.ps_smooth3d(self._ctx, float(h)), "smooth3d")
+
     # -- extensions --------------------------------------------------------
     def push_device(self, bx=None, by=None, bz=None):
         """Device pointers (int addresses, e.g. torch tensor.data_ptr()) of the

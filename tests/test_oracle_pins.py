@@ -246,3 +246,41 @@ def test_uup_closed_form_2d_gaussian():
 def test_bad_boundary_tuple_rejected():
     with pytest.raises(ValueError):
         O.greens3d((4, 4, 4), (0, 1, 0), (0.1,) * 3, 0)
+
+
+# ------------------------------------------------------------- smooth3d ----
+def test_smooth3d_gaussian_and_single_mode():
+    """smooth3d(h) (fortran poisson_solver_smooth3d.f:187, :236-240) multiplies
+    the spectrum by exp(-h^2 |k|^2 / 2), the transform of a unit Gaussian of
+    standard deviation h: an unbounded Gaussian of width s becomes the
+    Gaussian of width sqrt(s^2 + h^2) with the same mass; a periodic single
+    mode is scaled by exp(-h^2 k^2 / 2)."""
+    n = 48
+    x, dx = cell_centres(n, -1.0, 1.0)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    r2 = X * X + Y * Y + Z * Z
+    s, h = 0.08, 0.06
+    g = lambda w: np.exp(-r2 / (2 * w * w)) / ((2 * np.pi) ** 1.5 * w ** 3)
+    o = O.OracleSolver()
+    o.setup3d((n, n, n), (0, 0, 0), (dx, dx, dx), 0)
+    o.push(g(s))
+    o.smooth3d(h)
+    got = o.rhs[0][:, :, :n].real
+    assert O.rel_l2(got, g(np.hypot(s, h))) < 1e-9
+    # x, y periodic, z unbounded: a mode sin(2 pi x) cos(4 pi y) times a z Gaussian
+    n3 = (16, 8, 64)
+    d3 = (1.0 / 16, 1.0 / 8, 2.0 / 64)
+    xs = np.arange(16) / 16.0
+    ys = np.arange(8) / 8.0
+    zs, _ = cell_centres(64, -1.0, 1.0)
+    Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+    f = np.sin(2 * np.pi * X) * np.cos(4 * np.pi * Y) * np.exp(-Z ** 2 / (2 * s * s))
+    o = O.OracleSolver()
+    o.setup3d(n3, (1, 1, 0), d3, 0)
+    o.push(f)
+    o.smooth3d(h)
+    kp2 = (2 * np.pi) ** 2 * 5
+    sz = np.hypot(s, h)
+    want = (np.exp(-0.5 * h * h * kp2) * np.sin(2 * np.pi * X) * np.cos(4 * np.pi * Y)
+            * (s / sz) * np.exp(-Z ** 2 / (2 * sz * sz)))
+    assert O.rel_l2(o.rhs[0][:, :, :16].real, want) < 1e-9
