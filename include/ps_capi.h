@@ -29,7 +29,8 @@ enum {
                              (solve3d.cpp:57-60, :71-74, :87-90) */
     PS_ENOMEM = 4,        /* device allocation failed */
     PS_EHIP = 5,          /* HIP runtime error, or no device */
-    PS_EUNSUPPORTED = 6   /* grid length with a prime factor above 7, 2D solver */
+    PS_EUNSUPPORTED = 6   /* grid length with a prime factor above 7, or a transform whose pencil
+                             does not fit the LDS of a compute unit */
 };
 
 /* precision of the solve (extension; the reference is FP64 only) */
@@ -106,6 +107,14 @@ int ps_push_strided(ps_ctx* ctx, const double* base, int ncomp, const long strid
    are */
 int ps_pull_strided(ps_ctx* ctx, double* lhs_base, int lhs_ncomp, const long lhs_stride[4], double* rhs_base,
                     int rhs_ncomp, const long rhs_stride[4], const long first[3]);
+/* output: the VTK writer of the reference's test programs
+   (c++/test/unbounded3d/sources/main.cpp:329-455): <dir>/<stem>_P00.vti, an
+   ImageData piece whose CellData holds nfield 3-component Float64 arrays
+   (host arrays comps[3*a + d], x fastest, NULL = zeros), and <dir>/<stem>.pvti
+   listing the piece.  binary = 0: ascii as the reference writes it (%17.8e);
+   1: raw appended data (full precision).  Host only; no context needed. */
+int ps_write_vti(const char* dir, const char* stem, const int ncell[3], const double origin[3], const double dx[3],
+                 int nfield, const char* const* names, const double* const* comps, int binary);
 /* replaces poisson_solver_smooth3d(ps, h) (poisson_solver_smooth3d.f:8-401):
    every held rhs component becomes Re(IFFT(exp(-h^2 |k|^2 / 2) FFT(rhs))) on
    the domain-doubled grid; 3D contexts only */

@@ -67,6 +67,8 @@ SIGNATURES = {
     "ps_pull_strided": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _c_long_p, ctypes.c_void_p,
                                        ctypes.c_int, _c_long_p, _c_long_p]),
     "ps_smooth3d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
+    "ps_write_vti": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
     "ps_zpass_variant": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_dft32_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
     "ps_tuning_default": (ctypes.c_int, [_tun_p]),

@@ -57,7 +57,8 @@ def sources():
         units.append(("skern.hip", ["-DPS_SL=%d" % L], "skern_%d.o" % L))
     # the matrix-core kernels take magnitudes with fmax; no NaN handling needed
     units.append(("mkern.hip", ["-fno-honor-nans"], "mkern.o"))
-    for f in ("ps_greens.hip", "ps_io.hip", "ps_dispatch.cpp", "ps_plan.cpp", "ps_capi.cpp", "ps_batch.cpp", "poisson_solver.cpp"):
+    for f in ("ps_greens.hip", "ps_io.hip", "ps_dispatch.cpp", "ps_plan.cpp", "ps_capi.cpp", "ps_batch.cpp", "poisson_solver.cpp",
+              "ps_vtk.cpp"):
         units.append((f, [], os.path.splitext(f)[0] + ".o"))
     return units
 

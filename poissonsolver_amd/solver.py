@@ -313,6 +313,35 @@ class batch:
         return ms.value
 
 
+def write_vti(directory, stem, ncell, origin, dx, fields, binary=False):
+    """VTK output of the reference's test programs
+    (c++/test/unbounded3d/sources/main.cpp:329-455): fields maps a name to a
+    tuple of up to 3 float64 arrays of nx*ny*nz cells (x fastest; None =
+    zeros).  Writes <directory>/<stem>_P00.vti and <directory>/<stem>.pvti."""
+    names = list(fields)
+    keep = []
+    ptrs = []
+    for nm in names:
+        comps = list(fields[nm]) + [None] * (3 - len(fields[nm]))
+        for c in comps:
+            if c is None:
+                ptrs.append(None)
+                continue
+            a = np.ascontiguousarray(c, dtype=np.float64)
+            if a.size != int(ncell[0]) * int(ncell[1]) * int(ncell[2]):
+                raise ValueError("field %s: %d cells, grid has %d" % (nm, a.size, int(np.prod(ncell))))
+            keep.append(a)
+            ptrs.append(a.ctypes.data)
+    cn = (ctypes.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+    cp = (ctypes.c_void_p * max(1, len(ptrs)))(*ptrs)
+    st = _capi.lib().ps_write_vti(str(directory).encode(), str(stem).encode(),
+                                  (ctypes.c_int * 3)(*[int(v) for v in ncell]),
+                                  (ctypes.c_double * 3)(*[float(v) for v in origin]),
+                                  (ctypes.c_double * 3)(*[float(v) for v in dx]), len(names), cn, cp, int(bool(binary)))
+    if st != _capi.PS_OK:
+        raise OSError("ps_write_vti: " + _capi.strerror(st))
+
+
 def plan_describe(L: int, ncomp: int = 1, precision_bytes: int = 8):
     """(E, threads per pencil, radices) the library picks for length L, or None."""
     E, tp, ns = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
