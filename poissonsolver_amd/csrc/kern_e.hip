@@ -12,6 +12,7 @@ template hipError_t launch_x_c2r_E<float, PS_E>(const XArgs<float>&, hipStream_t
 template hipError_t launch_x_c2r_E<double, PS_E>(const XArgs<double>&, hipStream_t);
 template hipError_t launch_pencil_E<float, PS_E, 1>(const PencilArgs<float>&, hipStream_t);
 template hipError_t launch_pencil_E<double, PS_E, 1>(const PencilArgs<double>&, hipStream_t);
+template hipError_t launch_pencil_E<float, PS_E, 2>(const PencilArgs<float>&, hipStream_t);
 template hipError_t launch_pencil_E<float, PS_E, 3>(const PencilArgs<float>&, hipStream_t);
 template hipError_t launch_pencil_E<double, PS_E, 3>(const PencilArgs<double>&, hipStream_t);
 }  // namespace psamd

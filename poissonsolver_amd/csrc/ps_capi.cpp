@@ -230,10 +230,16 @@ bool arith(const bool present[3], int* first, int* step, int* n)
 }
 
 // 2D operators (solve2d.cpp:292-326).  grad and curl differentiate one
-// scalar in two directions: the x pass transforms it twice (input step 0),
-// so each output component is an in-place component-wise z pass.
+// scalar in two directions.  In single precision with the runtime-plan
+// kernel (long pencils) the scalar is transformed once and one launch writes
+// both components (nc 2).  Otherwise the x pass transforms the scalar twice
+// (input step 0) and each output is an in-place one-component z pass: the
+// specialised, pencil-per-wavefront and matrix-core kernels are
+// component-wise, and the FP64 two-output kernel would exceed 256 registers
+// (its input spectrum and an output in flight together) and spill.
 int select_op2d(ps_ctx* c, OpSel* s)
 {
+    const bool one_read = c->prec != PS_FP64 && !c->pz1.stat && !c->zmfma && !c->zwave;
     const bool rX = c->rhs_present[0], rY = c->rhs_present[1], rZ = c->rhs_present[2];
     if (!arith(c->rhs_present, &s->in_first, &s->in_step, &s->nin))
         return fail(c, PS_ESTATE, "[poisson_solver.solve2d]: no rhs field has been pushed");
@@ -242,9 +248,9 @@ int select_op2d(ps_ctx* c, OpSel* s)
             return fail(c, PS_ESTATE,
                         "[poisson_solver.solve2d]: Error rhs fields have not been pushed correctly for grad option");
         s->op = OP_GRAD;
-        s->nc = 1;
+        s->nc = one_read ? 2 : 1;
         s->kmap = KMAP_2D_GRAD;
-        s->in_first = 0; s->in_step = 0; s->nin = 2;
+        s->in_first = 0; s->in_step = 0; s->nin = one_read ? 1 : 2;
         s->out_first = 0; s->out_step = 1; s->nout = 2;
     } else if (c->div) {
         if (!(rX && rY))
@@ -260,9 +266,9 @@ int select_op2d(ps_ctx* c, OpSel* s)
             return fail(c, PS_ESTATE,
                         "[poisson_solver.solve2d]: Error rhs fields have not been pushed correctly for curl option");
         s->op = OP_GRAD;
-        s->nc = 1;
+        s->nc = one_read ? 2 : 1;
         s->kmap = KMAP_2D_CURL;
-        s->in_first = 2; s->in_step = 0; s->nin = 2;
+        s->in_first = 2; s->in_step = 0; s->nin = one_read ? 1 : 2;
         s->out_first = 0; s->out_step = 1; s->nout = 2;
     } else {
         s->op = OP_PLAIN;
@@ -415,7 +421,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     za.kmap = kmap;
     if (zarg >= 0.0) za.zarg = zarg;
     if (reg >= 0) za.reg = reg;
-    za.ncomp = (nc == 3) ? 1 : nin;
+    za.ncomp = (nc == 3) ? 1 : nin;     // nc 2: one input, outputs 0 and 1
     bool mfma_done = false;
     if constexpr (sizeof(T) == 4) {
         if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, op)) {
@@ -426,6 +432,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
             mfma_done = true;
         }
     }
+    if (nc == 2 && zp.stat) return fail(c, PS_EINVAL, "one-input two-output z pass needs the runtime-plan kernel");
     if (mfma_done) {
     } else if (zp.stat) CK(static_launch_pencil<T>(zp.p.L, za, nc, SM_FUSED, c->bc[2] == 0, s));
     else CK(launch_pencil<T>(zp.p.E, nc, za, s));

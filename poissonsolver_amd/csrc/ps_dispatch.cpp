@@ -35,6 +35,17 @@ template <typename T> hipError_t launch_pencil(int E, int NC, const PencilArgs<T
         default: return hipErrorInvalidValue;
         }
     }
+    if (NC == 2) {   // one input, two outputs: single precision only
+        if constexpr (sizeof(T) == 4) {
+            switch (E) {
+#define PS_CASE(e) case e: return launch_pencil_E<T, e, 2>(a, s);
+                PS_FOR_EACH_E(PS_CASE)
+#undef PS_CASE
+            default: return hipErrorInvalidValue;
+            }
+        }
+        return hipErrorInvalidValue;
+    }
     switch (E) {
 #define PS_CASE(e) case e: return launch_pencil_E<T, e, 3>(a, s);
         PS_FOR_EACH_E(PS_CASE)

@@ -201,10 +201,15 @@ __device__ __forceinline__ void kspace_op_inplace(cplx<T> (&v)[E], const PencilA
 
 // ---------------------------------------------------------------------------
 // strided pencil pass (y passes, fused z pass, DCT-I of the kernel octant)
+// NC = 1: one component in, one out (launch component = blockIdx range);
+// NC = 2: one component in, two out (a 2D grad or curl of one scalar: the
+//         scalar is loaded and transformed once, output o is component o);
+// NC = 3: three components in, one (div) or three out.
 // ---------------------------------------------------------------------------
 template <typename T, int E, int NC>
 __global__ void __launch_bounds__(kMaxThreads) k_pencil(PencilArgs<T> a)
 {
+    constexpr int NV = (NC == 3) ? 3 : 1;   // input pencils held
     using C = cplx<T>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
@@ -224,10 +229,10 @@ __global__ void __launch_bounds__(kMaxThreads) k_pencil(PencilArgs<T> a)
     const bool cv = col < a.ncol;
     C* my = lds + p;
 
-    C v[NC][E];
+    C v[NV][E];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        const int cc = (NC == 1) ? comp : c;
+    for (int c = 0; c < NV; ++c) {
+        const int cc = (NC == 3) ? c : comp;
         const long base = cc * a.ia.comp + col * a.ia.col + boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
 #pragma unroll
         for (int u = 0; u < E; ++u) {
@@ -246,19 +251,19 @@ __global__ void __launch_bounds__(kMaxThreads) k_pencil(PencilArgs<T> a)
     }
     if (a.do_fwd) {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) fft_pencil<E, -1, T>(v[c], a.plan, my, B, t);
+        for (int c = 0; c < NV; ++c) fft_pencil<E, -1, T>(v[c], a.plan, my, B, t);
     }
 
     const int op = a.op;
-    const int nout = (NC == 1 || op == OP_DIV) ? 1 : 3;
+    const int nout = (NC == 3) ? (op == OP_DIV ? 1 : 3) : NC;
     const TileK<T> tk = tile_k(a, col, slab, cv);
 
     // o is a compile-time index in each unrolled copy, so v[o] stays in registers
 #pragma unroll
-    for (int o = 0; o < (NC == 1 ? 1 : 3); ++o) {
+    for (int o = 0; o < NC; ++o) {
         if (o >= nout) break;
         C w[E];
-        kspace_op<T, E, NC>(w, v, a, tk, o, comp, t, TP);
+        kspace_op<T, E, NV>(w, v, a, tk, o, (NC == 2) ? o : comp, t, TP);
         if (a.do_inv) fft_pencil<E, +1, T>(w, a.plan, my, B, t);
 
         const int oc = (NC == 1) ? comp : o;
@@ -414,7 +419,7 @@ template <typename T, int E, int NC> hipError_t launch_pencil_E(const PencilArgs
     hipError_t e = allow_lds(k_pencil<T, E, NC>, shm);
     if (e != hipSuccess) return e;
     const long ntile = (long)a.ncoltile * a.nslab;
-    const long nwork = ntile * (NC == 1 ? a.ncomp : 1);
+    const long nwork = ntile * (NC == 3 ? 1 : a.ncomp);
     hipLaunchKernelGGL((k_pencil<T, E, NC>), dim3((unsigned)nwork), dim3(a.plan.TP * a.B), shm, s, a);
     return hipGetLastError();
 }
