@@ -486,7 +486,15 @@ template <typename T> int setup_impl(ps_ctx* c)
     // [z][ky][kx] (y pass writes contiguous planes, z pencils strided by a
     // plane), zb = -1 is [ky][z][kx] (z pencils contiguous, y pass strided).
     int zb = tn.zblk;
-    if (zb == -2) zb = (nz >= 32) ? 4 : 0;    // 16-plane blocks: measured best at 512^3
+    if (zb == -2) {
+        // 16-plane blocks (measured best at 512^3), fewer when the specialised
+        // z kernels' threads per pencil (Mz / E) are fewer than 16: a block
+        // must not split a thread's positions
+        zb = (nz >= 32) ? 4 : 0;
+        int E1 = 0, B1 = 0, P1 = 0;
+        if (zb > 0 && tn.specialised && static_policy(c->M[2], real_bytes(c), 1, &E1, &B1, &P1))
+            while (zb > 0 && (c->M[2] / E1) % (1 << zb) != 0) --zb;
+    }
     const long My = c->M[1];
     if (zb > 0 && (1 << zb) >= nz) zb = -1;
     if (c->dim2) zb = 0;
@@ -561,6 +569,13 @@ template <typename T> int setup_impl(ps_ctx* c)
     c->zwave = tn.zwave && c->pz1.stat && c->pz1.p.L % zwE == 0 && (64 % (c->pz1.p.L / zwE) == 0) &&
                c->pz1.p.nstage >= 2 && (c->zb <= 0 || (c->pz1.p.L / zwE) % (1 << c->zb) == 0);
     if (c->zwave && zwE == 32) c->pz1.tile = tn.halfb_z ? kZwaveB32 / 2 : kZwaveB32;
+    if (c->zwave && zwE == 16) {
+        // the wavefront kernel is built for the policy's tile width and half
+        // of it; a quarter-width tile (halfb_z = 2) runs the fused kernel
+        int E1 = 0, B1 = 0, P1 = 0;
+        if (!static_policy(c->pz1.p.L, eb, 1, &E1, &B1, &P1) || (c->pz1.tile != B1 && c->pz1.tile != B1 / 2))
+            c->zwave = 0;
+    }
     // matrix-core z pass (PS_MIXED): its lanes read Gk along kx; positions
     // z0 + 32 u must be linear in u (z blocks of at most 32 planes, or plain)
     // 32-bit byte offsets: one component of A2 and the Gk octant below 4 GB

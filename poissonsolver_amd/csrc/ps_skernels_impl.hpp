@@ -349,17 +349,21 @@ template <typename T, int L> hipError_t s_launch_zwave(const PencilArgs<T>& a, i
 {
     using Pol = SPolicy<T, L>;
     constexpr int EW = zwave_E<T, L>();
-    if constexpr (EW == 32) {
-        // 2 wavefronts per SIMD (registers): one workgroup of 8 pencils per CU
-        // (8 x 16.9 KB LDS) reads 64-byte row segments
-        if constexpr (64 % (L / EW) == 0 && SFft<L, EW>::cp.n >= 2) {
-            if (a.B == kZwaveB32) return s_launch_zwave_b<T, L, kZwaveB32>(a, half, s);
-            if (a.B == kZwaveB32 / 2) return s_launch_zwave_b<T, L, kZwaveB32 / 2>(a, half, s);
-        }
-    } else if constexpr (64 % (L / EW) == 0 && SFft<L, EW>::cp.n >= 2) {
-        if (a.B == Pol::B) return s_launch_zwave_b<T, L, Pol::B>(a, half, s);
-        if constexpr (Pol::B >= 2) {
-            if (a.B == Pol::B / 2) return s_launch_zwave_b<T, L, Pol::B / 2>(a, half, s);
+    // a pencil must fit one wavefront with a power-of-two split (lengths
+    // with a factor 3 or 5 have no pencil-per-wavefront form)
+    if constexpr (L % EW == 0 && 64 % (L / EW) == 0 && s_plan_ok(L, EW)) {
+        if constexpr (EW == 32) {
+            // 2 wavefronts per SIMD (registers): one workgroup of 8 pencils per CU
+            // (8 x 16.9 KB LDS) reads 64-byte row segments
+            if constexpr (SFft<L, EW>::cp.n >= 2) {
+                if (a.B == kZwaveB32) return s_launch_zwave_b<T, L, kZwaveB32>(a, half, s);
+                if (a.B == kZwaveB32 / 2) return s_launch_zwave_b<T, L, kZwaveB32 / 2>(a, half, s);
+            }
+        } else if constexpr (SFft<L, EW>::cp.n >= 2) {
+            if (a.B == Pol::B) return s_launch_zwave_b<T, L, Pol::B>(a, half, s);
+            if constexpr (Pol::B >= 2) {
+                if (a.B == Pol::B / 2) return s_launch_zwave_b<T, L, Pol::B / 2>(a, half, s);
+            }
         }
     }
     return hipErrorInvalidValue;

@@ -4,8 +4,10 @@
 #include <hip/hip_runtime.h>
 
 #include "ps_kernels.hpp"
+#include "ps_sfft.hpp"
 
-#define PS_FOR_EACH_SL(X) X(64) X(128) X(256) X(512) X(1024) X(2048)
+// powers of two, and the doubled lengths of 48, 96, 160 and 192 cells
+#define PS_FOR_EACH_SL(X) X(64) X(96) X(128) X(192) X(256) X(320) X(384) X(512) X(1024) X(2048)
 
 namespace psamd {
 
@@ -35,10 +37,26 @@ constexpr int s_pick_P(int TP, int L, int cb)
     return P;
 }
 
-// NC = 3 (the z pass of curl, div and reprojection) holds three pencils per
-// thread, so it uses E = 8 to stay in registers.
+// Elements per thread: the first candidate whose radices (the divisors of E
+// in 16, 8, 7, 5, 4, 3, 2) split L.  Powers of two take 16; lengths with a
+// factor 3 or 5 take 12 or 20.  NC = 3 (the z pass of curl, div and
+// reprojection) holds three pencils per thread, so it prefers 8 and, in
+// FP64, 10 over 20.
+constexpr bool s_plan_ok(int L, int E) { return E <= L && L % E == 0 && make_cplan(L, E).n >= 0; }
+constexpr int s_pick_E(int L, int NC, int elem_bytes)
+{
+    const int one[6] = {16, 12, 20, 24, 10, 8};
+    const int three32[6] = {8, 12, 20, 10, 6, 16};
+    const int three64[6] = {8, 12, 10, 6, 20, 16};
+    const int* c = (NC != 3) ? one : (elem_bytes == 8 ? three64 : three32);
+    for (int i = 0; i < 6; ++i)
+        if (s_plan_ok(L, c[i])) return c[i];
+    return -1;
+}
+
 template <typename T, int L, int NC = 1> struct SPolicy {
-    static constexpr int E = (NC == 3) ? 8 : 16;
+    static constexpr int E = s_pick_E(L, NC, (int)sizeof(T));
+    static_assert(E > 0, "no specialised radix split for this length");
     static constexpr int TP = L / E;
     static constexpr int B = s_pick_B(TP, L, 2 * (int)sizeof(T));
     static constexpr int P = s_pick_P(TP, L, 2 * (int)sizeof(T));
