@@ -12,8 +12,20 @@ namespace psamd {
 // x pass, forward (two real rows per complex pencil).  HALF: rows hold L/2
 // values followed by zero padding (unbounded x).
 // ---------------------------------------------------------------------------
+// Waves per SIMD that the x kernels' LDS allows (at most 4), the occupancy
+// bound handed to the compiler: it then keeps registers it would otherwise
+// spill to reach waves that cannot be resident (FP64 pencils of 4096 and 8192
+// hold 67 and 135 KB of LDS per row pair).
+template <typename T, int L> constexpr int s_x_minw()
+{
+    using Pol = SPolicy<T, L>;
+    const int lds = Pol::P * s_lds_len(L) * 2 * (int)sizeof(T);
+    const int waves = ((160 * 1024) / lds) * ((Pol::TP * Pol::P + 63) / 64) / 4;
+    return waves < 1 ? 1 : (waves > 4 ? 4 : waves);
+}
+
 template <typename T, int L, int HALF>
-__global__ void __launch_bounds__(kMaxThreads, 4) k_x_r2c_s(XArgs<T> a)
+__global__ void __launch_bounds__(kMaxThreads, (s_x_minw<T, L>())) k_x_r2c_s(XArgs<T> a)
 {
     using C = cplx<T>;
     using Pol = SPolicy<T, L>;

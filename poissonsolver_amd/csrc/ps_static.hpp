@@ -7,7 +7,7 @@
 #include "ps_sfft.hpp"
 
 // powers of two, and the doubled lengths of 48, 96, 160 and 192 cells
-#define PS_FOR_EACH_SL(X) X(64) X(96) X(128) X(192) X(256) X(320) X(384) X(512) X(1024) X(2048)
+#define PS_FOR_EACH_SL(X) X(64) X(96) X(128) X(192) X(256) X(320) X(384) X(512) X(1024) X(2048) X(4096) X(8192)
 
 namespace psamd {
 
@@ -42,7 +42,10 @@ constexpr int s_pick_P(int TP, int L, int cb)
 // factor 3 or 5 take 12 or 20.  NC = 3 (the z pass of curl, div and
 // reprojection) holds three pencils per thread, so it prefers 8 and, in
 // FP64, 10 over 20.
-constexpr bool s_plan_ok(int L, int E) { return E <= L && L % E == 0 && make_cplan(L, E).n >= 0; }
+constexpr bool s_plan_ok(int L, int E)
+{
+    return E <= L && L % E == 0 && L / E <= kMaxThreads && make_cplan(L, E).n >= 0;
+}
 constexpr int s_pick_E(int L, int NC, int elem_bytes)
 {
     const int one[6] = {16, 12, 20, 24, 10, 8};
