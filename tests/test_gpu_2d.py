@@ -211,3 +211,16 @@ def test_2d_profile_and_autotune():
     lo, _ = O.solve2d(n, (0, 0), (1 / 256, 1 / 256), 2, [b])
     assert O.rel_l2(lg[0], lo[0]) < 2e-6
     s.close()
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("mixed", 1e-6)])
+def test_2d_4096_specialised_8192(prec, tol):
+    """4096^2 unbounded (transform length 8192, the specialised 8192 kernels):
+    grad of the reference's unbounded2d bump against its analytic gradient."""
+    n = 4096
+    x, h = cell_centres(n, -1.0, 1.0)
+    X, Y = np.meshgrid(x, x)
+    A, B, Ax, Ay = bump(X, Y)
+    announce("2d 4096", prec)
+    lg, _ = gpu_solve2d((n, n), (0, 0), (h, h), 0, [B], "grad", prec=prec)
+    assert O.rel_l2([lg[0], lg[1]], [Ax, Ay]) < tol
