@@ -94,7 +94,7 @@ def is_zpass_kernel(name, nc):
     """The fused z pass: k_zmfma1024 / k_zmfma1024p (matrix-core, one
     component), k_zfused_w (pencil per wavefront, one component) or
     k_pencil_s<T, L, NC, MODE, ...> with MODE 2 and the given NC."""
-    if nc == 1 and ("k_zmfma1024" in name or "k_zfused_w<" in name):
+    if nc == 1 and ("k_zmfma1024" in name or "k_zfused_w<" in name or "k_zfused_r2<" in name):
         return True
     if "k_pencil_s<" not in name:
         return False

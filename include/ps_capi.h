@@ -149,7 +149,8 @@ int ps_is_2d(const ps_ctx* ctx);
    [lane 64][K chunk 2][Yr hi, Yr lo, Yi hi, Yi lo][8]) */
 int ps_dft32_fragments(unsigned short* out, int n);
 /* kernel family of the one-component z pass chosen at setup: 0 runtime-plan,
-   1 compile-time specialised, 2 pencil-per-wavefront, 3 matrix-core (PS_MIXED);
+   1 compile-time specialised, 2 pencil-per-wavefront, 3 matrix-core (PS_MIXED),
+   4 two-residue (FP32 z pencils of 2048 as two 1024-point transforms);
    negative: not set up */
 int ps_zpass_variant(const ps_ctx* ctx);
 
@@ -172,7 +173,9 @@ typedef struct ps_tuning {
     int twreg_z;      /* z pass keeps stage twiddles in registers (PS_TUNE_TWREG_Z, 1) */
     int twreg_y;      /* y passes keep stage twiddles in registers (PS_TUNE_TWREG_Y, 0) */
     int occ4_z;       /* z pass targets 4 waves per SIMD instead of 2 (PS_TUNE_OCC4_Z, 0) */
-    int zwave;        /* pencil-per-wavefront z kernel where the length allows (PS_TUNE_ZWAVE, 1) */
+    int zwave;        /* pencil-per-wavefront z kernel where the length allows (PS_TUNE_ZWAVE, 1);
+                         FP32 z pencils of 2048 (nz = 1024 unbounded): 1 two 1024-point residues,
+                         2 one 2048-point pencil per wavefront, 0 neither */
     int halfb_z;      /* z tile-width divisor 0/1/2 -> B, B/2, B/4 (PS_TUNE_HALFB_Z, 0) */
     int halfb_y;      /* y tile-width divisor 0/1 -> B, B/2 (PS_TUNE_HALFB_Y, 0) */
     int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT-32) kernels where the length

@@ -107,6 +107,8 @@ struct ps_ctx {
     bool tune_set = false;   // tune_req replaces the defaults
     int zb = 0;             // A2 layout: log2 z block (-1 = [ky][z][kx])
     int zwave = 0;          // z pass uses the pencil-per-wavefront kernel
+    bool zres = false;      // FP32 z pencils of 2048 (nz = 1024 unbounded) as two residues of 1024
+    void* tw_half = nullptr;   // 1024-point twiddle table of the two-residue z kernel
     bool zmfma = false;     // PS_MIXED: one-component z pass on the matrix cores
     int g_layout = 0;       // Gk octant: 0 [ky][kz][kx], 1 [ky][kx][kz]
     Addr a2_ypass;          // A2 addressing seen by the y passes (slab = z, pos = ky)
@@ -158,6 +160,8 @@ void release(ps_ctx* c)
     free_ptr(c->A2);
     free_ptr(c->G);
     free_ptr(c->mfrag);
+    free_ptr(c->tw_half);
+    c->zres = false;
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) free_ptr(d->tw);
     c->ncomp_alloc = 0;
     c->ready = false;
@@ -433,6 +437,12 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         }
     }
     if (nc == 2 && zp.stat) return fail(c, PS_EINVAL, "one-input two-output z pass needs the runtime-plan kernel");
+    if constexpr (sizeof(T) == 4) {
+        if (!mfma_done && c->zres && nc == 1) {
+            CK((s_launch_zres<float, 1024>(za, c->tw_half, s)));
+            mfma_done = true;
+        }
+    }
     if (mfma_done) {
     } else if (zp.stat) CK(static_launch_pencil<T>(zp.p.L, za, nc, SM_FUSED, c->bc[2] == 0, s));
     else CK(launch_pencil<T>(zp.p.E, nc, za, s));
@@ -582,7 +592,27 @@ template <typename T> int setup_impl(ps_ctx* c)
     const long zcomp = c->dim2 ? c->A1c : c->A2c;
     c->zmfma = c->prec == PS_MIXED && tn.mfma && zmfma_supported(c->M[2], c->bc[2] == 0, OP_PLAIN) &&
                c->zb <= 5 && zcomp * 8 < (1L << 32) && (long)c->O[0] * c->O[1] * c->O[2] * 4 < (1L << 32);
-    c->g_layout = (c->zwave && !c->zmfma) ? 1 : 0;
+    // two-residue z kernel: FP32 pencils of 2048 holding nz = 1024 unbounded
+    // cells run as two 1024-point wavefront transforms (zwave = 1; zwave = 2
+    // keeps one pencil of 2048 per wavefront with 32 elements per thread)
+    c->zres = c->prec != PS_FP64 && tn.zwave == 1 && !c->zmfma && c->pz1.stat && c->pz1.p.L == 2048 &&
+              c->bc[2] == 0 && (c->zb <= 0 || (1024 / 16) % (1 << c->zb) == 0);
+    if (c->zres) {
+        int E1 = 0, B1 = 0, P1 = 0;
+        static_policy(1024, eb, 1, &E1, &B1, &P1);
+        c->zwave = 0;
+        c->pz1.tile = tn.halfb_z ? B1 / 2 : B1;
+        std::vector<double> re, im;
+        twiddles(1024, re, im);
+        std::vector<float> h(2048);
+        for (int j = 0; j < 1024; ++j) {
+            h[2 * j] = (float)re[j];
+            h[2 * j + 1] = (float)im[j];
+        }
+        CK(hipMalloc(&c->tw_half, h.size() * sizeof(float)));
+        CK(hipMemcpy(c->tw_half, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+    c->g_layout = ((c->zwave || c->zres) && !c->zmfma) ? 1 : 0;
     if (c->zmfma) {
         std::vector<unsigned short> fr(kZmfmaFragHalves);
         zmfma_fragments(fr.data());
@@ -958,6 +988,7 @@ int ps_dft32_fragments(unsigned short* out, int n)
 int ps_zpass_variant(const ps_ctx* c)
 {
     if (!c || !c->ready) return -1;
+    if (c->zres) return 4;
     if (c->zmfma) return 3;
     if (c->zwave) return 2;
     return c->pz1.stat ? 1 : 0;
@@ -1247,7 +1278,7 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
     const std::vector<Knob> knobs = {
         {&Tuning::zblk, {-2, -1, 0, 2, 3, 4, 5}, false},
         {&Tuning::specialised, {1, 0}, false},
-        {&Tuning::zwave, {1, 0}, false},
+        {&Tuning::zwave, {1, 2, 0}, false},
         {&Tuning::halfb_z, {0, 1, 2}, false},
         {&Tuning::halfb_y, {0, 1}, false},
         {&Tuning::twreg_z, {1, 0}, false},

@@ -162,17 +162,19 @@ __device__ __forceinline__ void kspace_op(cplx<T> (&w)[E], const cplx<T> (&v)[NC
     }
 }
 
-// One-component operators (plain, grad, filtered rhs) in place on v.
+// One-component operators (plain, grad, filtered rhs) in place on v.  The
+// spectrum index of v[u] is kzm (t + TP u) + kza (kzm 2: one residue of a
+// transform split into even and odd frequencies).
 template <typename T, int E>
 __device__ __forceinline__ void kspace_op_inplace(cplx<T> (&v)[E], const PencilArgs<T>& a, const TileK<T>& tk,
-                                                  int comp, int t, int TP)
+                                                  int comp, int t, int TP, int kzm = 1, int kza = 0)
 {
     using C = cplx<T>;
     const int op = a.op;
     if (op == OP_NONE) return;
 #pragma unroll
     for (int u = 0; u < E; ++u) {
-        const int kz = t + TP * u;
+        const int kz = kzm * (t + TP * u) + kza;
         const int kzo = (kz <= a.Mz - kz) ? kz : a.Mz - kz;
         const T g = tk.grow ? tk.grow[kzo * a.g_kz] : T(0);
         if (op == OP_PLAIN) {

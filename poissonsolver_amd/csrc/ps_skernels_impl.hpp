@@ -340,6 +340,149 @@ __global__ void __launch_bounds__(kMaxThreads, 2) k_zfused_w(PencilArgs<T> a)
     }
 }
 
+// Fused z pass of a zero-padded pencil of 2 LH (unbounded z, nz = LH) as two
+// LH-point transforms of the unpadded data: the even and odd frequencies are
+//   X[2k + r] = FFT_LH(x[n] W_{2LH}^{r n})[k],  r = 0, 1,
+// and the kept outputs (all n < LH) are
+//   y[n] = sum_r W_{2LH}^{-r n} IFFT_LH(op(X[2k + r]))[n].
+// Each residue runs the pencil-per-wavefront machinery at E = 16 (a pencil of
+// LH = 1024 is one wavefront); the input and the sum stay in the coalesced
+// mapping's registers.  a.plan.tw is the LH-point table; the residue twiddles
+// W_{2LH}^n come from sincospi with an exact argument n / LH.
+template <typename T> __device__ __forceinline__ void zres_sincospi(T x, T* s, T* c)
+{
+    if constexpr (sizeof(T) == 4) sincospif(x, s, c);
+    else sincospi(x, s, c);
+}
+
+// cos / sin (2 pi u / 32), u < 16: the residue twiddles' constant factors
+struct Trig32h {
+    static constexpr double c[16] = {1.0, 9.80785280403230449126e-1, 9.23879532511286756128e-1, 8.31469612302545237079e-1, 7.07106781186547524401e-1, 5.55570233019602224743e-1, 3.82683432365089771728e-1, 1.95090322016128267848e-1, 0.0, -1.95090322016128267848e-1, -3.82683432365089771728e-1, -5.55570233019602224743e-1, -7.07106781186547524401e-1, -8.31469612302545237079e-1, -9.23879532511286756128e-1, -9.80785280403230449126e-1};
+    static constexpr double s[16] = {0.0, 1.95090322016128267848e-1, 3.82683432365089771728e-1, 5.55570233019602224743e-1, 7.07106781186547524401e-1, 8.31469612302545237079e-1, 9.23879532511286756128e-1, 9.80785280403230449126e-1, 1.0, 9.80785280403230449126e-1, 9.23879532511286756128e-1, 8.31469612302545237079e-1, 7.07106781186547524401e-1, 5.55570233019602224743e-1, 3.82683432365089771728e-1, 1.95090322016128267848e-1};
+};
+
+// TWR = 1: stage twiddles in registers, 0: from the table.  The input held
+// for the second residue and the sum take 64 registers beyond the one-pencil
+// kernel, so the kernel runs at 2 waves per SIMD (at 4 it spills 450+ bytes
+// per lane even when the input is read again instead of held).
+template <typename T, int LH, int B, int TWR>
+__global__ void __launch_bounds__(kMaxThreads, 2) k_zfused_r2(PencilArgs<T> a)
+{
+    using C = cplx<T>;
+    constexpr int E = 16, TP = LH / E;
+    constexpr int NS = SFft<LH, E>::cp.n;
+    static_assert(NS >= 2, "two or more stages");
+    static_assert(64 % TP == 0, "each pencil inside one wavefront");
+    constexpr int LP = lds_len(LH);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+
+    const int tid = threadIdx.x;
+    const int tl = tid / B, pl = tid % B;
+    const int pc = tid / TP, tc = tid % TP;
+    const long nblk = gridDim.x;
+    const long lin = a.xcd_remap ? xcd_remap(blockIdx.x, nblk) : (long)blockIdx.x;
+    const long ntile = (long)a.ncoltile * a.nslab;
+    const int comp = (int)(lin / ntile);
+    const long rem = lin - (long)comp * ntile;
+    const int slab0 = (int)(rem / a.ncoltile);
+    const int ct = (int)(rem - (long)slab0 * a.ncoltile);
+    const int slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    const int col0 = ct * B;
+    const bool full = col0 + B <= a.ncol;
+    const bool cvl = col0 + pl < a.ncol, cvc = col0 + pc < a.ncol;
+
+    STw<LH, E, T> W;
+    if constexpr (TWR) load_stwiddles<LH, E, T>(W, tc, a.plan.tw);
+
+    const long iu = (a.ia.pos_sh >= 30) ? TP * a.ia.pos_lo : (long)(TP >> a.ia.pos_sh) * a.ia.pos_hi;
+    const long ou = (a.oa.pos_sh >= 30) ? TP * a.oa.pos_lo : (long)(TP >> a.oa.pos_sh) * a.oa.pos_hi;
+
+    C x[E], acc[E];
+    {
+        const C* ip = reinterpret_cast<const C*>(a.in) + comp * a.ia.comp + (col0 + pl) * a.ia.col +
+                      boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi) +
+                      boff(tl, a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi);
+        const bool ok = full || cvl;
+#pragma unroll
+        for (int u = 0; u < E; ++u) x[u] = (ok && tl + TP * u < a.n_in) ? ip[u * iu] : C{0, 0};
+    }
+    const TileK<T> tk = tile_k(a, col0 + pc, slab, cvc);
+    C* reg = lds + pc * LP;
+    // W_{2LH}^n at n = tl + TP u is W_{2LH}^tl W_{2LH}^{TP u}, and TP u / 2LH = u / 32
+    static_assert(2 * LH == 32 * TP, "residue twiddles are W_{2LH}^tl times 32nd roots");
+    C wb;
+    {
+        T sn, cs;
+        zres_sincospi<T>((T)tl / (T)LH, &sn, &cs);
+        wb = C{cs, -sn};
+    }
+    // unrolled: the input x is dead once the second residue has read it, and
+    // the sum is only the first residue's result until then (two arrays live)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        C v[E];
+#pragma unroll
+        for (int u = 0; u < E; ++u)
+            v[u] = r ? cmul(x[u], cmul(wb, C{(T)Trig32h::c[u], (T)-Trig32h::s[u]})) : x[u];
+        if (r) __syncthreads();   // the previous residue's last reads of the coalesced regions are done
+        sstage<LH, E, -1, 0, TWR, T>(v, W, a.plan.tw, tl);
+        sxchg<LH, E, 0, 0, T>(v, lds + pl * LP, tl, reg, tc);
+        wstages<LH, E, -1, 1, NS, TWR, T>(v, W, a.plan.tw, reg, tc);
+        kspace_op_inplace<T, E>(v, a, tk, comp, tc, TP, 2, r);
+        wstages<LH, E, +1, 0, NS - 1, TWR, T>(v, W, a.plan.tw, reg, tc);
+        sxchg<LH, E, NS - 2, 0, T>(v, reg, tc, lds + pl * LP, tl);
+        sstage<LH, E, +1, NS - 1, 0, T>(v, W, a.plan.tw, tl);   // coalesced slot: table twiddles
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            if (r) {
+                const C w = cmul(wb, C{(T)Trig32h::c[u], (T)-Trig32h::s[u]});
+                acc[u] = cadd(acc[u], cmul(v[u], C{w.x, -w.y}));   // W_{2LH}^{-n}
+            } else {
+                acc[u] = v[u];
+            }
+        }
+    }
+
+    if (full || cvl) {
+        C* op_ = reinterpret_cast<C*>(a.out) + comp * a.oa.comp + (col0 + pl) * a.oa.col +
+                 boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi) + boff(tl, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
+#pragma unroll
+        for (int u = 0; u < E; ++u)
+            if (tl + TP * u < a.n_out) op_[u * ou] = acc[u];
+    }
+}
+
+// Launch of k_zfused_r2 for pencils of 2 LH; tw_half is the LH-point table.
+template <typename T, int LH> hipError_t s_launch_zres(const PencilArgs<T>& a, const void* tw_half, hipStream_t s)
+{
+    using Pol = SPolicy<T, LH>;
+    PencilArgs<T> b = a;
+    b.plan.tw = reinterpret_cast<const cplx<T>*>(tw_half);
+    constexpr int TP = LH / 16;
+    const size_t shm = (size_t)a.B * lds_len(LH) * sizeof(cplx<T>);
+    const long nwork = (long)a.ncoltile * a.nslab * a.ncomp;
+    hipError_t e;
+#define PS_ZRES(BB, TW)                                                                                   \
+    do {                                                                                                  \
+        if ((e = allow_lds(k_zfused_r2<T, LH, BB, TW>, shm)) != hipSuccess) return e;                     \
+        hipLaunchKernelGGL((k_zfused_r2<T, LH, BB, TW>), dim3((unsigned)nwork), dim3(TP * BB), shm, s, b); \
+        return hipGetLastError();                                                                         \
+    } while (0)
+    if (a.B == Pol::B) {
+        if (a.twreg) PS_ZRES(Pol::B, 1);
+        PS_ZRES(Pol::B, 0);
+    }
+    if constexpr (Pol::B >= 2) {
+        if (a.B == Pol::B / 2) {
+            if (a.twreg) PS_ZRES(Pol::B / 2, 1);
+            PS_ZRES(Pol::B / 2, 0);
+        }
+    }
+#undef PS_ZRES
+    return hipErrorInvalidValue;
+}
+
 template <typename T, int L, int B>
 hipError_t s_launch_zwave_b(const PencilArgs<T>& a, int half, hipStream_t s)
 {

@@ -70,6 +70,9 @@ template <typename T, int L, int NC = 1> struct SPolicy {
 bool static_policy(int L, int elem_bytes, int ncomp, int* E, int* B, int* P);
 
 template <typename T, int L> hipError_t s_launch_x(const XArgs<T>& a, int inverse, int half, hipStream_t s);
+// fused z pass of zero-padded pencils of 2 LH as two LH-point residues
+// (instantiated for float, LH = 1024); tw_half is the LH-point twiddle table
+template <typename T, int LH> hipError_t s_launch_zres(const PencilArgs<T>& a, const void* tw_half, hipStream_t s);
 template <typename T, int L>
 hipError_t s_launch_pencil(const PencilArgs<T>& a, int nc, int mode, int half, hipStream_t s);
 

@@ -200,7 +200,8 @@ class poisson_solver:
     def synchronize(self):
         self._report(self._lib.ps_synchronize(self._ctx), "synchronize")
 
-    ZPASS_VARIANTS = {0: "runtime-plan", 1: "specialised", 2: "pencil-per-wavefront", 3: "matrix-core"}
+    ZPASS_VARIANTS = {0: "runtime-plan", 1: "specialised", 2: "pencil-per-wavefront", 3: "matrix-core",
+                      4: "two-residue"}
 
     def zpass_variant(self) -> str:
         """Kernel family of the one-component z pass chosen by setup3d."""

@@ -15,10 +15,13 @@ def announce(*a):
     sys.stdout.flush()
 
 
-def gpu_solve(n, bc, dx, reg, rhs, op="plain", reproject=False, prec="fp64", want_rhs=False):
-    s = P.poisson_solver(PREC[prec])
-    s.setup3d(n, bc, dx, reg)
-    assert s.last_status == 0, s.last_status
+def gpu_solve(n, bc, dx, reg, rhs, op="plain", reproject=False, prec="fp64", want_rhs=False, solver=None):
+    """setup3d -> push -> solve3d -> pull on the GPU (on `solver`, already set
+    up, when given; it is left open)."""
+    s = solver or P.poisson_solver(PREC[prec])
+    if solver is None:
+        s.setup3d(n, bc, dx, reg)
+        assert s.last_status == 0, s.last_status
     s.set_return_grad(op == "grad")
     s.set_return_div(op == "div")
     s.set_return_curl(op == "curl")
@@ -33,7 +36,8 @@ def gpu_solve(n, bc, dx, reg, rhs, op="plain", reproject=False, prec="fp64", wan
     rb = [np.full(shape, np.nan) for _ in range(3)]
     s.pull(*lhs, *(rb if want_rhs else [None] * 3))
     assert s.last_status == 0
-    s.close()
+    if solver is None:
+        s.close()
     return lhs, rb
 
 

@@ -103,20 +103,24 @@ def test_fp64_parity_odd_radices(case):
 
 @pytest.mark.parametrize("prec,tol", [("fp32", 2e-6), ("mixed", 2e-6)])
 @pytest.mark.parametrize("op", ["plain", "grad"])
-def test_long_z_pencil_per_wavefront(prec, tol, op):
-    # nz = 1024 unbounded: z transforms of 2048, the pencil-per-wavefront z
-    # kernel with 32 elements per thread (FP32 storage)
+@pytest.mark.parametrize("form", [(1, "two-residue"), (2, "pencil-per-wavefront")], ids=lambda f: f[1])
+def test_long_z_pencil_per_wavefront(prec, tol, op, form):
+    # nz = 1024 unbounded: z transforms of 2048, FP32 storage, in both forms:
+    # two 1024-point residues per pencil (zwave 1, the default) or one
+    # 2048-point pencil per wavefront with 32 elements per thread (zwave 2)
     import poissonsolver_amd as P
+    zwave, name = form
     n, bc, dx = (8, 6, 1024), (0, 0, 0), (0.1, 0.12, 1 / 512)
-    announce("long z", n, prec, op)
-    s = P.poisson_solver(P.PS_FP32 if prec == "fp32" else P.PS_MIXED)
-    s.setup3d(n, bc, dx, 2)
-    assert s.zpass_variant() == "pencil-per-wavefront"
-    s.close()
+    announce("long z", n, prec, op, name)
     b = random_rhs(n, 47)
     rhs = [b[0], b[1], b[2]] if op == "grad" else [b[0]]
+    s = P.poisson_solver(P.PS_FP32 if prec == "fp32" else P.PS_MIXED)
+    s.set_tuning(zwave=zwave)
+    s.setup3d(n, bc, dx, 2)
+    assert s.zpass_variant() == name
+    lg, _ = gpu_solve(n, bc, dx, 2, rhs, op=op, prec=prec, solver=s)
+    s.close()
     lo, _ = O.solve(n, bc, dx, 2, rhs, op=op)
-    lg, _ = gpu_solve(n, bc, dx, 2, rhs, op=op, prec=prec)
     got = [lg[d] for d in range(3) if lo[d] is not None]
     ref = [lo[d] for d in range(3) if lo[d] is not None]
     assert O.rel_l2(got, ref) <= tol
