@@ -224,3 +224,18 @@ def test_2d_4096_specialised_8192(prec, tol):
     announce("2d 4096", prec)
     lg, _ = gpu_solve2d((n, n), (0, 0), (h, h), 0, [B], "grad", prec=prec)
     assert O.rel_l2([lg[0], lg[1]], [Ax, Ay]) < tol
+
+
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("op", ["plain", "grad"])
+def test_2d_two_residue_y_pass(prec, op):
+    """ny = 1024 unbounded: the y transform has length 2048 and the single-
+    precision forms run it on the two-residue kernel (two 1024-point
+    transforms of the unpadded data)."""
+    n = (48, 1024)
+    dx = (1 / 48, 1 / 512)
+    s = P.poisson_solver(P.PS_FP32 if prec == "fp32" else P.PS_MIXED)
+    s.setup2d(n, (0, 0), dx, 2)
+    assert s.zpass_variant() == "two-residue"
+    s.close()
+    _compare(n, (0, 0), dx, 2, op, prec, seed=13)
