@@ -124,3 +124,19 @@ def test_long_z_pencil_per_wavefront(prec, tol, op, form):
     got = [lg[d] for d in range(3) if lo[d] is not None]
     ref = [lo[d] for d in range(3) if lo[d] is not None]
     assert O.rel_l2(got, ref) <= tol
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("mixed", 2e-6)])
+@pytest.mark.parametrize("op", ["plain", "grad", "div", "curl"])
+def test_long_z_all_operators(prec, tol, op):
+    # z transforms of 2048 (nz = 1024 unbounded) with every operator: the
+    # three-component z pass (div, curl) at this length, and FP64
+    n, bc, dx = (6, 8, 1024), (0, 0, 0), (0.11, 0.1, 1 / 512)
+    announce("long z all operators", n, prec, op)
+    b = random_rhs(n, 53)
+    rhs = [b[0]] if op == "plain" else [b[0], b[1], b[2]]
+    lg, _ = gpu_solve(n, bc, dx, 4, rhs, op=op, prec=prec)
+    lo, _ = O.solve(n, bc, dx, 4, rhs, op=op)
+    got = [lg[d] for d in range(3) if lo[d] is not None]
+    ref = [lo[d] for d in range(3) if lo[d] is not None]
+    assert O.rel_l2(got, ref) <= tol
