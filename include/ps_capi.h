@@ -141,8 +141,10 @@ int ps_eval_special(const double* x, double* y, long n, int which);
 int ps_solve3d_async(ps_ctx* ctx);
 /* enqueue ps_solve2d on the context's stream and return without waiting */
 int ps_solve2d_async(ps_ctx* ctx);
-/* 1 if the context was set up by ps_setup2d (its passes: x, fused y, x^-1;
-   ps_profile_solve reports the fused y pass as the z entry), else 0 */
+/* nonzero if the context was set up by ps_setup2d (its passes: x, fused y, x^-1;
+   ps_profile_solve reports the fused y pass as the z entry): 2 when its fused
+   pass reads a transposed copy of the x pass output (tuning field t2d), else 1;
+   0 for a 3D or unset context */
 int ps_is_2d(const ps_ctx* ctx);
 /* the FP16 hi/lo fragments of the 32-point DFT matrix that the matrix-core
    kernels multiply by (binary16 bit patterns; n >= 4096 entries, layout
@@ -181,6 +183,9 @@ typedef struct ps_tuning {
     int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT-32) kernels where the length
                          has them: 1 one tile per workgroup, 2 persistent workgroups that
                          prefetch; 0 the FP32 kernels (PS_TUNE_MFMA, 1) */
+    int t2d;          /* 2D contexts whose fused y pass holds one or two columns per workgroup
+                         (long pencils): the x pass output is transposed so that y pencils are
+                         contiguous, and grad / curl transform their scalar once (PS_TUNE_T2D, 1) */
 } ps_tuning;
 /* the defaults (with PS_TUNE_* overrides applied) */
 int ps_tuning_default(ps_tuning* t);

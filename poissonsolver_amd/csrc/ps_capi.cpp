@@ -72,6 +72,7 @@ Tuning tuning_default()
     t.halfb_z = env_int("PS_TUNE_HALFB_Z", 0);
     t.halfb_y = env_int("PS_TUNE_HALFB_Y", 0);
     t.mfma = env_int("PS_TUNE_MFMA", 1);
+    t.t2d = env_int("PS_TUNE_T2D", 1);
     return t;
 }
 
@@ -111,6 +112,8 @@ struct ps_ctx {
     void* tw_half = nullptr;   // 1024-point twiddle table of the two-residue z kernel
     bool zmfma = false;     // PS_MIXED: one-component z pass on the matrix cores
     int g_layout = 0;       // Gk octant: 0 [ky][kz][kx], 1 [ky][kx][kz]
+    bool t2d = false;       // 2D: fused pass reads the x pass output transposed, A2 = [c][kx][nyp]
+    long nyp = 0;           // 2D transposed rows: the y extent rounded up to 16
     Addr a2_ypass;          // A2 addressing seen by the y passes (slab = z, pos = ky)
     Addr a2_zpass;          // A2 addressing seen by the z pass (slab = ky, pos = z)
 
@@ -236,7 +239,10 @@ bool arith(const bool present[3], int* first, int* step, int* n)
 // 2D operators (solve2d.cpp:292-326).  grad and curl differentiate one
 // scalar in two directions.  In single precision with the runtime-plan
 // kernel (long pencils) the scalar is transformed once and one launch writes
-// both components (nc 2).  Otherwise the x pass transforms the scalar twice
+// both components (nc 2).  On the transposed path (long pencils on the
+// specialised kernel) the scalar is also transformed once: the fused pass is
+// out of place and each output component reads the same transposed copy.
+// Otherwise the x pass transforms the scalar twice
 // (input step 0) and each output is an in-place one-component z pass: the
 // specialised, pencil-per-wavefront and matrix-core kernels are
 // component-wise, and the FP64 two-output kernel would exceed 256 registers
@@ -244,6 +250,8 @@ bool arith(const bool present[3], int* first, int* step, int* n)
 int select_op2d(ps_ctx* c, OpSel* s)
 {
     const bool one_read = c->prec != PS_FP64 && !c->pz1.stat && !c->zmfma && !c->zwave;
+    // transposed path: one x pass of the scalar, both outputs read it (nc 1, nin 1)
+    const int nin1 = (one_read || c->t2d) ? 1 : 2;
     const bool rX = c->rhs_present[0], rY = c->rhs_present[1], rZ = c->rhs_present[2];
     if (!arith(c->rhs_present, &s->in_first, &s->in_step, &s->nin))
         return fail(c, PS_ESTATE, "[poisson_solver.solve2d]: no rhs field has been pushed");
@@ -254,7 +262,7 @@ int select_op2d(ps_ctx* c, OpSel* s)
         s->op = OP_GRAD;
         s->nc = one_read ? 2 : 1;
         s->kmap = KMAP_2D_GRAD;
-        s->in_first = 0; s->in_step = 0; s->nin = one_read ? 1 : 2;
+        s->in_first = 0; s->in_step = 0; s->nin = nin1;
         s->out_first = 0; s->out_step = 1; s->nout = 2;
     } else if (c->div) {
         if (!(rX && rY))
@@ -272,7 +280,7 @@ int select_op2d(ps_ctx* c, OpSel* s)
         s->op = OP_GRAD;
         s->nc = one_read ? 2 : 1;
         s->kmap = KMAP_2D_CURL;
-        s->in_first = 2; s->in_step = 0; s->nin = one_read ? 1 : 2;
+        s->in_first = 2; s->in_step = 0; s->nin = nin1;
         s->out_first = 0; s->out_step = 1; s->nout = 2;
     } else {
         s->op = OP_PLAIN;
@@ -370,7 +378,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
 {
     hipStream_t s = c->stream;
     const int nx = c->ncell[0], ny = c->ncell[1], nz = c->ncell[2];
-    int r = ensure_comp_buffers(c, std::max(nin, nc));
+    int r = ensure_comp_buffers(c, std::max(std::max(nin, nc), nout));
     if (r) return r;
     if (ev) CK(hipEventRecord(ev[0], s));
 
@@ -426,6 +434,17 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     if (zarg >= 0.0) za.zarg = zarg;
     if (reg >= 0) za.reg = reg;
     za.ncomp = (nc == 3) ? 1 : nin;     // nc 2: one input, outputs 0 and 1
+    if (c->t2d) {
+        // 2D transposed path: A1 [c][y][Hp] -> A2 [c][kx][nyp], then the fused
+        // pass reads contiguous y pencils of A2 and writes A1; one input read by
+        // several one-component outputs (grad, curl) has component stride 0
+        CK(launch_transpose_c(c->A1, c->A2, 2 * (int)sizeof(T), nz, c->H, c->Hp, c->nyp, c->A1c, c->A2c, nin, s));
+        const bool shared_in = nc == 1 && nin == 1 && nout > 1;
+        za.in = c->A2;
+        za.ia = addr_plain(shared_in ? 0 : c->A2c, 0, 1, c->nyp);
+        za.out = c->A1;
+        if (nc == 1) za.ncomp = nout;
+    }
     bool mfma_done = false;
     if constexpr (sizeof(T) == 4) {
         if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, op)) {
@@ -612,7 +631,17 @@ template <typename T> int setup_impl(ps_ctx* c)
         CK(hipMalloc(&c->tw_half, h.size() * sizeof(float)));
         CK(hipMemcpy(c->tw_half, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     }
-    c->g_layout = ((c->zwave || c->zres) && !c->zmfma) ? 1 : 0;
+    // 2D, long y pencils (one or two columns per workgroup on the specialised
+    // fused kernel): a column read in the x pass layout touches one element
+    // per row; transposed, each pencil is contiguous.  The fused pass then
+    // reads the transposed copy A2 [c][kx][nyp] and writes the x pass layout
+    // of A1, so grad and curl need their scalar transformed only once.
+    c->t2d = c->dim2 && tn.t2d && c->pz1.stat && !c->zwave && !c->zres && !c->zmfma && c->pz1.tile <= 2;
+    if (c->t2d) {
+        c->nyp = (nz + 15) / 16 * 16;
+        c->A2c = (long)c->H * c->nyp;
+    }
+    c->g_layout = ((c->zwave || c->zres || c->t2d) && !c->zmfma) ? 1 : 0;
     if (c->zmfma) {
         std::vector<unsigned short> fr(kZmfmaFragHalves);
         zmfma_fragments(fr.data());
@@ -1024,7 +1053,7 @@ int ps_solve2d_async(ps_ctx* c)
     return (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
 }
 
-int ps_is_2d(const ps_ctx* c) { return (c && c->ready && c->dim2) ? 1 : 0; }
+int ps_is_2d(const ps_ctx* c) { return (c && c->ready && c->dim2) ? (c->t2d ? 2 : 1) : 0; }
 
 int ps_pull(ps_ctx* c, double* ax, double* ay, double* az, double* bx, double* by, double* bz)
 {
@@ -1289,6 +1318,7 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
         {&Tuning::tile_x, {0, 2, 4, 8, 16}, true},
         {&Tuning::xcd, {1, 0}, false},
         {&Tuning::mfma, {1, 2, 0}, false},
+        {&Tuning::t2d, {1, 0}, false},
     };
     bool best_all_stat = all_stat;
     for (const Knob& k : knobs) {

@@ -73,4 +73,43 @@ hipError_t launch_scatter(const void* src, int elem_bytes, double* dst, Strided 
     return hipGetLastError();
 }
 
+
+namespace {
+template <typename V>
+__global__ void __launch_bounds__(256) k_transpose_c(const V* __restrict__ in, V* __restrict__ out, int rows,
+                                                    int cols, long in_ld, long out_ld, long in_comp,
+                                                    long out_comp)
+{
+    __shared__ V tile[32][33];
+    const V* ip = in + blockIdx.z * in_comp;
+    V* op = out + blockIdx.z * out_comp;
+    const int j0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + r, j = j0 + tx;
+        if (i < rows && j < cols) tile[r][tx] = ip[(long)i * in_ld + j];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int j = j0 + r, i = i0 + tx;
+        if (j < cols && i < rows) op[(long)j * out_ld + i] = tile[tx][r];
+    }
+}
+}  // namespace
+
+hipError_t launch_transpose_c(const void* in, void* out, int elem_bytes, int rows, int cols, long in_ld,
+                              long out_ld, long in_comp, long out_comp, int ncomp, hipStream_t s)
+{
+    if (rows <= 0 || cols <= 0 || ncomp <= 0) return hipSuccess;
+    const dim3 grid((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32), (unsigned)ncomp);
+    const dim3 block(32, 8);
+    if (elem_bytes == 16)
+        hipLaunchKernelGGL(k_transpose_c<double2>, grid, block, 0, s, reinterpret_cast<const double2*>(in),
+                           reinterpret_cast<double2*>(out), rows, cols, in_ld, out_ld, in_comp, out_comp);
+    else
+        hipLaunchKernelGGL(k_transpose_c<float2>, grid, block, 0, s, reinterpret_cast<const float2*>(in),
+                           reinterpret_cast<float2*>(out), rows, cols, in_ld, out_ld, in_comp, out_comp);
+    return hipGetLastError();
+}
+
 }  // namespace psamd

@@ -20,4 +20,10 @@ hipError_t launch_gather(const double* src, Strided st, void* dst, int elem_byte
 hipError_t launch_scatter(const void* src, int elem_bytes, double* dst, Strided st, int cmask, int ncomp, int nx,
                           int ny, int nz, hipStream_t s);
 
+// out[c][j][i] = in[c][i][j] for i < rows, j < cols: complex elements of
+// elem_bytes (8 or 16), row strides in_ld / out_ld, component strides
+// in_comp / out_comp (all in elements)
+hipError_t launch_transpose_c(const void* in, void* out, int elem_bytes, int rows, int cols, long in_ld,
+                              long out_ld, long in_comp, long out_comp, int ncomp, hipStream_t s);
+
 }  // namespace psamd

@@ -205,7 +205,7 @@ def test_2d_profile_and_autotune():
     assert t["z"] > 0 and t["x"] > 0 and t["y"] < 0.05 and t["y_inv"] < 0.05
     res = s.autotune(2)
     assert res is not None
-    assert s._lib.ps_is_2d(s._ctx) == 1
+    assert s._lib.ps_is_2d(s._ctx) in (1, 2)
     b = np.random.default_rng(4).standard_normal((256, 256))
     lg, _ = gpu_solve2d(n, (0, 0), (1 / 256, 1 / 256), 2, [b], "plain", solver=s)
     lo, _ = O.solve2d(n, (0, 0), (1 / 256, 1 / 256), 2, [b])
@@ -239,3 +239,60 @@ def test_2d_two_residue_y_pass(prec, op):
     assert s.zpass_variant() == "two-residue"
     s.close()
     _compare(n, (0, 0), dx, 2, op, prec, seed=13)
+
+
+LONG_Y = [
+    # (ncell, bc, dx): y transforms of 8192 / 4096 (one or two columns per workgroup)
+    ((40, 4096), (0, 0), (1 / 40, 1 / 2048)),
+    ((24, 2048), (1, 0), (1 / 24, 1 / 1024)),
+    ((30, 4096), (0, 1), (0.05, 1 / 4096)),
+    ((16, 4096), (1, 1), (1 / 16, 1 / 4096)),
+]
+
+
+@pytest.mark.parametrize("case", LONG_Y, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("op", ["plain", "grad", "div", "curl"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("t2d", [1, 0])
+def test_2d_long_y_transposed_and_in_place(case, op, prec, t2d):
+    """Long y pencils with the fused pass reading a transposed copy of the x
+    pass output (t2d 1: contiguous pencils, grad / curl transform their scalar
+    once) and in place on the x pass layout (t2d 0), against the oracle."""
+    n, bc, dx = case
+    s = P.poisson_solver(P.PS_FP64 if prec == "fp64" else P.PS_FP32)
+    s.set_tuning(t2d=t2d)
+    s.setup2d(n, bc, dx, 2)
+    assert s.last_status == 0
+    assert s._lib.ps_is_2d(s._ctx) == (2 if t2d else 1)
+    announce("2d long y", n, bc, op, prec, "t2d", t2d)
+    b = _rhs(n, 29, op)
+    lg, rg = gpu_solve2d(n, bc, dx, 2, b, op=op, prec=prec, want_rhs=True, solver=s)
+    s.close()
+    lo, ro = O.solve2d(n, bc, dx, 2, b, op=op)
+    got = [lg[d] for d in range(3) if lo[d] is not None]
+    ref = [lo[d] for d in range(3) if lo[d] is not None]
+    assert O.rel_l2(got, ref) <= TOL[prec]
+    for d in range(3):
+        if lo[d] is None:
+            assert np.isnan(lg[d]).all()
+    gr = [rg[d] for d in range(3) if ro[d] is not None]
+    rr = [ro[d] for d in range(3) if ro[d] is not None]
+    assert O.rel_l2(gr, rr) <= TOL[prec]
+
+
+def test_2d_transposed_path_repeated_solves_and_ops():
+    """One context, one transposed set-up: plain, grad, curl, div and plain
+    again in sequence (buffers grow from one to three components between
+    solves), each against the oracle."""
+    n, bc, dx = (20, 4096), (0, 0), (0.05, 1 / 2048)
+    s = P.poisson_solver(P.PS_FP64)
+    s.setup2d(n, bc, dx, 0)
+    assert s._lib.ps_is_2d(s._ctx) == 2
+    for i, op in enumerate(["plain", "grad", "curl", "div", "plain"]):
+        b = _rhs(n, 40 + i, op)
+        lg, _ = gpu_solve2d(n, bc, dx, 0, b, op=op, prec="fp64", solver=s)
+        lo, _ = O.solve2d(n, bc, dx, 0, b, op=op)
+        got = [lg[d] for d in range(3) if lo[d] is not None]
+        ref = [lo[d] for d in range(3) if lo[d] is not None]
+        assert O.rel_l2(got, ref) <= 1e-12, op
+    s.close()
