@@ -462,10 +462,14 @@ def bench_2d(a):
         "pass_gbs": {k: Bb[k] / (prof[k] * 1e-3) / 1e9 for k in ("x", "z", "x_inv")},
         "rel_l2_vs_analytic_grad": err, "wall_s_timed": wall,
     }
+    if prof["y"] > 1e-3:
+        # long pencils: the x pass output transposed so that y pencils are
+        # contiguous (a layout copy, outside the algorithmic byte model)
+        out["pass_ms"]["transpose"] = prof["y"]
     if len(precs) == 2:
         p32, _, e32, zv32 = res["fp32"]
         out["fp32_path"] = {"value": 1000.0 / p32["solve"], "ms_per_step": p32["solve"], "z_kernel": zv32,
-                            "pass_ms": {k: p32[k] for k in ("x", "z", "x_inv")},
+                            "pass_ms": {k: p32[k] for k in ("x", "y", "z", "x_inv")},
                             "rel_l2_vs_analytic_grad": e32}
     if not (a.no_traffic or a.pmc_child):
         child = ["--pmc-child", "--workload", "uu2d", "--no-traffic", "--no-cpu", "--steps", "2", "--warmup", "1",

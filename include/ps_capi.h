@@ -143,8 +143,8 @@ int ps_solve3d_async(ps_ctx* ctx);
 int ps_solve2d_async(ps_ctx* ctx);
 /* nonzero if the context was set up by ps_setup2d (its passes: x, fused y, x^-1;
    ps_profile_solve reports the fused y pass as the z entry): 2 when its fused
-   pass reads a transposed copy of the x pass output (tuning field t2d), else 1;
-   0 for a 3D or unset context */
+   pass reads a transposed copy of the x pass output (tuning field t2d; the
+   transpose is reported as the y entry), else 1; 0 for a 3D or unset context */
 int ps_is_2d(const ps_ctx* ctx);
 /* the FP16 hi/lo fragments of the 32-point DFT matrix that the matrix-core
    kernels multiply by (binary16 bit patterns; n >= 4096 entries, layout

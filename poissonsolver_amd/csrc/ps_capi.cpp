@@ -439,6 +439,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         // pass reads contiguous y pencils of A2 and writes A1; one input read by
         // several one-component outputs (grad, curl) has component stride 0
         CK(launch_transpose_c(c->A1, c->A2, 2 * (int)sizeof(T), nz, c->H, c->Hp, c->nyp, c->A1c, c->A2c, nin, s));
+        if (ev) CK(hipEventRecord(ev[2], s));   // profiled as the (otherwise empty) y interval
         const bool shared_in = nc == 1 && nin == 1 && nout > 1;
         za.in = c->A2;
         za.ia = addr_plain(shared_in ? 0 : c->A2c, 0, 1, c->nyp);
