@@ -1324,6 +1324,8 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
     bool best_all_stat = all_stat;
     for (const Knob& k : knobs) {
         if (k.generic_only && best_all_stat && best.specialised) continue;
+        if (k.field == &Tuning::t2d && !dim2) continue;              // read by 2D contexts only
+        if (k.field == &Tuning::mfma && c->prec != PS_MIXED) continue;  // read by PS_MIXED only
         for (int v : k.values) {
             if (best.*k.field == v) continue;
             Tuning t = best;
