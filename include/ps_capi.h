@@ -203,7 +203,11 @@ int ps_autotune(ps_ctx* ctx, int iters, ps_tuning* best, double* best_ms);
 /* --- extension: batches of independent solves of uneven size ---------------
    One context per problem (own plans, tiles and Green's kernel); ps_batch_solve
    replays every problem's passes as one captured hipGraph launch, the problems
-   running on parallel branches of the graph.  Options apply to all problems. */
+   running on parallel branches of the graph.  Options apply to all problems.
+   Problems of one grid (ncell, bc, dx, reg_order) keep their sources and
+   solutions in one device block; when the operator is plain and their pushed
+   components agree they are solved as one multi-component solve (five launches
+   for the group), with results identical to solving each alone. */
 typedef struct ps_batch ps_batch;
 /* ncell, bc, dx: nprob x 3; reg_order: nprob entries */
 int ps_batch_create(ps_batch** out, int nprob, const int* ncell, const int* bc, const double* dx,
@@ -222,6 +226,8 @@ int ps_batch_solve(ps_batch* b);
 int ps_batch_pull(ps_batch* b, int prob, double* ax, double* ay, double* az, double* bx, double* by, double* bz);
 /* run `iters` batch solves, return the average wall time of one graph launch in ms */
 int ps_batch_profile(ps_batch* b, int iters, double* ms);
+/* problems solved inside group solves by the last solve (0 if none), -1 for NULL */
+int ps_batch_grouped(const ps_batch* b);
 const char* ps_batch_last_error(const ps_batch* b);
 
 #ifdef __cplusplus

@@ -87,6 +87,7 @@ SIGNATURES = {
     "ps_batch_solve": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_batch_pull": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int] + [ctypes.c_void_p] * 6),
     "ps_batch_profile": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, _c_dbl_p]),
+    "ps_batch_grouped": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_batch_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 

@@ -4,7 +4,12 @@
 // Green's kernel.  A batch solve is captured once into a hipGraph: an origin
 // stream forks to every problem's stream, each stream records its five pass
 // launches, and all branches join back; replaying the executable graph is one
-// hipGraphLaunch for the whole batch.  The graph is re-captured when the set
+// hipGraphLaunch for the whole batch.  Problems of one grid (cells, bc, dx,
+// reg) form a group whose sources and solutions are consecutive slices of one
+// device block: when their options and pushed components agree and the
+// operator is plain, the group is one solve whose components are the members'
+// fields (five launches for the group instead of five per member), which fills
+// the GPU where a small problem alone would not.  The graph is re-captured when the set
 // of pushed components or the operator options change (the captured launch
 // arguments depend on both); pushing new data of the same components reuses
 // the graph, which reads the contexts' fixed device buffers.
@@ -20,6 +25,8 @@
 namespace psamd {
 int ctx_mark_solved(ps_ctx* c);      // ps_capi.cpp
 int ctx_rhs_mask(const ps_ctx* c);
+int ctx_adopt_real(ps_ctx* c, void* rhs, void* lhs);
+int ctx_solve_group(ps_ctx* const* m, int K);
 }  // namespace psamd
 
 struct ps_batch {
@@ -28,6 +35,10 @@ struct ps_batch {
     std::vector<double> dx;             // per problem: 3 entries
     hipStream_t origin = nullptr;
     std::vector<hipEvent_t> fork, join;
+    std::vector<std::vector<int>> groups;   // problems of one grid, two or more
+    std::vector<void*> blocks;              // per group: rhs block, lhs block
+    int elem_bytes = 8;
+    int ngrouped = 0;                       // problems solved inside group solves, last enqueue
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;          // graph must be (re)captured
@@ -80,22 +91,105 @@ void note_push(ps_batch* b, int prob, int mask_before)
     }
 }
 
+// Problems of one grid share a device block for their sources and solutions:
+// member j's rhs and lhs are slices j of 3 n3 reals.  Called after every setup
+// of the members (a setup returns a context to buffers of its own).
+int form_groups(ps_batch* b)
+{
+    for (void* p : b->blocks)
+        if (p) (void)hipFree(p);
+    b->blocks.clear();
+    b->groups.clear();
+    const int n = (int)b->ctx.size();
+    std::vector<bool> taken(n, false);
+    for (int i = 0; i < n; ++i) {
+        if (taken[i]) continue;
+        std::vector<int> g{i};
+        for (int j = i + 1; j < n; ++j) {
+            bool same = b->reg[j] == b->reg[i];
+            for (int d = 0; d < 3; ++d)
+                same = same && b->ncell[3 * j + d] == b->ncell[3 * i + d] && b->bc[3 * j + d] == b->bc[3 * i + d] &&
+                       b->dx[3 * j + d] == b->dx[3 * i + d];
+            if (same) g.push_back(j);
+        }
+        if (g.size() < 2) continue;
+        for (int j : g) taken[j] = true;
+        const size_t slice = 3 * (size_t)b->ncell[3 * i] * b->ncell[3 * i + 1] * b->ncell[3 * i + 2] * b->elem_bytes;
+        void* rb = nullptr;
+        void* lb = nullptr;
+        BK(hipMalloc(&rb, slice * g.size()));
+        b->blocks.push_back(rb);
+        BK(hipMalloc(&lb, slice * g.size()));
+        b->blocks.push_back(lb);
+        for (size_t k = 0; k < g.size(); ++k) {
+            int st = sub(b, b->ctx[g[k]], psamd::ctx_adopt_real(b->ctx[g[k]], static_cast<char*>(rb) + k * slice,
+                                                               static_cast<char*>(lb) + k * slice));
+            if (st) return st;
+        }
+        b->groups.push_back(g);
+    }
+    return PS_OK;
+}
+
+bool grouped(const ps_batch* b, int prob)
+{
+    for (const auto& g : b->groups)
+        for (int j : g)
+            if (j == prob) return true;
+    return false;
+}
+
+// Enqueue the batch on the problems' streams (forked from and joined to the
+// origin stream): each group as one solve where it qualifies, every other
+// problem (and the members of a group that does not) on its own.
+int enqueue(ps_batch* b)
+{
+    int st = PS_OK;
+    auto fork_to = [&](int i) -> int {
+        hipStream_t s = (hipStream_t)ps_stream(b->ctx[i]);
+        if (hipEventRecord(b->fork[i], b->origin) != hipSuccess || hipStreamWaitEvent(s, b->fork[i], 0) != hipSuccess)
+            return bfail(b, PS_EHIP, "batch: fork failed");
+        return PS_OK;
+    };
+    auto join_from = [&](int i) -> int {
+        hipStream_t s = (hipStream_t)ps_stream(b->ctx[i]);
+        if (hipEventRecord(b->join[i], s) != hipSuccess || hipStreamWaitEvent(b->origin, b->join[i], 0) != hipSuccess)
+            return bfail(b, PS_EHIP, "batch: join failed");
+        return PS_OK;
+    };
+    auto one = [&](int i) -> int {
+        int r = fork_to(i);
+        if (r == PS_OK) r = sub(b, b->ctx[i], ps_solve3d_async(b->ctx[i]));
+        if (r == PS_OK) r = join_from(i);
+        return r;
+    };
+    b->ngrouped = 0;
+    for (const auto& g : b->groups) {
+        std::vector<ps_ctx*> m;
+        for (int j : g) m.push_back(b->ctx[j]);
+        const int lead = g[0];
+        if ((st = fork_to(lead))) return st;
+        st = psamd::ctx_solve_group(m.data(), (int)m.size());
+        if (st == PS_OK) {
+            if ((st = join_from(lead))) return st;
+            b->ngrouped += (int)g.size();
+            continue;
+        }
+        if (st != PS_EUNSUPPORTED) return sub(b, m[0], st);
+        if ((st = join_from(lead))) return st;
+        for (int j : g)
+            if ((st = one(j))) return st;
+    }
+    for (size_t i = 0; i < b->ctx.size(); ++i)
+        if (!grouped(b, (int)i) && (st = one((int)i))) return st;
+    return PS_OK;
+}
+
 int capture(ps_batch* b)
 {
     drop_graph(b);
     BK(hipStreamBeginCapture(b->origin, hipStreamCaptureModeThreadLocal));
-    int st = PS_OK;
-    for (size_t i = 0; i < b->ctx.size() && st == PS_OK; ++i) {
-        hipStream_t s = (hipStream_t)ps_stream(b->ctx[i]);
-        if (hipEventRecord(b->fork[i], b->origin) != hipSuccess || hipStreamWaitEvent(s, b->fork[i], 0) != hipSuccess) {
-            st = bfail(b, PS_EHIP, "graph capture: fork failed");
-            break;
-        }
-        st = sub(b, b->ctx[i], ps_solve3d_async(b->ctx[i]));
-        if (st == PS_OK &&
-            (hipEventRecord(b->join[i], s) != hipSuccess || hipStreamWaitEvent(b->origin, b->join[i], 0) != hipSuccess))
-            st = bfail(b, PS_EHIP, "graph capture: join failed");
-    }
+    int st = enqueue(b);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(b->origin, &g);
     if (st != PS_OK) {
@@ -135,6 +229,8 @@ int ps_batch_create(ps_batch** out, int nprob, const int* ncell, const int* bc, 
         if (st != PS_OK) b->err = "problem " + std::to_string(i) + ": " + ps_last_error(c);
     }
     if (st == PS_OK && hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking) != hipSuccess) st = PS_EHIP;
+    b->elem_bytes = precision == PS_FP64 ? 8 : 4;
+    if (st == PS_OK) st = form_groups(b);
     if (st == PS_OK) {
         b->fork.resize(nprob);
         b->join.resize(nprob);
@@ -173,6 +269,8 @@ int ps_batch_autotune(ps_batch* b, int iters)
             done[j] = true;
         }
     }
+    int st = form_groups(b);
+    if (st) return st;
     drop_graph(b);
     b->warmed = false;
     return PS_OK;
@@ -188,6 +286,8 @@ int ps_batch_destroy(ps_batch* b)
     for (hipEvent_t e : b->join)
         if (e) (void)hipEventDestroy(e);
     for (ps_ctx* c : b->ctx) ps_destroy(c);
+    for (void* p : b->blocks)
+        if (p) (void)hipFree(p);
     if (b->origin) (void)hipStreamDestroy(b->origin);
     delete b;
     return PS_OK;
@@ -242,8 +342,10 @@ int ps_batch_solve(ps_batch* b)
     int st;
     if (!b->warmed) {
         // eager pass: allocates every work buffer outside the capture
+        if ((st = enqueue(b))) return st;
+        BK(hipStreamSynchronize(b->origin));
         for (ps_ctx* c : b->ctx)
-            if ((st = sub(b, c, ps_solve3d(c)))) return st;
+            if ((st = sub(b, c, psamd::ctx_mark_solved(c)))) return st;
         b->warmed = true;
         b->dirty = true;
         return PS_OK;
@@ -283,6 +385,8 @@ int ps_batch_profile(ps_batch* b, int iters, double* ms)
     *ms = t / iters;
     return PS_OK;
 }
+
+int ps_batch_grouped(const ps_batch* b) { return b ? b->ngrouped : -1; }
 
 const char* ps_batch_last_error(const ps_batch* b) { return b ? b->err.c_str() : "NULL batch"; }
 

@@ -94,6 +94,7 @@ struct ps_ctx {
 
     void* rhs = nullptr;    // 3 x n3 real, pushed sources
     void* lhs = nullptr;    // 3 x n3 real, solutions
+    bool ext_rl = false;    // rhs / lhs are slices of a batch's block (not freed here)
     void* rhsf = nullptr;   // 3 x n3 real, filtered sources (computed on demand)
     void* A1 = nullptr;     // ncomp_alloc x nz x ny x Hp complex
     void* A2 = nullptr;     // ncomp_alloc x nz x My x Hp complex (not used by 2D contexts)
@@ -156,8 +157,13 @@ void free_ptr(void*& p)
 
 void release(ps_ctx* c)
 {
-    free_ptr(c->rhs);
-    free_ptr(c->lhs);
+    if (c->ext_rl) {
+        c->rhs = c->lhs = nullptr;
+        c->ext_rl = false;
+    } else {
+        free_ptr(c->rhs);
+        free_ptr(c->lhs);
+    }
     free_ptr(c->rhsf);
     free_ptr(c->A1);
     free_ptr(c->A2);
@@ -1222,6 +1228,74 @@ int ctx_mark_solved(ps_ctx* c)
 int ctx_rhs_mask(const ps_ctx* c)
 {
     return c ? (c->rhs_present[0] ? 1 : 0) | (c->rhs_present[1] ? 2 : 0) | (c->rhs_present[2] ? 4 : 0) : 0;
+}
+
+// rhs and lhs of a set-up context become slices of a caller's device blocks
+// (3 n3 reals each, in the solve precision), which the context does not free;
+// a later setup returns it to buffers of its own.  Pushed components and
+// solutions are discarded.
+int ctx_adopt_real(ps_ctx* c, void* rhs, void* lhs)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (!rhs || !lhs) return fail(c, PS_EINVAL, "adopt: NULL block");
+    if (!c->ext_rl) {
+        free_ptr(c->rhs);
+        free_ptr(c->lhs);
+    }
+    c->rhs = rhs;
+    c->lhs = lhs;
+    c->ext_rl = true;
+    for (int d = 0; d < 3; ++d) c->rhs_present[d] = c->lhs_present[d] = false;
+    c->solved_after_push = false;
+    c->rhsf_valid = false;
+    return PS_OK;
+}
+
+// One solve over K contexts of one grid whose rhs / lhs are consecutive
+// slices of 3 n3 reals of one block (ctx_adopt_real): the members' pushed
+// components become the components of one plain solve on the first member's
+// stream and work buffers (component stride 3 n3 for one pushed component,
+// n3 for three).  Returns PS_EUNSUPPORTED, enqueuing nothing, when the members
+// differ in grid, options, tuning or pushed components, or the operator is not
+// plain; the caller then solves them one by one.  Host bookkeeping (which lhs
+// components were written) is left to the caller (ctx_mark_solved).
+template <typename T> int solve_group_impl(ps_ctx* const* m, int K)
+{
+    ps_ctx* c = m[0];
+    OpSel s;
+    int r = select_op(c, &s);
+    if (r) return r;
+    if (s.op != OP_PLAIN || s.nc != 1 || !(s.nin == 1 || (s.nin == 3 && s.in_step == 1)))
+        return PS_EUNSUPPORTED;
+    if (s.nin == 1)
+        return run_pipeline<T>(c, c->rhs, s.in_first, 3, K, OP_PLAIN, 1, c->lhs, s.in_first, 3, K, nullptr);
+    return run_pipeline<T>(c, c->rhs, 0, 1, 3 * K, OP_PLAIN, 1, c->lhs, 0, 1, 3 * K, nullptr);
+}
+
+int ctx_solve_group(ps_ctx* const* m, int K)
+{
+    if (!m || K < 1) return PS_EINVAL;
+    ps_ctx* c = m[0];
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->dim2 || !c->ext_rl) return PS_EUNSUPPORTED;
+    const size_t slice = 3 * (size_t)c->n3 * real_bytes(c);
+    for (int j = 1; j < K; ++j) {
+        const ps_ctx* o = m[j];
+        if (!o || !o->ready || !o->ext_rl || o->dim2 || o->prec != c->prec) return PS_EUNSUPPORTED;
+        if (o->grad != c->grad || o->div != c->div || o->curl != c->curl || o->reproj != c->reproj ||
+            o->reg != c->reg || ctx_rhs_mask(o) != ctx_rhs_mask(c))
+            return PS_EUNSUPPORTED;
+        for (int d = 0; d < 3; ++d)
+            if (o->ncell[d] != c->ncell[d] || o->bc[d] != c->bc[d] || o->dx[d] != c->dx[d])
+                return PS_EUNSUPPORTED;
+        if (std::memcmp(&o->tune, &c->tune, sizeof(Tuning)) != 0) return PS_EUNSUPPORTED;
+        if (static_cast<const char*>(o->rhs) != static_cast<const char*>(c->rhs) + j * slice ||
+            static_cast<const char*>(o->lhs) != static_cast<const char*>(c->lhs) + j * slice)
+            return PS_EUNSUPPORTED;
+    }
+    return real_bytes(c) == 8 ? solve_group_impl<double>(m, K) : solve_group_impl<float>(m, K);
 }
 }  // namespace psamd
 

@@ -313,6 +313,10 @@ class batch:
         self._check(self._lib.ps_batch_profile(self._b, int(iters), ctypes.byref(ms)), "profile")
         return ms.value
 
+    def grouped(self) -> int:
+        """Problems the last solve ran inside group solves (one solve per grid)."""
+        return int(self._lib.ps_batch_grouped(self._b))
+
 
 def write_vti(directory, stem, ncell, origin, dx, fields, binary=False):
     """VTK output of the reference's test programs

@@ -141,3 +141,86 @@ def test_set_tuning_parity_fp64(fields):
     ref = O.solve(n, bc, dx, reg, [src])[0][0]
     assert O.rel_l2(out, ref) <= 1e-12
     s.close()
+
+
+GROUPED = [
+    # three of one grid, two of another, one alone (16^3 with another reg)
+    ((16, 16, 16), (0, 0, 0), (0.125, 0.125, 0.125), 2),
+    ((12, 10, 9), (0, 0, 0), (0.1, 0.12, 0.09), 0),
+    ((16, 16, 16), (0, 0, 0), (0.125, 0.125, 0.125), 2),
+    ((16, 16, 16), (0, 0, 0), (0.125, 0.125, 0.125), 4),
+    ((12, 10, 9), (0, 0, 0), (0.1, 0.12, 0.09), 0),
+    ((16, 16, 16), (0, 0, 0), (0.125, 0.125, 0.125), 2),
+]
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 2e-6), ("mixed", 2e-6)])
+@pytest.mark.parametrize("comps", ["x", "z", "xyz"])
+def test_batch_groups_equal_single_solves(prec, tol, comps):
+    """Problems of one grid solved as one multi-component solve give, to the
+    bit, what each gives solved alone (component strides 3 n3 for one pushed
+    component, n3 for three), and match the oracle."""
+    code = {"fp64": P.PS_FP64, "fp32": P.PS_FP32, "mixed": P.PS_MIXED}[prec]
+    announce("batch groups", prec, comps)
+    b = P.batch(GROUPED, code)
+    srcs = []
+    for i, (n, bc, dx, reg) in enumerate(GROUPED):
+        r = random_rhs(n, 500 + i, 3)
+        rhs = {"x": [r[0], None, None], "z": [None, None, r[2]], "xyz": r}[comps]
+        srcs.append(rhs)
+        b.push(i, *rhs)
+    for _ in range(3):
+        b.solve()
+        assert b.grouped() == 5
+    for i, (n, bc, dx, reg) in enumerate(GROUPED):
+        shape = (n[2], n[1], n[0])
+        got = [np.full(shape, np.nan) for _ in range(3)]
+        b.pull(i, *got)
+        single, _ = gpu_solve(n, bc, dx, reg, srcs[i], prec=prec)
+        ref, _ = O.solve(n, bc, dx, reg, srcs[i])
+        for d in range(3):
+            if srcs[i][d] is None:
+                assert np.isnan(got[d]).all(), (i, d)
+            else:
+                assert np.array_equal(got[d], single[d]), (i, d)
+        g = [got[d] for d in range(3) if srcs[i][d] is not None]
+        rr = [ref[d] for d in range(3) if srcs[i][d] is not None]
+        assert O.rel_l2(g, rr) <= tol, i
+    b.close()
+
+
+def test_batch_groups_fall_back():
+    """A group whose members pushed different components, or whose operator
+    is not plain, is solved member by member, with the same results."""
+    announce("batch group fallback")
+    b = P.batch(GROUPED, P.PS_FP64)
+    srcs = [random_rhs(p[0], 600 + i, 3) for i, p in enumerate(GROUPED)]
+    # problem 2 pushes X and Z, its group mates X only: that group falls back
+    for i, r in enumerate(srcs):
+        b.push(i, r[0], None, r[2] if i == 2 else None)
+    for _ in range(2):
+        b.solve()
+    assert b.grouped() == 2                # the (12, 10, 9) pair still groups
+    for i, (n, bc, dx, reg) in enumerate(GROUPED):
+        rhs = [srcs[i][0], None, srcs[i][2] if i == 2 else None]
+        shape = (n[2], n[1], n[0])
+        got = [np.full(shape, np.nan) for _ in range(3)]
+        b.pull(i, *got)
+        ref, _ = O.solve(n, bc, dx, reg, rhs)
+        g = [got[d] for d in range(3) if rhs[d] is not None]
+        rr = [ref[d] for d in range(3) if rhs[d] is not None]
+        assert O.rel_l2(g, rr) <= 1e-12, i
+    # grad: not a plain operator, every problem on its own
+    b.set_ops(grad=True)
+    for i, r in enumerate(srcs):
+        b.push(i, *r)
+    for _ in range(2):
+        b.solve()
+    assert b.grouped() == 0
+    for i, (n, bc, dx, reg) in enumerate(GROUPED):
+        shape = (n[2], n[1], n[0])
+        got = [np.full(shape, np.nan) for _ in range(3)]
+        b.pull(i, *got)
+        ref, _ = O.solve(n, bc, dx, reg, srcs[i], op="grad")
+        assert O.rel_l2(got, ref) <= 1e-12, i
+    b.close()
