@@ -178,7 +178,8 @@ typedef struct ps_tuning {
     int zwave;        /* pencil-per-wavefront z kernel where the length allows (PS_TUNE_ZWAVE, 1);
                          FP32 z pencils of 2048 (nz = 1024 unbounded): 1 two 1024-point residues,
                          2 one 2048-point pencil per wavefront, 0 neither */
-    int halfb_z;      /* z tile-width divisor 0/1/2 -> B, B/2, B/4 (PS_TUNE_HALFB_Z, 0) */
+    int halfb_z;      /* z tile-width divisor 0/1/2 -> B, B/2, B/4; -1 = planner: B/2 for z
+                         transforms of 512 or shorter, else B (PS_TUNE_HALFB_Z, -1) */
     int halfb_y;      /* y tile-width divisor 0/1 -> B, B/2 (PS_TUNE_HALFB_Y, 0) */
     int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT-32) kernels where the length
                          has them: 1 one tile per workgroup, 2 persistent workgroups that
