@@ -6,8 +6,8 @@
 #include "ps_kernels.hpp"
 #include "ps_sfft.hpp"
 
-// powers of two, and the doubled lengths of 48, 96, 160 and 192 cells
-#define PS_FOR_EACH_SL(X) X(64) X(96) X(128) X(192) X(256) X(320) X(384) X(512) X(1024) X(2048) X(4096) X(8192)
+// powers of two, and the doubled lengths of 48, 96, 160, 192, 320 and 384 cells
+#define PS_FOR_EACH_SL(X) X(64) X(96) X(128) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024) X(2048) X(4096) X(8192)
 
 namespace psamd {
 

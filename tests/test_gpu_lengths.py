@@ -1,5 +1,5 @@
-"""Specialised kernels at lengths with a factor 3 or 5 (96, 192, 320, 384:
-the doubled 48, 96, 160, 192 cells) in each pass position, and the
+"""Specialised kernels at lengths with a factor 3 or 5 (96, 192, 320, 384,
+640, 768: the doubled 48, 96, 160, 192, 320, 384 cells) in each pass position, and the
 pencil-per-wavefront z kernel at 64 and 128 with z blocks of 4 and 8
 planes, against the oracle (FP64 rel-L2 <= 1e-12, FP32 <= 2e-6)."""
 import pytest
@@ -21,6 +21,12 @@ CASES = [
     ((96, 192, 12), (0, 0, 1)),     # 192, 384, 12
     ((16, 16, 32), (0, 0, 0)),      # z 64: wavefront kernel, 4-plane z blocks
     ((16, 24, 64), (0, 0, 0)),      # z 128: wavefront kernel, 8-plane z blocks
+    ((320, 6, 8), (0, 0, 0)),       # 640, 12, 16
+    ((6, 384, 8), (0, 0, 0)),       # 12, 768, 16
+    ((8, 6, 320), (0, 0, 0)),       # 16, 12, 640
+    ((6, 8, 384), (0, 0, 0)),       # 12, 16, 768
+    ((640, 8, 6), (1, 1, 0)),       # 640 periodic, 8, 12
+    ((8, 6, 768), (1, 1, 1)),       # 8, 6, 768 periodic
 ]
 
 
@@ -45,7 +51,8 @@ def test_specialised_kernels_selected():
     """The new lengths run the specialised kernels (not the runtime-plan ones),
     and 64 / 128 z pencils the pencil-per-wavefront kernel."""
     for n, want in (((48, 96, 160), "specialised"), ((16, 16, 32), "pencil-per-wavefront"),
-                    ((16, 24, 64), "pencil-per-wavefront"), ((8, 8, 192), "specialised")):
+                    ((16, 24, 64), "pencil-per-wavefront"), ((8, 8, 192), "specialised"),
+                    ((8, 8, 320), "specialised"), ((8, 8, 384), "specialised")):
         s = P.poisson_solver(P.PS_FP32)
         s.setup3d(n, (0, 0, 0), (0.1, 0.1, 0.1), 2)
         assert s.zpass_variant() == want, (n, s.zpass_variant())
