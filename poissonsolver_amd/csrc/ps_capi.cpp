@@ -573,13 +573,14 @@ template <typename T> int setup_impl(ps_ctx* c)
     // specialised kernels: fixed E and tile shape per length
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) {
         int E = 0, B = 0, P = 0;
-        // FP64 pencils of 640 (E = 20: 80 VGPRs of data per pencil slice) are
-        // slower on the specialised fused z and x kernels: measured 4.51 / 0.57 ms
-        // (z, x^-1) against 2.41 / 0.44 ms on the runtime-plan kernels at 320^3;
-        // their y passes keep the specialised kernels (0.50 against 0.67 ms)
-        const bool slow = eb == 8 && d->p.L == 640 && d != &c->py;
-        const bool allowed = tn.specialised && (d != &c->pz3 || tn.specialised3) && !slow;
-        if (allowed && static_policy(d->p.L, eb, d == &c->pz3 ? 3 : 1, &E, &B, &P)) {
+        const bool allowed = tn.specialised && (d != &c->pz3 || tn.specialised3);
+        const bool have = allowed && static_policy(d->p.L, eb, d == &c->pz3 ? 3 : 1, &E, &B, &P);
+        // FP64 with 20 elements per thread (lengths 320, 640: 80 VGPRs of data
+        // per thread) is slower on the specialised fused z and x kernels than on
+        // the runtime-plan ones (z 0.39 / 4.51 ms against 0.28 / 2.41 ms at
+        // 160^3 / 320^3); the y passes keep the specialised kernels (320^3:
+        // 1.03 against 1.40 ms for y and y^-1)
+        if (have && !(eb == 8 && E == 20 && d != &c->py)) {
             d->stat = true;
             d->p.E = E;
             d->p.TP = d->p.L / E;
