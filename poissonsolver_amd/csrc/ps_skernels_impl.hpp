@@ -24,6 +24,13 @@ template <typename T, int L> constexpr int s_x_minw()
     return waves < 1 ? 1 : (waves > 4 ? 4 : waves);
 }
 
+// The inverse x kernel of FP64 pencils of 8192 reads its stage twiddles from
+// the table: held in registers they take about 150 VGPRs besides the pencil's
+// 64 and the kernel spilled (x^-1 of a 4096^2 2D solve 0.446 -> 0.335 ms).  The
+// forward kernel, bounded to the occupancy its LDS allows, keeps them in
+// registers (table twiddles measured 0.145 -> 0.163 ms there).
+template <typename T, int L> constexpr bool s_xi_twreg() { return !(sizeof(T) == 8 && L >= 8192); }
+
 template <typename T, int L, int HALF>
 __global__ void __launch_bounds__(kMaxThreads, (s_x_minw<T, L>())) k_x_r2c_s(XArgs<T> a)
 {
@@ -99,7 +106,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
     const C* in = reinterpret_cast<const C*>(a.in) + comp * a.c_comp;
 
     STw<L, E, T> W;
-    load_stwiddles<L, E, T>(W, t, a.plan.tw);
+    if constexpr (s_xi_twreg<T, L>()) load_stwiddles<L, E, T>(W, t, a.plan.tw);
     C v[E];
 #pragma unroll
     for (int u = 0; u < E; ++u) {
@@ -123,7 +130,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
     }
     C* my = lds + pp * lds_len(L);
     constexpr int WS = (64 % TP == 0) ? 1 : 0;
-    sfft<L, E, 1, +1, T, WS>(v, W, my, t);
+    if constexpr (s_xi_twreg<T, L>()) sfft<L, E, 1, +1, T, WS>(v, W, my, t);
+    else sfft_g<L, E, 1, +1, T, WS>(v, a.plan.tw, my, t);
     T* out = reinterpret_cast<T*>(a.out) + comp * a.r_comp;
 #pragma unroll
     for (int u = 0; u < (HALF ? E / 2 : E); ++u) {
