@@ -224,3 +224,52 @@ def test_batch_groups_fall_back():
         ref, _ = O.solve(n, bc, dx, reg, srcs[i], op="grad")
         assert O.rel_l2(got, ref) <= 1e-12, i
     b.close()
+
+
+def test_batch_group_reproject_and_filtered_rhs():
+    """A grouped plain solve with reprojection: each member's solution and the
+    filtered, reprojected rhs it pulls match the oracle."""
+    announce("batch group reproject")
+    b = P.batch(GROUPED, P.PS_FP64)
+    b.set_ops(reproject=True)
+    srcs = [random_rhs(p[0], 700 + i, 3) for i, p in enumerate(GROUPED)]
+    for i, r in enumerate(srcs):
+        b.push(i, *r)
+    for _ in range(2):
+        b.solve()
+    assert b.grouped() == 5
+    for i, (n, bc, dx, reg) in enumerate(GROUPED):
+        shape = (n[2], n[1], n[0])
+        lhs = [np.full(shape, np.nan) for _ in range(3)]
+        rhs = [np.full(shape, np.nan) for _ in range(3)]
+        b.pull(i, *lhs, *rhs)
+        lo, ro = O.solve(n, bc, dx, reg, srcs[i], reproject=True)
+        assert O.rel_l2(lhs, lo) <= 1e-12, i
+        assert O.rel_l2(rhs, ro) <= 1e-12, i
+    b.close()
+
+
+def test_batch_member_set_up_again_leaves_its_group():
+    """A member set up again through its context returns to buffers of its
+    own; its group is then solved member by member, with the same results."""
+    import ctypes
+    announce("batch member re-setup")
+    b = P.batch(GROUPED, P.PS_FP32)
+    c = ctypes.c_void_p()
+    assert b._lib.ps_batch_context(b._b, 2, ctypes.byref(c)) == 0
+    n, bc, dx, reg = GROUPED[2]
+    st = b._lib.ps_setup3d(c, (ctypes.c_int * 3)(*n), (ctypes.c_int * 3)(*bc), (ctypes.c_double * 3)(*dx), reg)
+    assert st == 0
+    srcs = [random_rhs(p[0], 800 + i, 1)[0] for i, p in enumerate(GROUPED)]
+    for i, s in enumerate(srcs):
+        b.push(i, s)
+    for _ in range(2):
+        b.solve()
+    assert b.grouped() == 2                # only the (12, 10, 9) pair
+    for i, (n, bc, dx, reg) in enumerate(GROUPED):
+        shape = (n[2], n[1], n[0])
+        got = [np.full(shape, np.nan) for _ in range(3)]
+        b.pull(i, *got)
+        ref, _ = O.solve(n, bc, dx, reg, [srcs[i]])
+        assert O.rel_l2(got[0], ref[0]) <= 2e-6, i
+    b.close()
