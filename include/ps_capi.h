@@ -176,6 +176,7 @@ typedef struct ps_tuning {
     int twreg_y;      /* y passes keep stage twiddles in registers (PS_TUNE_TWREG_Y, 0) */
     int occ4_z;       /* z pass targets 4 waves per SIMD instead of 2 (PS_TUNE_OCC4_Z, 0) */
     int zwave;        /* pencil-per-wavefront z kernel where the length allows (PS_TUNE_ZWAVE, 1);
+                         1 leaves FP64 pencils of 1024 on the fused kernel (2 does not);
                          FP32 z pencils of 2048 (nz = 1024 unbounded): 1 two 1024-point residues,
                          2 one 2048-point pencil per wavefront, 0 neither */
     int halfb_z;      /* z tile-width divisor 0/1/2 -> B, B/2, B/4; -1 = planner: B/2 for z

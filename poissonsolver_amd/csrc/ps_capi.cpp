@@ -611,8 +611,13 @@ template <typename T> int setup_impl(ps_ctx* c)
     // inside one wavefront, two or more stages; it reads Gk along kz
     // (its elements per thread: 16, or 32 for FP32 pencils of 2048 (zwave_E))
     const int zwE = (c->pz1.p.L > 1024 && c->prec != PS_FP64) ? 32 : 16;
-    c->zwave = tn.zwave && c->pz1.stat && c->pz1.p.L % zwE == 0 && (64 % (c->pz1.p.L / zwE) == 0) &&
-               c->pz1.p.nstage >= 2 && (c->zb <= 0 || (c->pz1.p.L / zwE) % (1 << c->zb) == 0);
+    // planner (zwave 1): FP64 pencils of 1024 run the fused specialised kernel
+    // (z pass of 512^3 4.16 against 4.53 ms; the wavefront kernel is ahead up
+    // to 256 and level at 512); zwave 2 still selects the wavefront kernel
+    const bool zw_fp64_long = tn.zwave == 1 && c->prec == PS_FP64 && c->pz1.p.L >= 1024;
+    c->zwave = tn.zwave && !zw_fp64_long && c->pz1.stat && c->pz1.p.L % zwE == 0 &&
+               (64 % (c->pz1.p.L / zwE) == 0) && c->pz1.p.nstage >= 2 &&
+               (c->zb <= 0 || (c->pz1.p.L / zwE) % (1 << c->zb) == 0);
     if (c->zwave && zwE == 32) c->pz1.tile = tn.halfb_z ? kZwaveB32 / 2 : kZwaveB32;
     if (c->zwave && zwE == 16) {
         // the wavefront kernel is built for the policy's tile width and half
