@@ -162,7 +162,8 @@ int ps_zpass_variant(const ps_ctx* ctx);
    override the defaults.  No setting changes results beyond round-off. */
 typedef struct ps_tuning {
     int zblk;         /* z blocking of the y/z-transposed spectrum: log2 block > 0,
-                         0 = [z][ky][kx], -1 = [ky][z][kx], -2 = planner (PS_TUNE_ZBLK) */
+                         0 = [z][ky][kx], -1 = [ky][z][kx], -2 = planner: 16-plane blocks,
+                         fewer for short z pencils, none from 1024 planes (PS_TUNE_ZBLK) */
     int tile_x;       /* row pairs per workgroup, x passes; 0 = planner (PS_TUNE_PX) */
     int tile_y;       /* columns per workgroup, y passes; 0 = planner (PS_TUNE_BY) */
     int tile_z;       /* columns per workgroup, z pass; 0 = planner (PS_TUNE_BZ) */
@@ -173,7 +174,8 @@ typedef struct ps_tuning {
     int specialised;  /* compile-time specialised kernels where the length has one (PS_TUNE_STATIC, 1) */
     int specialised3; /* ... also for the 3-component z pass (PS_TUNE_STATIC3, 1) */
     int twreg_z;      /* z pass keeps stage twiddles in registers (PS_TUNE_TWREG_Z, 1) */
-    int twreg_y;      /* y passes keep stage twiddles in registers (PS_TUNE_TWREG_Y, 0) */
+    int twreg_y;      /* y passes keep stage twiddles in registers; -1 = planner: yes for single
+                         precision y transforms of 1024 and longer (PS_TUNE_TWREG_Y, -1) */
     int occ4_z;       /* z pass targets 4 waves per SIMD instead of 2 (PS_TUNE_OCC4_Z, 0) */
     int zwave;        /* pencil-per-wavefront z kernel where the length allows (PS_TUNE_ZWAVE, 1);
                          1 leaves FP64 pencils of 1024 on the fused kernel (2 does not);

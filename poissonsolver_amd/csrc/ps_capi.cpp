@@ -66,7 +66,7 @@ Tuning tuning_default()
     t.specialised = env_int("PS_TUNE_STATIC", 1);
     t.specialised3 = env_int("PS_TUNE_STATIC3", 1);
     t.twreg_z = env_int("PS_TUNE_TWREG_Z", 1);
-    t.twreg_y = env_int("PS_TUNE_TWREG_Y", 0);
+    t.twreg_y = env_int("PS_TUNE_TWREG_Y", -1);
     t.occ4_z = env_int("PS_TUNE_OCC4_Z", 0);
     t.zwave = env_int("PS_TUNE_ZWAVE", 1);
     t.halfb_z = env_int("PS_TUNE_HALFB_Z", -1);
@@ -519,6 +519,10 @@ template <typename T> int setup_impl(ps_ctx* c)
     // z tiles of half width where the z transform is 512 or shorter (measured
     // 3-10 % faster z passes for Mz = 256 ... 512, 2 % slower at 1024)
     if (tn.halfb_z < 0) tn.halfb_z = (c->M[2] <= 512) ? 1 : 0;
+    // y-pass stage twiddles in registers for single-precision y transforms of
+    // 1024 and longer (y^-1 of 512^3 0.82 -> 0.77 ms, y + y^-1 of 1024^3
+    // 16.6 -> 15.1 ms); FP64 keeps them in the table (1024^3: 48.5 -> 86 ms)
+    if (tn.twreg_y < 0) tn.twreg_y = (c->prec != PS_FP64 && c->M[1] >= 1024) ? 1 : 0;
     const int lds_bytes = tn.lds_kb * 1024;
 
     // A2 layout: z blocked by Zb = 2^zb planes, [z/Zb][ky][z%Zb][kx]; zb = 0 is
@@ -528,8 +532,9 @@ template <typename T> int setup_impl(ps_ctx* c)
     if (zb == -2) {
         // 16-plane blocks (measured best at 512^3), fewer when the specialised
         // z kernels' threads per pencil (Mz / E) are fewer than 16: a block
-        // must not split a thread's positions
-        zb = (nz >= 32) ? 4 : 0;
+        // must not split a thread's positions; from 1024 planes the plain
+        // [z][ky][kx] layout (z pass of 1024^3 26.8 -> 25.8 ms)
+        zb = (nz >= 1024) ? 0 : (nz >= 32) ? 4 : 0;
         int E1 = 0, B1 = 0, P1 = 0;
         if (zb > 0 && tn.specialised && static_policy(c->M[2], real_bytes(c), 1, &E1, &B1, &P1))
             while (zb > 0 && (c->M[2] / E1) % (1 << zb) != 0) --zb;
