@@ -57,3 +57,35 @@ def test_specialised_kernels_selected():
         s.setup3d(n, (0, 0, 0), (0.1, 0.1, 0.1), 2)
         assert s.zpass_variant() == want, (n, s.zpass_variant())
         s.close()
+
+
+LONG_Y3D = [
+    ((6, 512, 8), (0, 0, 0)),       # y transform 1024
+    ((8, 1024, 6), (0, 0, 0)),      # y transform 2048
+    ((8, 1024, 6), (1, 1, 0)),      # y periodic 1024
+]
+
+
+@pytest.mark.parametrize("case", LONG_Y3D, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+@pytest.mark.parametrize("op", ["plain", "grad", "curl"])
+@pytest.mark.parametrize("twreg_y", [-1, 0])
+def test_long_y_single_precision(case, prec, op, twreg_y):
+    """Single-precision y transforms of 1024 and 2048: stage twiddles in
+    registers (the planner's choice there, -1) and from the table (0)."""
+    n, bc = case
+    dx = (0.1, 1.0 / n[1], 0.12)
+    announce("long y", n, bc, prec, op, "twreg_y", twreg_y)
+    b = random_rhs(n, 61)
+    if op == "plain":
+        b = [b[0], None, None]
+    s = P.poisson_solver(P.PS_FP32 if prec == "fp32" else P.PS_MIXED)
+    s.set_tuning(twreg_y=twreg_y)
+    s.setup3d(n, bc, dx, 2)
+    assert s.last_status == 0
+    lg, _ = gpu_solve(n, bc, dx, 2, b, op=op, prec=prec, solver=s)
+    s.close()
+    lo, _ = O.solve(n, bc, dx, 2, b, op=op)
+    got = [lg[d] for d in range(3) if lo[d] is not None]
+    ref = [lo[d] for d in range(3) if lo[d] is not None]
+    assert O.rel_l2(got, ref) <= TOL[prec]
