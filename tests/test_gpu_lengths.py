@@ -89,3 +89,25 @@ def test_long_y_single_precision(case, prec, op, twreg_y):
     got = [lg[d] for d in range(3) if lo[d] is not None]
     ref = [lo[d] for d in range(3) if lo[d] is not None]
     assert O.rel_l2(got, ref) <= TOL[prec]
+
+
+@pytest.mark.parametrize("op", ["plain", "grad", "curl"])
+@pytest.mark.parametrize("zwave,want", [(1, "specialised"), (2, "pencil-per-wavefront")])
+def test_fp64_z_1024_fused_and_wavefront(op, zwave, want):
+    """FP64 z transforms of 1024: the planner's fused specialised kernel
+    (zwave 1) and the pencil-per-wavefront kernel (zwave 2)."""
+    n, bc, dx = (6, 8, 512), (0, 0, 0), (0.1, 0.12, 1 / 256)
+    announce("fp64 z 1024", op, zwave)
+    b = random_rhs(n, 67)
+    if op == "plain":
+        b = [b[0], None, None]
+    s = P.poisson_solver(P.PS_FP64)
+    s.set_tuning(zwave=zwave)
+    s.setup3d(n, bc, dx, 2)
+    assert s.zpass_variant() == want
+    lg, _ = gpu_solve(n, bc, dx, 2, b, op=op, prec="fp64", solver=s)
+    s.close()
+    lo, _ = O.solve(n, bc, dx, 2, b, op=op)
+    got = [lg[d] for d in range(3) if lo[d] is not None]
+    ref = [lo[d] for d in range(3) if lo[d] is not None]
+    assert O.rel_l2(got, ref) <= 1e-12
