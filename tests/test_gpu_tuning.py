@@ -44,3 +44,28 @@ def test_variant_parity(env, case, monkeypatch):
     got = [lg[d] for d in range(3) if lo[d] is not None]
     ref = [lo[d] for d in range(3) if lo[d] is not None]
     assert O.rel_l2(got, ref) <= 1e-12
+
+
+def test_planner_rules_resolve_as_documented():
+    """The planner's size rules, read back from the tuning a setup used:
+    half-width z tiles for z transforms up to 512, register y twiddles for
+    single-precision y transforms from 1024 (never FP64), the fused z kernel
+    for FP64 pencils of 1024 and the wavefront kernel for FP32 ones."""
+    import poissonsolver_amd as P
+    cases = [
+        # (ncell, precision, halfb_z, twreg_y, z variant)
+        ((6, 512, 8), P.PS_FP32, 1, 1, None),        # My 1024, Mz 16
+        ((6, 512, 8), P.PS_FP64, 1, 0, None),
+        ((6, 256, 8), P.PS_FP32, 1, 0, None),        # My 512
+        ((6, 8, 512), P.PS_FP32, 0, 0, "pencil-per-wavefront"),   # Mz 1024
+        ((6, 8, 512), P.PS_FP64, 0, 0, "specialised"),
+        ((6, 8, 256), P.PS_FP64, 1, 0, "pencil-per-wavefront"),   # Mz 512
+    ]
+    for n, prec, hb, ty, zv in cases:
+        s = P.poisson_solver(prec)
+        s.setup3d(n, (0, 0, 0), (0.1, 0.1, 0.1), 2)
+        t = s.tuning()
+        assert (t["halfb_z"], t["twreg_y"]) == (hb, ty), (n, prec, t)
+        if zv is not None:
+            assert s.zpass_variant() == zv, (n, prec, s.zpass_variant())
+        s.close()
