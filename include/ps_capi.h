@@ -67,7 +67,14 @@ int ps_set_reprojection(ps_ctx* ctx, int toggle);
 /* replaces poisson_solver::push(double*, double*, double*) (push.cpp:9-397);
    NULL = component absent */
 int ps_push(ps_ctx* ctx, const double* bx, const double* by, const double* bz);
-/* replaces poisson_solver::solve3d() (solve3d.cpp:8-635) */
+/* replaces poisson_solver::solve3d() (solve3d.cpp:8-635).  Solves the held
+   rhs: after a push, the pushed field; after a solve with regularisation (or
+   reprojection of three components), the filtered field that solve left in
+   the reference's rhs arrays -- z and y truncated, x extended over the doubled
+   rows (solve3d.cpp:403-629), mapped and transformed again by the next solve
+   (map.cpp:50-60).  The reference's x pencils also carry an imaginary part
+   where its reprojection uses the negative Nyquist wavenumber; the held field
+   here is their real part (what pull returns). */
 int ps_solve3d(ps_ctx* ctx);
 /* replaces poisson_solver::solve2d() (poisson_solver.hpp:118, solve2d.cpp:8-475):
    plain (every pushed component), grad (of X, two components), div (X, Y ->
@@ -129,15 +136,24 @@ int ps_pull_device(ps_ctx* ctx, void* ax, void* ay, void* az, void* bx, void* by
 /* the context's HIP stream (hipStream_t) */
 void* ps_stream(ps_ctx* ctx);
 int ps_synchronize(ps_ctx* ctx);
-/* run `iters` solves and time each pass with HIP events on the context's
-   stream: ms[0..4] = x, y, z, y^-1, x^-1 pass averages, ms[5] = whole solve */
+/* run `iters` solves of the held rhs and time each pass with HIP events on
+   the context's stream: ms[0..4] = x, y, z, y^-1, x^-1 pass averages, ms[5] =
+   whole solve.  Every timed solve solves the held rhs as it stands: the
+   filter a solve3d leaves on the held rhs is not applied between them. */
 int ps_profile_solve(ps_ctx* ctx, int iters, double ms[6]);
 /* describe the plan chosen for transform length L: E, threads per pencil,
    number of stages and radices (up to 14); returns PS_EUNSUPPORTED if none */
 int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int* nstage, int radix[14]);
 /* evaluate the device Si (which = 0) or Ji0 (which = 1) at n host arguments */
 int ps_eval_special(const double* x, double* y, long n, int which);
-/* enqueue ps_solve3d on the context's stream and return without waiting */
+/* device memory a context may hold (setup, work buffers, staging): a request
+   beyond it fails with PS_ENOMEM, as a device out of memory does; 0 = no
+   limit (the default).  Buffers already held are kept. */
+int ps_set_device_memory_limit(ps_ctx* ctx, unsigned long long bytes);
+/* device bytes the context holds now */
+int ps_device_memory_used(const ps_ctx* ctx, unsigned long long* bytes);
+/* enqueue ps_solve3d on the context's stream and return without waiting;
+   push, pull and solve calls that follow run after it (same stream) */
 int ps_solve3d_async(ps_ctx* ctx);
 /* enqueue ps_solve2d on the context's stream and return without waiting */
 int ps_solve2d_async(ps_ctx* ctx);
@@ -211,7 +227,11 @@ int ps_autotune(ps_ctx* ctx, int iters, ps_tuning* best, double* best_ms);
    Problems of one grid (ncell, bc, dx, reg_order) keep their sources and
    solutions in one device block; when the operator is plain and their pushed
    components agree they are solved as one multi-component solve (five launches
-   for the group), with results identical to solving each alone. */
+   for the group), with results identical to solving each alone.  Every batch
+   solve solves each problem's pushed rows (a filtered rhs that an earlier
+   solve left held is dropped).  Before a replay every member is checked; a
+   member set up again, pushed with other components or given other options
+   since the capture makes the batch solve eagerly once and capture again. */
 typedef struct ps_batch ps_batch;
 /* ncell, bc, dx: nprob x 3; reg_order: nprob entries */
 int ps_batch_create(ps_batch** out, int nprob, const int* ncell, const int* bc, const double* dx,

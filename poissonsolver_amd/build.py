@@ -55,26 +55,34 @@ def sources():
         units.append(("kern_e.hip", ["-DPS_E=%d" % e], "kern_e%d.o" % e))
     for L in sl_values():
         units.append(("skern.hip", ["-DPS_SL=%d" % L], "skern_%d.o" % L))
-    # the matrix-core kernels take magnitudes with fmax; no NaN handling needed
-    units.append(("mkern.hip", ["-fno-honor-nans"], "mkern.o"))
+    units.append(("mkern.hip", [], "mkern.o"))
     for f in ("ps_greens.hip", "ps_io.hip", "ps_dispatch.cpp", "ps_plan.cpp", "ps_capi.cpp", "ps_batch.cpp", "poisson_solver.cpp",
               "ps_vtk.cpp"):
         units.append((f, [], os.path.splitext(f)[0] + ".o"))
     return units
 
 
-def _deps_mtime():
-    newest = 0.0
-    for root in (CSRC, os.path.join(os.path.dirname(HERE), "include")):
-        for f in os.listdir(root):
-            if f.endswith((".hpp", ".h", ".hip", ".cpp")):
-                newest = max(newest, os.path.getmtime(os.path.join(root, f)))
+_INC = re.compile(r'^\s*#\s*include\s+"([^"]+)"', re.M)
+
+
+def _deps_mtime(path, seen=None):
+    """Newest modification time of a source and the project headers it
+    includes (quoted includes, followed recursively)."""
+    seen = set() if seen is None else seen
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return 0.0
+    seen.add(path)
+    newest = os.path.getmtime(path)
+    for inc in _INC.findall(open(path).read()):
+        newest = max(newest, _deps_mtime(os.path.join(os.path.dirname(path), inc), seen))
     return newest
 
 
 def _compile(unit, force, newest):
     src, defs, obj = unit
     out = os.path.join(BUILD, obj)
+    newest = max(newest, _deps_mtime(os.path.join(CSRC, src)))
     if not force and os.path.exists(out) and os.path.getmtime(out) >= newest:
         return out, ""
     lang = [] if src.endswith(".hip") else ["-x", "hip"]
@@ -89,7 +97,7 @@ def _compile(unit, force, newest):
 
 def build(jobs=None, force=False, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    newest = _deps_mtime()
+    newest = os.path.getmtime(os.path.abspath(__file__))   # compile flags live here
     units = sources()
     jobs = jobs or min(8, os.cpu_count() or 4)
     objs = []

@@ -27,6 +27,9 @@ int ctx_mark_solved(ps_ctx* c);      // ps_capi.cpp
 int ctx_rhs_mask(const ps_ctx* c);
 int ctx_adopt_real(ps_ctx* c, void* rhs, void* lhs);
 int ctx_solve_group(ps_ctx* const* m, int K);
+void ctx_reset_held(ps_ctx* c);
+unsigned long ctx_signature(const ps_ctx* c);
+int ctx_check_solvable(ps_ctx* c);
 }  // namespace psamd
 
 struct ps_batch {
@@ -43,6 +46,7 @@ struct ps_batch {
     hipGraphExec_t exec = nullptr;
     bool dirty = true;          // graph must be (re)captured
     bool warmed = false;        // one eager solve done (buffers allocated)
+    std::vector<unsigned long> sig;   // members' signatures when the graph was captured
     std::string err;
 };
 
@@ -340,6 +344,19 @@ int ps_batch_solve(ps_batch* b)
 {
     if (!b) return PS_EINVAL;
     int st;
+    // every member must be solvable before anything is launched
+    std::vector<unsigned long> sig(b->ctx.size());
+    for (size_t i = 0; i < b->ctx.size(); ++i) {
+        if ((st = sub(b, b->ctx[i], psamd::ctx_check_solvable(b->ctx[i])))) return st;
+        psamd::ctx_reset_held(b->ctx[i]);
+        sig[i] = psamd::ctx_signature(b->ctx[i]);
+    }
+    // a member set up again (new buffers), pushed with other components or
+    // given other options since the capture: solve eagerly, then re-capture
+    if (b->warmed && sig != b->sig) {
+        b->warmed = false;
+        drop_graph(b);
+    }
     if (!b->warmed) {
         // eager pass: allocates every work buffer outside the capture
         if ((st = enqueue(b))) return st;
@@ -348,6 +365,7 @@ int ps_batch_solve(ps_batch* b)
             if ((st = sub(b, c, psamd::ctx_mark_solved(c)))) return st;
         b->warmed = true;
         b->dirty = true;
+        b->sig = sig;
         return PS_OK;
     }
     if (b->dirty && (st = capture(b))) return st;

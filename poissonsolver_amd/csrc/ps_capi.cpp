@@ -95,7 +95,9 @@ struct ps_ctx {
     void* rhs = nullptr;    // 3 x n3 real, pushed sources
     void* lhs = nullptr;    // 3 x n3 real, solutions
     bool ext_rl = false;    // rhs / lhs are slices of a batch's block (not freed here)
-    void* rhsf = nullptr;   // 3 x n3 real, filtered sources (computed on demand)
+    void* rhsf = nullptr;   // 3 x n3 real, the held rhs as pull returns it (x < nx) when it is `spec`
+    void* spec = nullptr;   // 3 x A1c complex: x spectra of the held rhs over the x-extended rows
+    void* stage = nullptr;  // kStageReals doubles: host-pointer push / pull staging (FP32 forms)
     void* A1 = nullptr;     // ncomp_alloc x nz x ny x Hp complex
     void* A2 = nullptr;     // ncomp_alloc x nz x My x Hp complex (not used by 2D contexts)
     long A1c = 0, A2c = 0;  // complex elements per component
@@ -120,10 +122,24 @@ struct ps_ctx {
 
     bool rhs_present[3] = {false, false, false};
     bool lhs_present[3] = {false, false, false};
-    bool solved_after_push = false;
-    bool rhsf_valid = false;
-    bool solve_filter_active = false;   // reg > 0 or reprojection, at the last solve
-    bool solve_reproj = false;
+    // The held rhs (the input of the next solve and what pull returns as rhs),
+    // as the reference keeps it in rhsX/Y/Z: after a push, the pushed field;
+    // after a solve with a filter (regularisation, or reprojection of three
+    // components), the filtered field on the x-extended rows
+    // (solve3d.cpp:403-439, :447-629), which the next solve maps and
+    // transforms again (map.cpp:50-60).  Here the filter is applied when
+    // something reads the held rhs (filt_pending), and the filtered field is
+    // kept as its x spectra (`spec`, src_spec), from which the next solve
+    // starts at the y pass.
+    bool src_spec = false;      // held rhs = spec (else the pushed `rhs`)
+    bool filt_pending = false;  // held rhs = filter(held rhs), not yet applied
+    bool filt_reproj = false;   // ... with reprojection
+    bool rhsf_valid = false;    // rhsf holds the real rows of spec
+    unsigned gen = 0;           // setup generation (a batch re-captures when it changes)
+
+    size_t mem_used = 0;        // device bytes held by this context
+    size_t mem_cap = ~(size_t)0;   // ps_set_device_memory_limit
+    std::vector<std::pair<void*, size_t>> allocs;
 
     hipEvent_t ev[6] = {};
     bool ev_made = false;
@@ -149,9 +165,34 @@ int fail(ps_ctx* c, int status, const std::string& msg)
                         std::string(#call) + ": " + hipGetErrorString(e_));                      \
     } while (0)
 
-void free_ptr(void*& p)
+// Device memory of a context: counted against its limit
+// (ps_set_device_memory_limit); a request beyond it fails as the runtime's
+// own out-of-memory does.
+hipError_t dev_alloc(ps_ctx* c, void** p, size_t bytes)
 {
-    if (p) (void)hipFree(p);
+    *p = nullptr;
+    if (bytes > c->mem_cap || c->mem_used > c->mem_cap - bytes) return hipErrorOutOfMemory;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    c->allocs.emplace_back(*p, bytes);
+    c->mem_used += bytes;
+    return hipSuccess;
+}
+
+void free_ptr(ps_ctx* c, void*& p)
+{
+    if (!p) return;
+    for (size_t i = 0; i < c->allocs.size(); ++i)
+        if (c->allocs[i].first == p) {
+            c->mem_used -= c->allocs[i].second;
+            c->allocs[i] = c->allocs.back();
+            c->allocs.pop_back();
+            break;
+        }
+    (void)hipFree(p);
     p = nullptr;
 }
 
@@ -161,19 +202,22 @@ void release(ps_ctx* c)
         c->rhs = c->lhs = nullptr;
         c->ext_rl = false;
     } else {
-        free_ptr(c->rhs);
-        free_ptr(c->lhs);
+        free_ptr(c, c->rhs);
+        free_ptr(c, c->lhs);
     }
-    free_ptr(c->rhsf);
-    free_ptr(c->A1);
-    free_ptr(c->A2);
-    free_ptr(c->G);
-    free_ptr(c->mfrag);
-    free_ptr(c->tw_half);
+    free_ptr(c, c->rhsf);
+    free_ptr(c, c->spec);
+    free_ptr(c, c->stage);
+    free_ptr(c, c->A1);
+    free_ptr(c, c->A2);
+    free_ptr(c, c->G);
+    free_ptr(c, c->mfrag);
+    free_ptr(c, c->tw_half);
     c->zres = false;
-    for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) free_ptr(d->tw);
+    for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) free_ptr(c, d->tw);
     c->ncomp_alloc = 0;
     c->ready = false;
+    c->src_spec = c->filt_pending = c->filt_reproj = c->rhsf_valid = false;
 }
 
 template <typename T> int upload_twiddles(ps_ctx* c, DevPlan* d)
@@ -185,7 +229,7 @@ template <typename T> int upload_twiddles(ps_ctx* c, DevPlan* d)
         h[2 * j] = (T)re[j];
         h[2 * j + 1] = (T)im[j];
     }
-    CK(hipMalloc(&d->tw, h.size() * sizeof(T)));
+    CK(dev_alloc(c, &d->tw, h.size() * sizeof(T)));
     CK(hipMemcpy(d->tw, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return PS_OK;
 }
@@ -204,11 +248,11 @@ int plan_dim(ps_ctx* c, DevPlan* d, int L, int ncomp, const char* what)
 int ensure_comp_buffers(ps_ctx* c, int ncomp)
 {
     if (ncomp <= c->ncomp_alloc) return PS_OK;
-    free_ptr(c->A1);
-    free_ptr(c->A2);
+    free_ptr(c, c->A1);
+    free_ptr(c, c->A2);
     const size_t cb = 2 * (size_t)real_bytes(c);
-    CK(hipMalloc(&c->A1, (size_t)ncomp * c->A1c * cb));
-    if (c->A2c > 0) CK(hipMalloc(&c->A2, (size_t)ncomp * c->A2c * cb));
+    CK(dev_alloc(c, &c->A1, (size_t)ncomp * c->A1c * cb));
+    if (c->A2c > 0) CK(dev_alloc(c, &c->A2, (size_t)ncomp * c->A2c * cb));
     c->ncomp_alloc = ncomp;
     return PS_OK;
 }
@@ -216,7 +260,15 @@ int ensure_comp_buffers(ps_ctx* c, int ncomp)
 int ensure_real(ps_ctx* c, void** p)
 {
     if (*p) return PS_OK;
-    CK(hipMalloc(p, 3 * (size_t)c->n3 * real_bytes(c)));
+    CK(dev_alloc(c, p, 3 * (size_t)c->n3 * real_bytes(c)));
+    return PS_OK;
+}
+
+// x spectra of the held rhs: three components shaped like A1
+int ensure_spec(ps_ctx* c)
+{
+    if (c->spec) return PS_OK;
+    CK(dev_alloc(c, &c->spec, 3 * (size_t)c->A1c * 2 * real_bytes(c)));
     return PS_OK;
 }
 
@@ -372,21 +424,79 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
     a.fzero = 0.0f;
 }
 
+// Where a pipeline starts and ends (run_pipeline flags).
+enum PipeFlags : int {
+    PF_SRC_SPEC = 1,   // no x pass: the y pass reads the held x spectra (spec), components as for src
+    PF_DST_SPEC = 2,   // no x^-1 pass: the y^-1 pass writes x spectra into spec, components as for dst
+    PF_X_ONLY = 4,     // only the x pass, writing the x spectra of src into spec (components of dst)
+    PF_XINV_ONLY = 8,  // only the x^-1 pass, from spec (components of src) into dst
+};
+
 // The five passes (three for a 2D context).  `src` holds nin real components
 // starting at in_first with step in_step (step 0: one component transformed
 // nin times); `dst` receives nout components at out_first/out_step.
 // zarg / reg >= 0 replace the regularisation filter's argument scale and
 // order (the Gaussian of smooth3d is order 2 with zarg = h^2 / 2).
+// `flags` (PipeFlags) start or end the pipeline at the held x spectra.
 template <typename T>
 int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin, int op, int nc, void* dst,
                  int out_first, int out_step, int nout, hipEvent_t* ev, int kmap = KMAP_3D, double zarg = -1.0,
-                 int reg = -1)
+                 int reg = -1, int flags = 0)
 {
     hipStream_t s = c->stream;
     const int nx = c->ncell[0], ny = c->ncell[1], nz = c->ncell[2];
+    const size_t cbytes = 2 * sizeof(T);
+    if (flags & (PF_SRC_SPEC | PF_DST_SPEC | PF_X_ONLY | PF_XINV_ONLY)) {
+        int r = ensure_spec(c);
+        if (r) return r;
+    }
+    char* spec = static_cast<char*>(c->spec);
+    if (flags & (PF_X_ONLY | PF_XINV_ONLY)) {
+        // one strided pass between real rows and the held x spectra
+        XArgs<T> xa{};
+        xa.plan = device_plan<T>(c->px.p, reinterpret_cast<const cplx<T>*>(c->px.tw));
+        xa.r_row = nx;
+        xa.c_row = c->Hp;
+        xa.nrows = ny * nz;
+        xa.nvalid = nx;
+        xa.H = c->H;
+        xa.P = c->px.tile;
+        xa.npairs = (ny * nz + 1) / 2;
+        if (flags & PF_X_ONLY) {
+            xa.in = reinterpret_cast<const T*>(src) + (long)in_first * c->n3;
+            xa.r_comp = (long)in_step * c->n3;
+            xa.out = spec + (size_t)out_first * c->A1c * cbytes;
+            xa.c_comp = (long)out_step * c->A1c;
+            xa.ncomp = nin;
+            if (c->px.stat) CK(static_launch_x<T>(c->px.p.L, xa, 0, c->bc[0] == 0, s));
+            else CK(launch_x_r2c<T>(c->px.p.E, xa, s));
+        } else {
+            xa.in = spec + (size_t)in_first * c->A1c * cbytes;
+            xa.c_comp = (long)in_step * c->A1c;
+            xa.out = reinterpret_cast<T*>(dst) + (long)out_first * c->n3;
+            xa.r_comp = (long)out_step * c->n3;
+            xa.ncomp = nout;
+            if (c->px.stat) CK(static_launch_x<T>(c->px.p.L, xa, 1, c->bc[0] == 0, s));
+            else CK(launch_x_c2r<T>(c->px.p.E, xa, s));
+            // spec holds the unnormalised x spectra: the inverse divides by Mx
+            for (int k = 0; k < nout; ++k) {
+                T* p = reinterpret_cast<T*>(dst) + (long)(out_first + k * out_step) * c->n3;
+                CK(launch_scale(p, p, sizeof(T), c->n3, 1.0 / c->M[0], s));
+            }
+        }
+        return PS_OK;
+    }
     int r = ensure_comp_buffers(c, std::max(std::max(nin, nc), nout));
     if (r) return r;
     if (ev) CK(hipEventRecord(ev[0], s));
+    if ((flags & PF_SRC_SPEC) && c->dim2) {
+        // 2D: the fused pass reads the x pass output A1 (in place); start it
+        // from copies of the held spectra
+        for (int k = 0; k < nin; ++k)
+            CK(hipMemcpyAsync(static_cast<char*>(c->A1) + (size_t)k * c->A1c * cbytes,
+                              spec + (size_t)(in_first + k * in_step) * c->A1c * cbytes, c->A1c * cbytes,
+                              hipMemcpyDeviceToDevice, s));
+    }
 
     XArgs<T> xa{};
     xa.plan = device_plan<T>(c->px.p, reinterpret_cast<const cplx<T>*>(c->px.tw));
@@ -402,8 +512,10 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     xa.P = c->px.tile;
     xa.npairs = (ny * nz + 1) / 2;
     xa.ncomp = nin;
-    if (c->px.stat) CK(static_launch_x<T>(c->px.p.L, xa, 0, c->bc[0] == 0, s));
-    else CK(launch_x_r2c<T>(c->px.p.E, xa, s));
+    if (!(flags & PF_SRC_SPEC)) {
+        if (c->px.stat) CK(static_launch_x<T>(c->px.p.L, xa, 0, c->bc[0] == 0, s));
+        else CK(launch_x_r2c<T>(c->px.p.E, xa, s));
+    }
     if (ev) CK(hipEventRecord(ev[1], s));
 
     PencilArgs<T> ya{};
@@ -412,6 +524,10 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     ya.in = c->A1;
     ya.out = c->A2;
     ya.ia = addr_plain(c->A1c, (long)ny * c->Hp, c->Hp, 1);
+    if (flags & PF_SRC_SPEC) {
+        ya.in = spec + (size_t)in_first * c->A1c * cbytes;
+        ya.ia.comp = (long)in_step * c->A1c;
+    }
     ya.oa = c->a2_ypass;
     ya.xcd_remap = c->tune.xcd;
     ya.twreg = c->tune.twreg_y;
@@ -480,6 +596,12 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     yi.out = c->A1;
     yi.ia = c->a2_ypass;
     yi.oa = addr_plain(c->A1c, (long)ny * c->Hp, c->Hp, 1);
+    if (flags & PF_DST_SPEC) {
+        yi.out = spec + (size_t)out_first * c->A1c * cbytes;
+        yi.oa.comp = (long)out_step * c->A1c;
+    }
+    // (the pipeline's 1/Mx is folded into the kernel multipliers: spec is
+    // rescaled below to the unnormalised x spectra the x pass would give)
     yi.n_in = c->M[1];
     yi.n_out = ny;
     yi.ncomp = nout;
@@ -488,7 +610,20 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     if (c->py.stat) CK(static_launch_pencil<T>(c->py.p.L, yi, 1, SM_INV, c->bc[1] == 0, s));
     else CK(launch_pencil<T>(c->py.p.E, 1, yi, s));
     }
+    if (flags & PF_DST_SPEC) {
+        // y^-1 output (3D) or the fused pass's A1 (2D) times Mx: the
+        // unnormalised x spectra of the held rhs
+        for (int k = 0; k < nout; ++k) {
+            char* o = spec + (size_t)(out_first + k * out_step) * c->A1c * cbytes;
+            const char* i = c->dim2 ? static_cast<char*>(c->A1) + (size_t)k * c->A1c * cbytes : o;
+            CK(launch_scale(i, o, sizeof(T), 2 * c->A1c, (double)c->M[0], s));
+        }
+    }
     if (ev) CK(hipEventRecord(ev[4], s));
+    if (flags & PF_DST_SPEC) {
+        if (ev) CK(hipEventRecord(ev[5], s));
+        return PS_OK;
+    }
 
     XArgs<T> xi = xa;
     xi.in = c->A1;
@@ -654,7 +789,7 @@ template <typename T> int setup_impl(ps_ctx* c)
             h[2 * j] = (float)re[j];
             h[2 * j + 1] = (float)im[j];
         }
-        CK(hipMalloc(&c->tw_half, h.size() * sizeof(float)));
+        CK(dev_alloc(c, &c->tw_half, h.size() * sizeof(float)));
         CK(hipMemcpy(c->tw_half, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
     }
     // 2D, long y pencils (one or two columns per workgroup on the specialised
@@ -671,85 +806,185 @@ template <typename T> int setup_impl(ps_ctx* c)
     if (c->zmfma) {
         std::vector<unsigned short> fr(kZmfmaFragHalves);
         zmfma_fragments(fr.data());
-        CK(hipMalloc(&c->mfrag, fr.size() * sizeof(unsigned short)));
+        CK(dev_alloc(c, &c->mfrag, fr.size() * sizeof(unsigned short)));
         CK(hipMemcpy(c->mfrag, fr.data(), fr.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
     }
-    CK(hipMalloc(&c->G, (size_t)c->O[0] * c->O[1] * c->O[2] * sizeof(T)));
+    CK(dev_alloc(c, &c->G, (size_t)c->O[0] * c->O[1] * c->O[2] * sizeof(T)));
     r = build_greens<T>(c->ncell, c->bc, c->dx, c->reg, reinterpret_cast<T*>(c->G), c->g_layout, c->stream,
                         &c->err, c->dim2 ? 1 : 0);
     if (r) return r;
     return PS_OK;
 }
 
-// Host bookkeeping of a solve: which lhs components it wrote and which
-// filter the rhs returned by pull carries.
-void after_solve(ps_ctx* c, const OpSel& s)
+// Host bookkeeping of a solve: which lhs components it wrote, and the filter
+// it leaves pending on the held rhs: regularisation (solve3d.cpp:329-343) and
+// reprojection when all three components are held (:388-396; none in 2D).
+void mark_lhs(ps_ctx* c, const OpSel& s)
 {
     for (int k = 0; k < s.nout; ++k) c->lhs_present[s.out_first + k * s.out_step] = true;
-    c->solved_after_push = true;
-    c->rhsf_valid = false;
-    const bool all3 = c->rhs_present[0] && c->rhs_present[1] && c->rhs_present[2];
-    c->solve_reproj = !c->dim2 && c->reproj && all3;              // solve3d.cpp:388; none in 2D
-    c->solve_filter_active = (c->reg > 0) || c->solve_reproj;
 }
 
-template <typename T> int solve_impl(ps_ctx* c, hipEvent_t* ev)
+void after_solve(ps_ctx* c, const OpSel& s)
 {
-    OpSel s;
-    int r = select_op(c, &s);
+    mark_lhs(c, s);
+    const bool all3 = c->rhs_present[0] && c->rhs_present[1] && c->rhs_present[2];
+    const bool reproj = !c->dim2 && c->reproj && all3;
+    if (c->reg > 0 || reproj) {
+        c->filt_pending = true;
+        c->filt_reproj = reproj;
+        c->rhsf_valid = false;
+    }
+}
+
+// Apply the pending filter to the held rhs: one pipeline from the held rhs
+// (pushed rows, or x spectra) to x spectra, i.e. the reference's rhs after
+// its solve (the filtered field, z and y truncated, x extended).
+template <typename T> int materialize_impl(ps_ctx* c)
+{
+    if (!c->filt_pending) return PS_OK;
+    int first, step, n;
+    if (!arith(c->rhs_present, &first, &step, &n)) {
+        c->filt_pending = false;
+        return PS_OK;
+    }
+    const int op = c->filt_reproj ? OP_RHSF_PROJ : OP_RHSF;
+    const int nc = c->filt_reproj ? 3 : 1;
+    int r = run_pipeline<T>(c, c->rhs, first, step, n, op, nc, nullptr, first, step, n, nullptr, KMAP_3D, -1.0, -1,
+                            PF_DST_SPEC | (c->src_spec ? PF_SRC_SPEC : 0));
     if (r) return r;
-    if ((r = ensure_real(c, &c->lhs))) return r;
-    r = run_pipeline<T>(c, c->rhs, s.in_first, s.in_step, s.nin, s.op, s.nc, c->lhs, s.out_first, s.out_step,
-                        s.nout, ev, s.kmap);
-    if (r) return r;
-    after_solve(c, s);
+    c->src_spec = true;
+    c->filt_pending = false;
+    c->rhsf_valid = false;
     return PS_OK;
 }
 
-// The filtered (and reprojected) rhs the reference leaves in rhs* after a
-// solve (solve3d.cpp:329-343, :388-396, :403-424), computed when pulled.
-template <typename T> int rhs_filter_impl(ps_ctx* c)
+int materialize(ps_ctx* c)
+{
+    if (!c->filt_pending) return PS_OK;
+    return (c->prec == PS_FP64) ? materialize_impl<double>(c) : materialize_impl<float>(c);
+}
+
+// A solve of the held rhs.  mutate: the solve's filter becomes pending on the
+// held rhs (a reference solve3d / solve2d); otherwise the held rhs is left as
+// it was (timed repeats of one solve).
+template <typename T> int solve_impl(ps_ctx* c, hipEvent_t* ev, bool mutate = true)
+{
+    int r = materialize(c);
+    if (r) return r;
+    OpSel s;
+    if ((r = select_op(c, &s))) return r;
+    if ((r = ensure_real(c, &c->lhs))) return r;
+    r = run_pipeline<T>(c, c->rhs, s.in_first, s.in_step, s.nin, s.op, s.nc, c->lhs, s.out_first, s.out_step,
+                        s.nout, ev, s.kmap, -1.0, -1, c->src_spec ? PF_SRC_SPEC : 0);
+    if (r) return r;
+    if (mutate) after_solve(c, s);
+    else mark_lhs(c, s);
+    return PS_OK;
+}
+
+// Real rows (x < nx) of the held x spectra, into rhsf, for pull.
+template <typename T> int spec_rows_impl(ps_ctx* c)
 {
     int first, step, n;
     if (!arith(c->rhs_present, &first, &step, &n)) return PS_OK;
     int r;
     if ((r = ensure_real(c, &c->rhsf))) return r;
-    const int op = c->solve_reproj ? OP_RHSF_PROJ : OP_RHSF;
-    const int nc = c->solve_reproj ? 3 : 1;
-    r = run_pipeline<T>(c, c->rhs, first, step, n, op, nc, c->rhsf, first, step, n, nullptr);
+    r = run_pipeline<T>(c, nullptr, first, step, n, OP_NONE, 1, c->rhsf, first, step, n, nullptr, KMAP_3D, -1.0, -1,
+                        PF_XINV_ONLY);
     if (r) return r;
-    CK(hipStreamSynchronize(c->stream));
     c->rhsf_valid = true;
     return PS_OK;
 }
 
-template <typename T> int push_host(ps_ctx* c, const double* const b[3])
+// Staged host transfers: FP64 host data crosses PCIe as is and is converted
+// on the device (chunks of kStageReals through one device buffer).
+constexpr long kStageReals = 1L << 23;   // 64 MB of doubles
+
+int ensure_stage(ps_ctx* c)
 {
-    std::vector<T> tmp;
-    for (int d = 0; d < 3; ++d) {
-        if (!b[d]) continue;
-        T* dst = reinterpret_cast<T*>(c->rhs) + (long)d * c->n3;
-        if (sizeof(T) == sizeof(double)) {
-            CK(hipMemcpy(dst, b[d], c->n3 * sizeof(double), hipMemcpyHostToDevice));
-        } else {
-            tmp.resize(c->n3);
-            for (long i = 0; i < c->n3; ++i) tmp[i] = (T)b[d][i];
-            CK(hipMemcpy(dst, tmp.data(), c->n3 * sizeof(T), hipMemcpyHostToDevice));
-        }
+    if (c->stage) return PS_OK;
+    CK(dev_alloc(c, &c->stage, (size_t)std::min(kStageReals, c->n3) * sizeof(double)));
+    return PS_OK;
+}
+
+// device component d of `dst_base` (solve precision) <- host FP64 array
+int push_host(ps_ctx* c, void* dst_base, int d, const double* src)
+{
+    char* dst = static_cast<char*>(dst_base) + (size_t)d * c->n3 * real_bytes(c);
+    if (real_bytes(c) == 8) {
+        CK(hipMemcpyAsync(dst, src, c->n3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        return PS_OK;
+    }
+    int r = ensure_stage(c);
+    if (r) return r;
+    for (long i0 = 0; i0 < c->n3; i0 += kStageReals) {
+        const long m = std::min(kStageReals, c->n3 - i0);
+        CK(hipMemcpyAsync(c->stage, src + i0, m * sizeof(double), hipMemcpyHostToDevice, c->stream));
+        CK(launch_cast(c->stage, 8, dst + (size_t)i0 * 4, 4, m, c->stream));
     }
     return PS_OK;
 }
 
-template <typename T> int pull_host(ps_ctx* c, double* dst, const void* src_base, int d)
+// host FP64 array <- device component d of `src_base`; waits for the copy
+int pull_host(ps_ctx* c, double* dst, const void* src_base, int d)
 {
-    const T* src = reinterpret_cast<const T*>(src_base) + (long)d * c->n3;
-    if (sizeof(T) == sizeof(double)) {
-        CK(hipMemcpy(dst, src, c->n3 * sizeof(double), hipMemcpyDeviceToHost));
-    } else {
-        std::vector<T> tmp(c->n3);
-        CK(hipMemcpy(tmp.data(), src, c->n3 * sizeof(T), hipMemcpyDeviceToHost));
-        for (long i = 0; i < c->n3; ++i) dst[i] = (double)tmp[i];
+    const char* src = static_cast<const char*>(src_base) + (size_t)d * c->n3 * real_bytes(c);
+    if (real_bytes(c) == 8) {
+        CK(hipMemcpyAsync(dst, src, c->n3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        CK(hipStreamSynchronize(c->stream));
+        return PS_OK;
     }
+    int r = ensure_stage(c);
+    if (r) return r;
+    for (long i0 = 0; i0 < c->n3; i0 += kStageReals) {
+        const long m = std::min(kStageReals, c->n3 - i0);
+        CK(launch_cast(src + (size_t)i0 * 4, 4, c->stage, 8, m, c->stream));
+        CK(hipMemcpyAsync(dst + i0, c->stage, m * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    }
+    CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+// A push of the components in `mask`.  When a held component that is not
+// pushed again carries a filtered state (a filtered solve preceded), it keeps
+// it: the pending filter is applied before the new rows overwrite the pushed
+// buffer (reprojection mixes components), and after the copy the pushed
+// components join the held x spectra (their x pass runs then).  Otherwise the
+// held rhs is simply the pushed rows.
+bool push_keeps_spec(const ps_ctx* c, int mask)
+{
+    bool other = false;
+    for (int d = 0; d < 3; ++d)
+        if (c->rhs_present[d] && !((mask >> d) & 1)) other = true;
+    return other && (c->src_spec || c->filt_pending);
+}
+
+int before_push(ps_ctx* c, int mask)
+{
+    return push_keeps_spec(c, mask) ? materialize(c) : PS_OK;
+}
+
+int after_push(ps_ctx* c, int mask)
+{
+    int r;
+    if (push_keeps_spec(c, mask)) {
+        bool m3[3] = {(mask & 1) != 0, (mask & 2) != 0, (mask & 4) != 0};
+        int first, step, n;
+        if (arith(m3, &first, &step, &n)) {
+            r = (c->prec == PS_FP64)
+                    ? run_pipeline<double>(c, c->rhs, first, step, n, OP_NONE, 1, nullptr, first, step, n, nullptr,
+                                           KMAP_3D, -1.0, -1, PF_X_ONLY)
+                    : run_pipeline<float>(c, c->rhs, first, step, n, OP_NONE, 1, nullptr, first, step, n, nullptr,
+                                          KMAP_3D, -1.0, -1, PF_X_ONLY);
+            if (r) return r;
+        }
+    } else {
+        c->src_spec = false;
+        c->filt_pending = false;
+    }
+    for (int d = 0; d < 3; ++d)
+        if ((mask >> d) & 1) c->rhs_present[d] = true;
+    c->rhsf_valid = false;
     return PS_OK;
 }
 
@@ -777,7 +1012,7 @@ int setup_common(ps_ctx* c, const int ncell[3], const int bc[3], const double dx
     }
     c->dim2 = dim2;
     c->reg = reg_order;
-    c->solved_after_push = c->rhsf_valid = false;
+    ++c->gen;
     int r = (c->prec == PS_FP64) ? setup_impl<double>(c) : setup_impl<float>(c);
     if (r) {
         release(c);
@@ -787,14 +1022,17 @@ int setup_common(ps_ctx* c, const int ncell[3], const int bc[3], const double dx
     return PS_OK;
 }
 
-// Where rhs component d is read from by pull: the pushed field unless a solve
-// since the last push applied a filter (regularisation or reprojection).
+// Where pull reads the held rhs (components as pushed, x < nx): the pushed
+// rows, or the real rows of the held x spectra after a filtered solve
+// (pull.cpp:101-131 takes the real part of the x pencils).
 int rhs_source(ps_ctx* c, const void** src)
 {
+    int r = materialize(c);
+    if (r) return r;
     *src = c->rhs;
-    if (!c->solved_after_push || !c->solve_filter_active) return PS_OK;
+    if (!c->src_spec) return PS_OK;
     if (!c->rhsf_valid) {
-        int r = (c->prec == PS_FP64) ? rhs_filter_impl<double>(c) : rhs_filter_impl<float>(c);
+        r = (c->prec == PS_FP64) ? spec_rows_impl<double>(c) : spec_rows_impl<float>(c);
         if (r) return r;
     }
     *src = c->rhsf;
@@ -822,12 +1060,11 @@ bool strided_args(const ps_ctx* c, const long stride[4], const long first[3], in
     return true;
 }
 
-struct DevScratch {
+// a temporary device buffer of a context, freed at scope exit
+struct CtxScratch {
+    ps_ctx* c;
     void* p = nullptr;
-    ~DevScratch()
-    {
-        if (p) (void)hipFree(p);
-    }
+    ~CtxScratch() { free_ptr(c, p); }
 };
 
 }  // namespace
@@ -916,12 +1153,14 @@ int ps_push(ps_ctx* c, const double* bx, const double* by, const double* bz)
     if (r) return r;
     if ((r = ensure_real(c, &c->rhs))) return r;
     const double* b[3] = {bx, by, bz};
-    r = (c->prec == PS_FP64) ? push_host<double>(c, b) : push_host<float>(c, b);
-    if (r) return r;
+    int mask = 0;
     for (int d = 0; d < 3; ++d)
-        if (b[d]) c->rhs_present[d] = true;
-    c->solved_after_push = false;
-    c->rhsf_valid = false;
+        if (b[d]) mask |= 1 << d;
+    if ((r = before_push(c, mask))) return r;
+    for (int d = 0; d < 3; ++d)
+        if (b[d] && (r = push_host(c, c->rhs, d, b[d]))) return r;
+    if ((r = after_push(c, mask))) return r;
+    CK(hipStreamSynchronize(c->stream));
     return PS_OK;
 }
 
@@ -932,14 +1171,16 @@ int ps_push_device(ps_ctx* c, const void* bx, const void* by, const void* bz)
     if ((r = ensure_real(c, &c->rhs))) return r;
     const void* b[3] = {bx, by, bz};
     const size_t bytes = c->n3 * real_bytes(c);
-    for (int d = 0; d < 3; ++d) {
-        if (!b[d]) continue;
-        CK(hipMemcpyAsync(static_cast<char*>(c->rhs) + d * bytes, b[d], bytes, hipMemcpyDeviceToDevice, c->stream));
-        c->rhs_present[d] = true;
-    }
+    int mask = 0;
+    for (int d = 0; d < 3; ++d)
+        if (b[d]) mask |= 1 << d;
+    if ((r = before_push(c, mask))) return r;
+    for (int d = 0; d < 3; ++d)
+        if (b[d])
+            CK(hipMemcpyAsync(static_cast<char*>(c->rhs) + d * bytes, b[d], bytes, hipMemcpyDeviceToDevice,
+                              c->stream));
+    if ((r = after_push(c, mask))) return r;
     CK(hipStreamSynchronize(c->stream));
-    c->solved_after_push = false;
-    c->rhsf_valid = false;
     return PS_OK;
 }
 
@@ -953,15 +1194,14 @@ int ps_push_strided(ps_ctx* c, const double* base, int ncomp, const long stride[
     if (!strided_args(c, stride, first, ncomp, &st, &off, &span))
         return fail(c, PS_EINVAL, "push_strided: ncomp must be 1..3, strides positive and offsets non-negative");
     if ((r = ensure_real(c, &c->rhs))) return r;
-    DevScratch stage;
-    CK(hipMalloc(&stage.p, (size_t)span * sizeof(double)));
+    if ((r = before_push(c, (1 << ncomp) - 1))) return r;
+    CtxScratch stage{c};
+    CK(dev_alloc(c, &stage.p, (size_t)span * sizeof(double)));
     CK(hipMemcpyAsync(stage.p, base, (size_t)span * sizeof(double), hipMemcpyHostToDevice, c->stream));
     CK(launch_gather(static_cast<const double*>(stage.p) + off, st, c->rhs, real_bytes(c), ncomp, c->ncell[0],
                      c->ncell[1], c->ncell[2], c->stream));
+    if ((r = after_push(c, (1 << ncomp) - 1))) return r;
     CK(hipStreamSynchronize(c->stream));
-    for (int d = 0; d < ncomp; ++d) c->rhs_present[d] = true;
-    c->solved_after_push = false;
-    c->rhsf_valid = false;
     return PS_OK;
 }
 
@@ -988,8 +1228,8 @@ int ps_pull_strided(ps_ctx* c, double* lhs_base, int lhs_ncomp, const long lhs_s
         const void* src = c->lhs;
         if (which && (r = rhs_source(c, &src))) return r;
         // the block's ghost cells and unheld components keep their values
-        DevScratch stage;
-        CK(hipMalloc(&stage.p, (size_t)span * sizeof(double)));
+        CtxScratch stage{c};
+        CK(dev_alloc(c, &stage.p, (size_t)span * sizeof(double)));
         CK(hipMemcpyAsync(stage.p, base, (size_t)span * sizeof(double), hipMemcpyHostToDevice, c->stream));
         CK(launch_scatter(src, real_bytes(c), static_cast<double*>(stage.p) + off, st, mask, ncomp, c->ncell[0],
                           c->ncell[1], c->ncell[2], c->stream));
@@ -1007,16 +1247,20 @@ int ps_smooth3d(ps_ctx* c, double h)
     if (!(h >= 0.0) || !std::isfinite(h)) return fail(c, PS_EINVAL, "smooth3d: h must be finite and >= 0");
     int first, step, n;
     if (!c->rhs || !arith(c->rhs_present, &first, &step, &n)) return PS_OK;   // nothing held: no-op
-    // exp(-h^2 |k|^2 / 2) / N is the order-2 filter with argument scale h^2 / 2
+    if ((r = materialize(c))) return r;
+    // exp(-h^2 |k|^2 / 2) / N is the order-2 filter with argument scale h^2 / 2;
+    // the smoothed field stays x-extended, as the Fortran routine leaves it
+    // in its x pencils (poisson_solver_smooth3d.f:333-386)
     const double zarg = 0.5 * h * h;
+    const int flags = PF_DST_SPEC | (c->src_spec ? PF_SRC_SPEC : 0);
     r = (c->prec == PS_FP64)
-            ? run_pipeline<double>(c, c->rhs, first, step, n, OP_RHSF, 1, c->rhs, first, step, n, nullptr,
-                                   KMAP_3D, zarg, 2)
-            : run_pipeline<float>(c, c->rhs, first, step, n, OP_RHSF, 1, c->rhs, first, step, n, nullptr, KMAP_3D,
-                                  zarg, 2);
+            ? run_pipeline<double>(c, c->rhs, first, step, n, OP_RHSF, 1, nullptr, first, step, n, nullptr,
+                                   KMAP_3D, zarg, 2, flags)
+            : run_pipeline<float>(c, c->rhs, first, step, n, OP_RHSF, 1, nullptr, first, step, n, nullptr, KMAP_3D,
+                                  zarg, 2, flags);
     if (r) return r;
     CK(hipStreamSynchronize(c->stream));
-    c->solved_after_push = false;      // the held rhs is the smoothed field
+    c->src_spec = true;
     c->rhsf_valid = false;
     return PS_OK;
 }
@@ -1047,6 +1291,20 @@ int ps_zpass_variant(const ps_ctx* c)
     if (c->zmfma) return 3;
     if (c->zwave) return 2;
     return c->pz1.stat ? 1 : 0;
+}
+
+int ps_set_device_memory_limit(ps_ctx* c, unsigned long long bytes)
+{
+    if (!c) return PS_EINVAL;
+    c->mem_cap = bytes ? (size_t)bytes : ~(size_t)0;
+    return PS_OK;
+}
+
+int ps_device_memory_used(const ps_ctx* c, unsigned long long* bytes)
+{
+    if (!c || !bytes) return PS_EINVAL;
+    *bytes = c->mem_used;
+    return PS_OK;
 }
 
 int ps_solve3d_async(ps_ctx* c)
@@ -1087,10 +1345,10 @@ int ps_pull(ps_ctx* c, double* ax, double* ay, double* az, double* bx, double* b
     if (r) return r;
     double* a[3] = {ax, ay, az};
     double* b[3] = {bx, by, bz};
+    // copies run on the context's stream, after any solve enqueued on it
     for (int d = 0; d < 3; ++d) {
         if (!a[d] || !c->lhs_present[d]) continue;
-        r = (c->prec == PS_FP64) ? pull_host<double>(c, a[d], c->lhs, d) : pull_host<float>(c, a[d], c->lhs, d);
-        if (r) return r;
+        if ((r = pull_host(c, a[d], c->lhs, d))) return r;
     }
     bool want = false;
     for (int d = 0; d < 3; ++d) want = want || (b[d] && c->rhs_present[d]);
@@ -1099,8 +1357,7 @@ int ps_pull(ps_ctx* c, double* ax, double* ay, double* az, double* bx, double* b
     if ((r = rhs_source(c, &src))) return r;
     for (int d = 0; d < 3; ++d) {
         if (!b[d] || !c->rhs_present[d]) continue;
-        r = (c->prec == PS_FP64) ? pull_host<double>(c, b[d], src, d) : pull_host<float>(c, b[d], src, d);
-        if (r) return r;
+        if ((r = pull_host(c, b[d], src, d))) return r;
     }
     return PS_OK;
 }
@@ -1187,9 +1444,12 @@ int ps_profile_solve(ps_ctx* c, int iters, double ms[6])
         for (hipEvent_t& e : c->ev) CK(hipEventCreate(&e));
         c->ev_made = true;
     }
+    // every timed solve solves the held rhs as it stands (the filter a
+    // reference solve leaves on it is not applied between them)
+    if ((r = materialize(c))) return r;
     double acc[6] = {0, 0, 0, 0, 0, 0};
     for (int it = 0; it < iters; ++it) {
-        r = (c->prec == PS_FP64) ? solve_impl<double>(c, c->ev) : solve_impl<float>(c, c->ev);
+        r = (c->prec == PS_FP64) ? solve_impl<double>(c, c->ev, false) : solve_impl<float>(c, c->ev, false);
         if (r) return r;
         CK(hipEventSynchronize(c->ev[5]));
         for (int k = 0; k < 5; ++k) {
@@ -1259,16 +1519,46 @@ int ctx_adopt_real(ps_ctx* c, void* rhs, void* lhs)
     if (r) return r;
     if (!rhs || !lhs) return fail(c, PS_EINVAL, "adopt: NULL block");
     if (!c->ext_rl) {
-        free_ptr(c->rhs);
-        free_ptr(c->lhs);
+        free_ptr(c, c->rhs);
+        free_ptr(c, c->lhs);
     }
     c->rhs = rhs;
     c->lhs = lhs;
     c->ext_rl = true;
     for (int d = 0; d < 3; ++d) c->rhs_present[d] = c->lhs_present[d] = false;
-    c->solved_after_push = false;
-    c->rhsf_valid = false;
+    c->src_spec = c->filt_pending = c->rhsf_valid = false;
     return PS_OK;
+}
+
+// A batch solve solves every member's pushed rows: the filtered rhs a
+// previous solve left held is dropped.
+void ctx_reset_held(ps_ctx* c)
+{
+    if (!c) return;
+    c->src_spec = c->filt_pending = c->rhsf_valid = false;
+}
+
+// What the launches a batch captured for this context depend on: its setup
+// generation, pushed components and options; 0 if it cannot be solved.
+unsigned long ctx_signature(const ps_ctx* c)
+{
+    if (!c || !c->ready || !c->rhs) return 0;
+    unsigned long sig = (unsigned long)c->gen << 16;
+    sig |= (unsigned long)ctx_rhs_mask(c) << 8;
+    sig |= (c->grad ? 1u : 0u) | (c->div ? 2u : 0u) | (c->curl ? 4u : 0u) | (c->reproj ? 8u : 0u) |
+           (c->ext_rl ? 16u : 0u) | 32u;
+    return sig;
+}
+
+// Checks the operator's inputs as a solve would, without launching.
+int ctx_check_solvable(ps_ctx* c)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->dim2) return fail(c, PS_ESTATE, "solve3d: the solver was set up by setup2d");
+    if (!c->rhs) return fail(c, PS_ESTATE, "[poisson_solver.solve3d]: no rhs field has been pushed");
+    OpSel s;
+    return select_op(c, &s);
 }
 
 // One solve over K contexts of one grid whose rhs / lhs are consecutive

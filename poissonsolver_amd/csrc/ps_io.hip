@@ -73,6 +73,53 @@ hipError_t launch_scatter(const void* src, int elem_bytes, double* dst, Strided 
     return hipGetLastError();
 }
 
+namespace {
+// four elements per thread: 32-byte double and 16-byte float accesses
+template <typename S, typename D> __global__ void __launch_bounds__(256) k_cast(const S* src, D* dst, long n)
+{
+    const long stride = 4L * gridDim.x * blockDim.x;
+    for (long i = 4L * (blockIdx.x * (long)blockDim.x + threadIdx.x); i < n; i += stride) {
+        if (i + 4 <= n) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) dst[i + e] = (D)src[i + e];
+        } else {
+            for (long e = i; e < n; ++e) dst[e] = (D)src[e];
+        }
+    }
+}
+}  // namespace
+
+hipError_t launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, long n, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const unsigned g = grid_for((n + 3) / 4);
+    if (src_bytes == 8 && dst_bytes == 4)
+        hipLaunchKernelGGL((k_cast<double, float>), dim3(g), dim3(256), 0, s, (const double*)src, (float*)dst, n);
+    else if (src_bytes == 4 && dst_bytes == 8)
+        hipLaunchKernelGGL((k_cast<float, double>), dim3(g), dim3(256), 0, s, (const float*)src, (double*)dst, n);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+namespace {
+template <typename T> __global__ void __launch_bounds__(256) k_scale(const T* src, T* dst, long n, T f)
+{
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        dst[i] = f * src[i];
+}
+}  // namespace
+
+hipError_t launch_scale(const void* src, void* dst, int elem_bytes, long n, double f, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    const unsigned g = grid_for(n);
+    if (elem_bytes == 8)
+        hipLaunchKernelGGL(k_scale<double>, dim3(g), dim3(256), 0, s, (const double*)src, (double*)dst, n, f);
+    else
+        hipLaunchKernelGGL(k_scale<float>, dim3(g), dim3(256), 0, s, (const float*)src, (float*)dst, n, (float)f);
+    return hipGetLastError();
+}
 
 namespace {
 template <typename V>

@@ -20,6 +20,14 @@ hipError_t launch_gather(const double* src, Strided st, void* dst, int elem_byte
 hipError_t launch_scatter(const void* src, int elem_bytes, double* dst, Strided st, int cmask, int ncomp, int nx,
                           int ny, int nz, hipStream_t s);
 
+// dst[i] = src[i], i < n, converting between double and float (elem bytes
+// 8 or 4 each): the host-pointer push / pull stage FP64 data on the device
+// and convert it there
+hipError_t launch_cast(const void* src, int src_bytes, void* dst, int dst_bytes, long n, hipStream_t s);
+
+// dst[i] = f * src[i], i < n (elem bytes 8 or 4; src == dst allowed)
+hipError_t launch_scale(const void* src, void* dst, int elem_bytes, long n, double f, hipStream_t s);
+
 // out[c][j][i] = in[c][i][j] for i < rows, j < cols: complex elements of
 // elem_bytes (8 or 16), row strides in_ld / out_ld, component strides
 // in_comp / out_comp (all in elements)

@@ -197,6 +197,19 @@ class poisson_solver:
     def stream(self) -> int:
         return self._lib.ps_stream(self._ctx) or 0
 
+    def solve3d_async(self):
+        """Enqueue solve3d on the context's stream and return."""
+        self._report(self._lib.ps_solve3d_async(self._ctx), "solve3d_async")
+
+    def set_device_memory_limit(self, nbytes: int):
+        """Device bytes this context may hold (0: no limit)."""
+        return self._report(self._lib.ps_set_device_memory_limit(self._ctx, int(nbytes)), "set_device_memory_limit")
+
+    def device_memory_used(self) -> int:
+        v = ctypes.c_ulonglong()
+        self._report(self._lib.ps_device_memory_used(self._ctx, ctypes.byref(v)), "device_memory_used")
+        return int(v.value)
+
     def synchronize(self):
         self._report(self._lib.ps_synchronize(self._ctx), "synchronize")
 

@@ -68,3 +68,47 @@ def gpu_solve2d(n, bc, dx, reg, rhs, op="plain", prec="fp64", want_rhs=False, so
     if solver is None:
         s.close()
     return lhs, rb
+
+
+class DeviceBuffer:
+    """Device memory from the HIP runtime the library itself links
+    (hipMalloc / hipMemcpy through ctypes), for device-pointer calls."""
+    _hip = None
+
+    @classmethod
+    def hip(cls):
+        import ctypes
+        if cls._hip is None:
+            h = ctypes.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            h.hipFree.argtypes = [ctypes.c_void_p]
+            h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes):
+        import ctypes
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        assert self.hip().hipMalloc(ctypes.byref(self.ptr), self.nbytes) == 0
+
+    def upload(self, arr):
+        a = np.ascontiguousarray(arr)
+        assert a.nbytes == self.nbytes
+        assert self.hip().hipMemcpy(self.ptr, a.ctypes.data, self.nbytes, 1) == 0   # host to device
+
+    def download(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes == self.nbytes
+        assert self.hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2) == 0  # device to host
+        return out
+
+    @property
+    def address(self):
+        return self.ptr.value
+
+    def free(self):
+        if self.ptr.value:
+            self.hip().hipFree(self.ptr)
+            self.ptr.value = None
