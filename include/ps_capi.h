@@ -152,6 +152,19 @@ int ps_eval_special(const double* x, double* y, long n, int which);
 int ps_set_device_memory_limit(ps_ctx* ctx, unsigned long long bytes);
 /* device bytes the context holds now */
 int ps_device_memory_used(const ps_ctx* ctx, unsigned long long* bytes);
+/* no counterpart in the reference: for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  Enqueues on the context's stream the solve of the
+   context's operator on `src` (3 x n3 reals of the solve precision, component
+   c at src + c n3, those in src_mask bits 0..2 present) into `dst` (same
+   shape; only the components the operator writes are written), and returns
+   without waiting.  Calls no allocator: the work space is a static device
+   array of 256 MiB (per device, held until the process ends); a grid and
+   operator needing more return PS_EUNSUPPORTED before launching.  The
+   context must be set up (setup allocates) and its device current.  The held
+   rhs / lhs of the context are not touched.  Calls that overlap on one device
+   (any contexts, any streams) share the array and race. */
+int ps_solve3d_noalloc(ps_ctx* ctx, const void* src, int src_mask, void* dst);
 /* enqueue ps_solve3d on the context's stream and return without waiting;
    push, pull and solve calls that follow run after it (same stream) */
 int ps_solve3d_async(ps_ctx* ctx);

@@ -62,6 +62,7 @@ SIGNATURES = {
                                         _c_int_p, _c_int_p]),
     "ps_eval_special": (ctypes.c_int, [_c_dbl_p, _c_dbl_p, ctypes.c_long, ctypes.c_int]),
     "ps_solve3d_async": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_solve3d_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
     "ps_set_device_memory_limit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong]),
     "ps_device_memory_used": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]),
     "ps_solve2d_async": (ctypes.c_int, [ctypes.c_void_p]),

@@ -20,6 +20,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -387,10 +388,10 @@ int select_op(ps_ctx* c, OpSel* s)
 }
 
 template <typename T>
-void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
+void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d, void* A1, void* A2)
 {
     a.plan = device_plan<T>(d.p, reinterpret_cast<const cplx<T>*>(d.tw));
-    a.in = a.out = c->dim2 ? c->A1 : c->A2;           // 2D: the x pass output, in place
+    a.in = a.out = c->dim2 ? A1 : A2;                 // 2D: the x pass output, in place
     a.ia = a.oa = c->a2_zpass;                        // slab = ky, pos = kz, col = kx
     a.xcd_remap = c->tune.xcd;
     a.slab_pair = 1;
@@ -424,6 +425,13 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d)
     a.fzero = 0.0f;
 }
 
+// Work buffers of a pipeline supplied by the caller instead of the
+// context's own (A1, A2 of the shapes the context's setup chose).
+struct WorkBufs {
+    void* A1;
+    void* A2;
+};
+
 // Where a pipeline starts and ends (run_pipeline flags).
 enum PipeFlags : int {
     PF_SRC_SPEC = 1,   // no x pass: the y pass reads the held x spectra (spec), components as for src
@@ -441,7 +449,7 @@ enum PipeFlags : int {
 template <typename T>
 int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin, int op, int nc, void* dst,
                  int out_first, int out_step, int nout, hipEvent_t* ev, int kmap = KMAP_3D, double zarg = -1.0,
-                 int reg = -1, int flags = 0)
+                 int reg = -1, int flags = 0, const WorkBufs* wb = nullptr)
 {
     hipStream_t s = c->stream;
     const int nx = c->ncell[0], ny = c->ncell[1], nz = c->ncell[2];
@@ -486,14 +494,20 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         }
         return PS_OK;
     }
-    int r = ensure_comp_buffers(c, std::max(std::max(nin, nc), nout));
-    if (r) return r;
+    void* A1 = wb ? wb->A1 : nullptr;
+    void* A2 = wb ? wb->A2 : nullptr;
+    if (!wb) {
+        int r = ensure_comp_buffers(c, std::max(std::max(nin, nc), nout));
+        if (r) return r;
+        A1 = c->A1;
+        A2 = c->A2;
+    }
     if (ev) CK(hipEventRecord(ev[0], s));
     if ((flags & PF_SRC_SPEC) && c->dim2) {
         // 2D: the fused pass reads the x pass output A1 (in place); start it
         // from copies of the held spectra
         for (int k = 0; k < nin; ++k)
-            CK(hipMemcpyAsync(static_cast<char*>(c->A1) + (size_t)k * c->A1c * cbytes,
+            CK(hipMemcpyAsync(static_cast<char*>(A1) + (size_t)k * c->A1c * cbytes,
                               spec + (size_t)(in_first + k * in_step) * c->A1c * cbytes, c->A1c * cbytes,
                               hipMemcpyDeviceToDevice, s));
     }
@@ -501,7 +515,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     XArgs<T> xa{};
     xa.plan = device_plan<T>(c->px.p, reinterpret_cast<const cplx<T>*>(c->px.tw));
     xa.in = reinterpret_cast<const T*>(src) + (long)in_first * c->n3;
-    xa.out = c->A1;
+    xa.out = A1;
     xa.r_comp = (long)in_step * c->n3;
     xa.r_row = nx;
     xa.c_comp = c->A1c;
@@ -521,8 +535,8 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     PencilArgs<T> ya{};
     if (!c->dim2) {
     ya.plan = device_plan<T>(c->py.p, reinterpret_cast<const cplx<T>*>(c->py.tw));
-    ya.in = c->A1;
-    ya.out = c->A2;
+    ya.in = A1;
+    ya.out = A2;
     ya.ia = addr_plain(c->A1c, (long)ny * c->Hp, c->Hp, 1);
     if (flags & PF_SRC_SPEC) {
         ya.in = spec + (size_t)in_first * c->A1c * cbytes;
@@ -550,7 +564,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
 
     const DevPlan& zp = (nc == 3) ? c->pz3 : c->pz1;
     PencilArgs<T> za{};
-    fill_zargs<T>(c, za, zp);
+    fill_zargs<T>(c, za, zp, A1, A2);
     za.op = op;
     za.kmap = kmap;
     if (zarg >= 0.0) za.zarg = zarg;
@@ -560,12 +574,12 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         // 2D transposed path: A1 [c][y][Hp] -> A2 [c][kx][nyp], then the fused
         // pass reads contiguous y pencils of A2 and writes A1; one input read by
         // several one-component outputs (grad, curl) has component stride 0
-        CK(launch_transpose_c(c->A1, c->A2, 2 * (int)sizeof(T), nz, c->H, c->Hp, c->nyp, c->A1c, c->A2c, nin, s));
+        CK(launch_transpose_c(A1, A2, 2 * (int)sizeof(T), nz, c->H, c->Hp, c->nyp, c->A1c, c->A2c, nin, s));
         if (ev) CK(hipEventRecord(ev[2], s));   // profiled as the (otherwise empty) y interval
         const bool shared_in = nc == 1 && nin == 1 && nout > 1;
-        za.in = c->A2;
+        za.in = A2;
         za.ia = addr_plain(shared_in ? 0 : c->A2c, 0, 1, c->nyp);
-        za.out = c->A1;
+        za.out = A1;
         if (nc == 1) za.ncomp = nout;
     }
     bool mfma_done = false;
@@ -592,8 +606,8 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
 
     if (!c->dim2) {
     PencilArgs<T> yi = ya;
-    yi.in = c->A2;
-    yi.out = c->A1;
+    yi.in = A2;
+    yi.out = A1;
     yi.ia = c->a2_ypass;
     yi.oa = addr_plain(c->A1c, (long)ny * c->Hp, c->Hp, 1);
     if (flags & PF_DST_SPEC) {
@@ -615,7 +629,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         // unnormalised x spectra of the held rhs
         for (int k = 0; k < nout; ++k) {
             char* o = spec + (size_t)(out_first + k * out_step) * c->A1c * cbytes;
-            const char* i = c->dim2 ? static_cast<char*>(c->A1) + (size_t)k * c->A1c * cbytes : o;
+            const char* i = c->dim2 ? static_cast<char*>(A1) + (size_t)k * c->A1c * cbytes : o;
             CK(launch_scale(i, o, sizeof(T), 2 * c->A1c, (double)c->M[0], s));
         }
     }
@@ -626,7 +640,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     }
 
     XArgs<T> xi = xa;
-    xi.in = c->A1;
+    xi.in = A1;
     xi.out = reinterpret_cast<T*>(dst) + (long)out_first * c->n3;
     xi.r_comp = (long)out_step * c->n3;
     xi.ncomp = nout;
@@ -1305,6 +1319,80 @@ int ps_device_memory_used(const ps_ctx* c, unsigned long long* bytes)
     if (!c || !bytes) return PS_EINVAL;
     *bytes = c->mem_used;
     return PS_OK;
+}
+
+}  // extern "C"
+
+// ---- solve without allocation: work space in a static device array -------
+namespace {
+
+// Work space of ps_solve3d_noalloc: A1 and A2 of one solve.  Each device the
+// library runs on holds its own copy of the array until the process ends.
+constexpr size_t kNoallocScratchBytes = (size_t)256 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};   // device address of `scratch` per device
+
+// The array's device address on device `dev` (current), looked up once per
+// device; host code uses this address, never the symbol itself.
+int scratch_address(ps_ctx* c, int dev, void** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return fail(c, PS_EHIP, "solve3d_noalloc: device index beyond 64");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        CK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(c, PS_EHIP, "solve3d_noalloc: no device address for the work space");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PS_OK;
+}
+
+template <typename T> int solve_noalloc_impl(ps_ctx* c, const void* src, int src_mask, void* dst)
+{
+    bool held[3];
+    for (int d = 0; d < 3; ++d) {
+        held[d] = c->rhs_present[d];
+        c->rhs_present[d] = (src_mask >> d) & 1;
+    }
+    OpSel s;
+    int r = select_op(c, &s);
+    for (int d = 0; d < 3; ++d) c->rhs_present[d] = held[d];
+    if (r) return r;
+    const size_t cb = 2 * sizeof(T);
+    const size_t ncomp = (size_t)std::max(std::max(s.nin, s.nc), s.nout);
+    const size_t a1 = (ncomp * c->A1c * cb + 255) / 256 * 256;
+    const size_t a2 = ncomp * c->A2c * cb;
+    if (a1 + a2 > kNoallocScratchBytes)
+        return fail(c, PS_EUNSUPPORTED,
+                    "solve3d_noalloc: this grid and operator need " + std::to_string(a1 + a2) +
+                        " bytes of work space; the static maximum is " + std::to_string(kNoallocScratchBytes));
+    int dev = 0;
+    CK(hipGetDevice(&dev));
+    if (dev != c->device) return fail(c, PS_ESTATE, "solve3d_noalloc: the context's device is not current");
+    void* base = nullptr;
+    if ((r = scratch_address(c, dev, &base))) return r;
+    const WorkBufs wb{base, static_cast<char*>(base) + a1};
+    return run_pipeline<T>(c, src, s.in_first, s.in_step, s.nin, s.op, s.nc, dst, s.out_first, s.out_step, s.nout,
+                           nullptr, s.kmap, -1.0, -1, 0, &wb);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ps_solve3d_noalloc(ps_ctx* c, const void* src, int src_mask, void* dst)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->dim2) return fail(c, PS_ESTATE, "solve3d_noalloc: the solver was set up by setup2d");
+    if (!src || !dst || src_mask < 1 || src_mask > 7)
+        return fail(c, PS_EINVAL, "solve3d_noalloc: NULL block or component mask outside 1..7");
+    return (c->prec == PS_FP64) ? solve_noalloc_impl<double>(c, src, src_mask, dst)
+                                : solve_noalloc_impl<float>(c, src, src_mask, dst);
 }
 
 int ps_solve3d_async(ps_ctx* c)
