@@ -146,6 +146,26 @@ int ps_profile_solve(ps_ctx* ctx, int iters, double ms[6]);
 int ps_plan_describe(int L, int ncomp, int precision_bytes, int* E, int* tp, int* nstage, int radix[14]);
 /* evaluate the device Si (which = 0) or Ji0 (which = 1) at n host arguments */
 int ps_eval_special(const double* x, double* y, long n, int which);
+/* --- extension: the potential and its gradient from one scalar -----------
+   No reference counterpart: the reference's grad is component-wise
+   (solve3d.cpp:348-353), so a caller wanting A and grad A pushes the scalar
+   three times and solves twice (c++/test/mixed_ffp3d/sources/main.cpp:164-178).
+   ps_solve3d_potential transforms the held scalar rhs X once in x and y; its
+   z pass writes every selected quantity from that one spectrum:
+     PS_POT_A    A = G * B                  (the plain solve of X)
+     PS_POT_DX.. dA/dx, dA/dy, dA/dz         (grad of X, as solve3d's grad)
+     PS_POT_LAP  Laplacian of A, -|k|^2 Ghat  (div of grad A)
+   with the context's regularisation; results stay in the context until the
+   next potential solve.  The held rhs is left as it was.  3D contexts. */
+enum { PS_POT_A = 1, PS_POT_DX = 2, PS_POT_DY = 4, PS_POT_DZ = 8, PS_POT_LAP = 16 };
+int ps_solve3d_potential(ps_ctx* ctx, int outputs);
+/* host arrays (x fastest) of the quantities computed; NULL skips one */
+int ps_pull_potential(ps_ctx* ctx, double* A, double* dAdx, double* dAdy, double* dAdz, double* lapA);
+/* device arrays of the solve precision's real type */
+int ps_pull_potential_device(ps_ctx* ctx, void* A, void* dAdx, void* dAdy, void* dAdz, void* lapA);
+/* `iters` potential solves timed per pass as ps_profile_solve */
+int ps_profile_potential(ps_ctx* ctx, int outputs, int iters, double ms[6]);
+
 /* device memory a context may hold (setup, work buffers, staging): a request
    beyond it fails with PS_ENOMEM, as a device out of memory does; 0 = no
    limit (the default).  Buffers already held are kept. */

@@ -526,6 +526,11 @@ class OracleSolver:
             lk = [G * (iky * r[2] - ikz * r[1]),
                   G * (ikz * r[0] - ikx * r[2]),
                   G * (ikx * r[1] - iky * r[0])]
+        elif getattr(self, "lhs_lap", False):
+            # not a reference option: the Laplacian of A = G * r_X, i.e. div of
+            # the (component-wise) grad of one scalar, (ik . ik) G r
+            lk[0] = (ikx * ikx + iky * iky + ikz * ikz) * G * r[0]
+            lflag = (True, False, False)
         else:                                                      # :369-383
             lk = [G * r[d] if lflag[d] else None for d in range(3)]
         if self.rhs_reproject and rX and rY and rZ:                # :388-396
@@ -563,12 +568,15 @@ class OracleSolver:
 
 def solve(ncell, bc, dx, reg_order, rhs, op="plain", reproject=False, use_glassman=False):
     """One-shot helper: setup, push ``rhs`` (list of up to 3 arrays or None),
-    solve with ``op`` in {plain, grad, div, curl}, return (lhs list, rhs list)."""
+    solve with ``op`` in {plain, grad, div, curl} (or "lap": the Laplacian of
+    the potential of rhs X, an extension with no reference option), return
+    (lhs list, rhs list)."""
     s = OracleSolver(use_glassman)
     s.setup3d(ncell, bc, dx, reg_order)
     s.set_return_grad(op == "grad")
     s.set_return_div(op == "div")
     s.set_return_curl(op == "curl")
+    s.lhs_lap = (op == "lap")
     s.set_reprojection(reproject)
     rhs = list(rhs) + [None] * (3 - len(rhs))
     s.push(*rhs)

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(HERE, "libpoissonsolver_amd.so")
 
 PS_OK, PS_EINVAL, PS_EBC, PS_ESTATE, PS_ENOMEM, PS_EHIP, PS_EUNSUPPORTED = range(7)
 PS_FP64, PS_FP32, PS_MIXED = 0, 1, 2
+PS_POT_A, PS_POT_DX, PS_POT_DY, PS_POT_DZ, PS_POT_LAP = 1, 2, 4, 8, 16
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_dbl_p = ctypes.POINTER(ctypes.c_double)
@@ -63,6 +64,10 @@ SIGNATURES = {
     "ps_eval_special": (ctypes.c_int, [_c_dbl_p, _c_dbl_p, ctypes.c_long, ctypes.c_int]),
     "ps_solve3d_async": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_solve3d_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]),
+    "ps_solve3d_potential": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_pull_potential": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5),
+    "ps_pull_potential_device": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.c_void_p] * 5),
+    "ps_profile_potential": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _c_dbl_p]),
     "ps_set_device_memory_limit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_ulonglong]),
     "ps_device_memory_used": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_ulonglong)]),
     "ps_solve2d_async": (ctypes.c_int, [ctypes.c_void_p]),

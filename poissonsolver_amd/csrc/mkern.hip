@@ -104,7 +104,8 @@ hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s)
 
 bool zmfma_supported(int L, int half, int op)
 {
-    return L == MfmaZ::L && half && (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF);
+    return L == MfmaZ::L && half &&
+           (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF || op == OP_POTGRAD);
 }
 
 hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t s)
@@ -115,6 +116,7 @@ hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t 
     case OP_PLAIN: return launch_zmfma_op<OP_PLAIN>(a, persistent, s);
     case OP_GRAD: return launch_zmfma_op<OP_GRAD>(a, persistent, s);
     case OP_RHSF: return launch_zmfma_op<OP_RHSF>(a, persistent, s);
+    case OP_POTGRAD: return launch_zmfma_op<OP_POTGRAD>(a, persistent, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -98,6 +98,8 @@ struct ps_ctx {
     bool ext_rl = false;    // rhs / lhs are slices of a batch's block (not freed here)
     void* rhsf = nullptr;   // 3 x n3 real, the held rhs as pull returns it (x < nx) when it is `spec`
     void* spec = nullptr;   // 3 x A1c complex: x spectra of the held rhs over the x-extended rows
+    void* pot = nullptr;    // 5 x n3 real: outputs of the last potential solve (compact, by bit)
+    int pot_mask = 0;       // quantities (PS_POT_*) that pot holds
     void* stage = nullptr;  // kStageReals doubles: host-pointer push / pull staging (FP32 forms)
     void* A1 = nullptr;     // ncomp_alloc x nz x ny x Hp complex
     void* A2 = nullptr;     // ncomp_alloc x nz x My x Hp complex (not used by 2D contexts)
@@ -208,6 +210,8 @@ void release(ps_ctx* c)
     }
     free_ptr(c, c->rhsf);
     free_ptr(c, c->spec);
+    free_ptr(c, c->pot);
+    c->pot_mask = 0;
     free_ptr(c, c->stage);
     free_ptr(c, c->A1);
     free_ptr(c, c->A2);
@@ -438,6 +442,8 @@ enum PipeFlags : int {
     PF_DST_SPEC = 2,   // no x^-1 pass: the y^-1 pass writes x spectra into spec, components as for dst
     PF_X_ONLY = 4,     // only the x pass, writing the x spectra of src into spec (components of dst)
     PF_XINV_ONLY = 8,  // only the x^-1 pass, from spec (components of src) into dst
+    PF_FANOUT = 16,    // one input (nin 1) through x and y once; the z pass reads it for each
+                       // of the nout outputs (component stride 0) and writes them beside it
 };
 
 // The five passes (three for a 2D context).  `src` holds nin real components
@@ -496,8 +502,9 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     }
     void* A1 = wb ? wb->A1 : nullptr;
     void* A2 = wb ? wb->A2 : nullptr;
+    const bool fan = (flags & PF_FANOUT) != 0;
     if (!wb) {
-        int r = ensure_comp_buffers(c, std::max(std::max(nin, nc), nout));
+        int r = ensure_comp_buffers(c, fan ? nout + 1 : std::max(std::max(nin, nc), nout));
         if (r) return r;
         A1 = c->A1;
         A2 = c->A2;
@@ -570,6 +577,13 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     if (zarg >= 0.0) za.zarg = zarg;
     if (reg >= 0) za.reg = reg;
     za.ncomp = (nc == 3) ? 1 : nin;     // nc 2: one input, outputs 0 and 1
+    if (fan) {
+        // outputs 0..nout-1 of the one input spectrum (A2 component 0) go to
+        // A2 components 1..nout
+        za.ia.comp = 0;
+        za.out = static_cast<char*>(A2) + (size_t)c->A2c * cbytes;
+        za.ncomp = nout;
+    }
     if (c->t2d) {
         // 2D transposed path: A1 [c][y][Hp] -> A2 [c][kx][nyp], then the fused
         // pass reads contiguous y pencils of A2 and writes A1; one input read by
@@ -606,7 +620,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
 
     if (!c->dim2) {
     PencilArgs<T> yi = ya;
-    yi.in = A2;
+    yi.in = fan ? static_cast<void*>(static_cast<char*>(A2) + (size_t)c->A2c * cbytes) : A2;
     yi.out = A1;
     yi.ia = c->a2_ypass;
     yi.oa = addr_plain(c->A1c, (long)ny * c->Hp, c->Hp, 1);
@@ -1002,6 +1016,25 @@ int after_push(ps_ctx* c, int mask)
     return PS_OK;
 }
 
+// A, grad A and the Laplacian of A from the held scalar X: one x and one y
+// pass, then the z pass writes each selected quantity from the one spectrum
+// (OP_POTGRAD), and the inverse passes return them into pot (compact, in bit
+// order).  The held rhs is left as it was.
+template <typename T> int solve_potential_impl(ps_ctx* c, int mask, hipEvent_t* ev)
+{
+    int r = materialize(c);
+    if (r) return r;
+    if (!c->rhs || !c->rhs_present[0])
+        return fail(c, PS_ESTATE, "solve3d_potential: the scalar rhs X has not been pushed");
+    const int nout = __builtin_popcount((unsigned)mask);
+    if (!c->pot) CK(dev_alloc(c, &c->pot, 5 * (size_t)c->n3 * sizeof(T)));
+    r = run_pipeline<T>(c, c->rhs, 0, 1, 1, OP_POTGRAD, 1, c->pot, 0, 1, nout, ev, mask, -1.0, -1,
+                        PF_FANOUT | (c->src_spec ? PF_SRC_SPEC : 0));
+    if (r) return r;
+    c->pot_mask = mask;
+    return PS_OK;
+}
+
 int check_ready(ps_ctx* c)
 {
     if (!c) return PS_EINVAL;
@@ -1305,6 +1338,84 @@ int ps_zpass_variant(const ps_ctx* c)
     if (c->zmfma) return 3;
     if (c->zwave) return 2;
     return c->pz1.stat ? 1 : 0;
+}
+
+int ps_solve3d_potential(ps_ctx* c, int outputs)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->dim2) return fail(c, PS_ESTATE, "solve3d_potential: the solver was set up by setup2d");
+    if (outputs < 1 || outputs > 31) return fail(c, PS_EINVAL, "solve3d_potential: outputs must be a mask of 1..31");
+    r = (c->prec == PS_FP64) ? solve_potential_impl<double>(c, outputs, nullptr)
+                             : solve_potential_impl<float>(c, outputs, nullptr);
+    if (r) return r;
+    CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+// output slot of quantity bit qn in pot, or -1 if it was not computed
+static int pot_slot(const ps_ctx* c, int qn)
+{
+    if (!((c->pot_mask >> qn) & 1)) return -1;
+    return __builtin_popcount((unsigned)c->pot_mask & ((1u << qn) - 1));
+}
+
+int ps_pull_potential(ps_ctx* c, double* A, double* dx, double* dy, double* dz, double* lap)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    double* o[5] = {A, dx, dy, dz, lap};
+    for (int qn = 0; qn < 5; ++qn) {
+        const int k = pot_slot(c, qn);
+        if (!o[qn] || k < 0) continue;
+        if ((r = pull_host(c, o[qn], c->pot, k))) return r;
+    }
+    return PS_OK;
+}
+
+int ps_pull_potential_device(ps_ctx* c, void* A, void* dx, void* dy, void* dz, void* lap)
+{
+    int r = check_ready(c);
+    if (r) return r;
+    void* o[5] = {A, dx, dy, dz, lap};
+    const size_t bytes = c->n3 * real_bytes(c);
+    for (int qn = 0; qn < 5; ++qn) {
+        const int k = pot_slot(c, qn);
+        if (!o[qn] || k < 0) continue;
+        CK(hipMemcpyAsync(o[qn], static_cast<const char*>(c->pot) + k * bytes, bytes, hipMemcpyDeviceToDevice,
+                          c->stream));
+    }
+    CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+int ps_profile_potential(ps_ctx* c, int outputs, int iters, double ms[6])
+{
+    int r = check_ready(c);
+    if (r) return r;
+    if (c->dim2) return fail(c, PS_ESTATE, "profile_potential: the solver was set up by setup2d");
+    if (iters < 1 || !ms || outputs < 1 || outputs > 31) return fail(c, PS_EINVAL, "profile_potential: bad argument");
+    if (!c->ev_made) {
+        for (hipEvent_t& e : c->ev) CK(hipEventCreate(&e));
+        c->ev_made = true;
+    }
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int it = 0; it < iters; ++it) {
+        r = (c->prec == PS_FP64) ? solve_potential_impl<double>(c, outputs, c->ev)
+                                 : solve_potential_impl<float>(c, outputs, c->ev);
+        if (r) return r;
+        CK(hipEventSynchronize(c->ev[5]));
+        for (int k = 0; k < 5; ++k) {
+            float t = 0.f;
+            CK(hipEventElapsedTime(&t, c->ev[k], c->ev[k + 1]));
+            acc[k] += t;
+        }
+        float tt = 0.f;
+        CK(hipEventElapsedTime(&tt, c->ev[0], c->ev[5]));
+        acc[5] += tt;
+    }
+    for (int k = 0; k < 6; ++k) ms[k] = acc[k] / iters;
+    return PS_OK;
 }
 
 int ps_set_device_memory_limit(ps_ctx* c, unsigned long long bytes)

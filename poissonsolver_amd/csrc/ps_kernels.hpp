@@ -21,7 +21,12 @@ enum PencilOp : int {
     OP_CURL = 4,      // l = Gk i k x r
     OP_RHSF = 5,      // r_c * zeta / N               (filtered rhs, no reprojection)
     OP_RHSF_PROJ = 6, // r_d*zeta/N - Gk (kh_d kh.r + kn_d kn.r)  (filtered + reprojected)
+    OP_POTGRAD = 7,   // one scalar r, output c = the c-th quantity whose bit is set in kmap:
+                      //   bit 0 Gk r, bits 1..3 i k_d Gk r (d = x, y, z), bit 4 -|k|^2 Gk r
 };
+
+// Quantities of OP_POTGRAD (bits of PencilArgs::kmap).
+enum PotQuantity : int { POT_A = 1, POT_DX = 2, POT_DY = 4, POT_DZ = 8, POT_LAP = 16 };
 
 // Wavenumber axis of each vector component for OP_GRAD and OP_DIV.  A 2D
 // solve runs as a 3D one of shape (nx, 1, ny): its y direction is the pass

@@ -83,6 +83,31 @@ __device__ __forceinline__ T kcomp(int kmap, int c, T kx, T ky, T kz)
     return (c == 0) ? kx : (c == 1) ? kz : T(0);
 }
 
+// OP_POTGRAD: quantity of output component c, the c-th set bit of `mask`
+// (0 A, 1..3 d/dx, d/dy, d/dz, 4 Laplacian)
+__device__ __forceinline__ int pot_quantity(int mask, int c)
+{
+    for (int qn = 0; qn < 5; ++qn) {
+        if (!((mask >> qn) & 1)) continue;
+        if (c == 0) return qn;
+        --c;
+    }
+    return 0;
+}
+
+// OP_POTGRAD multiplier of quantity qn at one spectrum point: real m (A and
+// the Laplacian: Gk, -|k|^2 Gk) or imaginary i m (derivatives: i k_d Gk with
+// the Hermitian part of k_d, as OP_GRAD); kzh Hermitian, kzf full wavenumber
+template <typename T>
+__device__ __forceinline__ T pot_factor(int qn, const TileK<T>& tk, T kzh, double kzf, T g)
+{
+    if (qn == 0) return g;
+    if (qn == 1) return tk.kxh * g;
+    if (qn == 2) return tk.kyh * g;
+    if (qn == 3) return kzh * g;
+    return (T)(-(tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf)) * g;
+}
+
 // k-space operator of the fused z pass on one thread's E spectrum values
 // (solve3d.cpp:329-396 in the Hermitian-part form of the half spectrum).
 // Output o of the NC inputs v goes to w; positions are kz = t + TP u.
@@ -115,6 +140,11 @@ __device__ __forceinline__ void kspace_op(cplx<T> (&w)[E], const cplx<T> (&v)[NC
             const T kd = kcomp<T>(a.kmap, comp, tk.kxh, tk.kyh, kzh);
             const C s = v[0][u];
             r = C{-kd * g * s.y, kd * g * s.x};
+        } else if (op == OP_POTGRAD) {
+            const int qn = pot_quantity(a.kmap, comp);
+            const T m = pot_factor<T>(qn, tk, kzh, kzf, g);
+            const C s = v[0][u];
+            r = (qn >= 1 && qn <= 3) ? C{-m * s.y, m * s.x} : C{m * s.x, m * s.y};
         } else if (op == OP_DIV || op == OP_CURL) {
             const C sx = v[0][u], sy = v[vi][u], sz = v[wi][u];
             C q;
@@ -172,6 +202,7 @@ __device__ __forceinline__ void kspace_op_inplace(cplx<T> (&v)[E], const PencilA
     using C = cplx<T>;
     const int op = a.op;
     if (op == OP_NONE) return;
+    const int qn = (op == OP_POTGRAD) ? pot_quantity(a.kmap, comp) : 0;
 #pragma unroll
     for (int u = 0; u < E; ++u) {
         const int kz = kzm * (t + TP * u) + kza;
@@ -179,6 +210,12 @@ __device__ __forceinline__ void kspace_op_inplace(cplx<T> (&v)[E], const PencilA
         const T g = tk.grow ? tk.grow[kzo * a.g_kz] : T(0);
         if (op == OP_PLAIN) {
             v[u] = cscale(v[u], g);
+        } else if (op == OP_POTGRAD) {
+            const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
+            const T kzh = (2 * kz == a.Mz) ? T(0) : (T)kzf;
+            const T m = pot_factor<T>(qn, tk, kzh, kzf, g);
+            const C s = v[u];
+            v[u] = (qn >= 1 && qn <= 3) ? C{-m * s.y, m * s.x} : C{m * s.x, m * s.y};
         } else if (op == OP_GRAD) {
             const T kzh = (2 * kz == a.Mz) ? T(0) : (T)kappa_full(kz, a.Mz, a.dk[2]);
             const T kd = kcomp<T>(a.kmap, comp, tk.kxh, tk.kyh, kzh);

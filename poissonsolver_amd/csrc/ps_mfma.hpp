@@ -319,15 +319,19 @@ __device__ __forceinline__ void mf_kfactors(float (&f)[8], const PencilArgs<floa
             float v;
             if constexpr (OPK == OP_NONE) {
                 v = 1.0f;
-            } else if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD) {
+            } else if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_POTGRAD) {
                 const unsigned go = h ? (unsigned)goff1 - (unsigned)e * (unsigned)gstep
                                       : (unsigned)goff0 + (unsigned)e * (unsigned)gstep;
                 const float g = tk.grow ? *reinterpret_cast<const float*>(reinterpret_cast<const char*>(tk.grow) + go) : 0.0f;
                 if constexpr (OPK == OP_PLAIN) {
                     v = g;
-                } else {
+                } else if constexpr (OPK == OP_GRAD) {
                     const float kzh = (2 * kz == a.Mz) ? 0.0f : (float)kappa_full(kz, a.Mz, a.dk[2]);
                     v = kcomp<float>(a.kmap, comp, tk.kxh, tk.kyh, kzh) * g;
+                } else {
+                    const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
+                    const float kzh = (2 * kz == a.Mz) ? 0.0f : (float)kzf;
+                    v = pot_factor<float>(pot_quantity(a.kmap, comp), tk, kzh, kzf, g);
                 }
             } else {
                 const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
@@ -346,6 +350,18 @@ __device__ __forceinline__ void mf_kfactors(float (&f)[8], const PencilArgs<floa
             f[4 * h + e] = v;
         }
     }
+}
+
+// whether the k-space multiplier of component `comp` is imaginary (i k_d Gk:
+// grad, and the derivative quantities of OP_POTGRAD); uniform per tile
+template <int OPK> __device__ __forceinline__ bool mf_imag_multiplier(const PencilArgs<float>& a, int comp)
+{
+    if constexpr (OPK == OP_GRAD) return true;
+    if constexpr (OPK == OP_POTGRAD) {
+        const int qn = pot_quantity(a.kmap, comp);
+        return qn >= 1 && qn <= 3;
+    }
+    return false;
 }
 
 // One tile per workgroup (4 wavefronts per SIMD, two workgroups per CU).
@@ -377,6 +393,7 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
     // columns past the grid repeat the last one: loads stay in bounds, their
     // results are never stored, and no pencil's data mixes with another's
     const int colr = cv ? col : a.ncol - 1;
+    const bool mimag = mf_imag_multiplier<OPK>(a, comp);
 
     for (int i = tid; i < L; i += 512) stw[i] = a.plan.tw[i];
     if (tid < 2 * P) smax[tid] = 0u;
@@ -470,7 +487,7 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
-            if constexpr (OPK == OP_GRAD) {
+            if (mimag) {
                 // i k_d Gk: purely imaginary
                 z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
                 z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};
@@ -671,10 +688,11 @@ __global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
             // the k-space operator keeps the factor s0
             float2 z0[4], z1[4];
             float mz = 0.0f;
+            const bool mimag = mf_imag_multiplier<OPK>(a, T.comp);
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
-                if constexpr (OPK == OP_GRAD) {
+                if (mimag) {
                     // i k_d Gk: purely imaginary
                     z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
                     z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};

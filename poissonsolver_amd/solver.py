@@ -201,6 +201,29 @@ class poisson_solver:
         """Enqueue solve3d on the context's stream and return."""
         self._report(self._lib.ps_solve3d_async(self._ctx), "solve3d_async")
 
+    POT_NAMES = ("A", "dx", "dy", "dz", "lap")
+
+    def solve3d_potential(self, outputs=("A", "dx", "dy", "dz")):
+        """A, grad A and / or the Laplacian of A from the held scalar X in one
+        pipeline (ps_solve3d_potential); outputs from POT_NAMES."""
+        mask = 0
+        for name in outputs:
+            mask |= 1 << self.POT_NAMES.index(name)
+        self._report(self._lib.ps_solve3d_potential(self._ctx, mask), "solve3d_potential")
+
+    def pull_potential(self, A=None, dx=None, dy=None, dz=None, lap=None):
+        arrs = [self._host(a, True) for a in (A, dx, dy, dz, lap)]
+        self._report(self._lib.ps_pull_potential(self._ctx, *[_ptr(a) for a in arrs]), "pull_potential")
+
+    def profile_potential(self, outputs=("A", "dx", "dy", "dz"), iters: int = 1):
+        mask = 0
+        for name in outputs:
+            mask |= 1 << self.POT_NAMES.index(name)
+        ms = (ctypes.c_double * 6)()
+        if not self._report(self._lib.ps_profile_potential(self._ctx, mask, int(iters), ms), "profile_potential"):
+            return None
+        return dict(zip(("x", "y", "z", "y_inv", "x_inv", "solve"), list(ms)))
+
     def set_device_memory_limit(self, nbytes: int):
         """Device bytes this context may hold (0: no limit)."""
         return self._report(self._lib.ps_set_device_memory_limit(self._ctx, int(nbytes)), "set_device_memory_limit")

@@ -1,0 +1,97 @@
+"""The potential extension (ps_solve3d_potential): A, grad A and the
+Laplacian of A from one pushed scalar in one pipeline, against the oracle
+(A: plain solve, grad: the reference's component-wise grad of the scalar
+pushed three times, Laplacian: (ik.ik) G r, which the oracle pins to the
+closed form -Gaussian(s_eff) for the config-1 source) and against the
+component-wise GPU solves it replaces."""
+import math
+
+import numpy as np
+import pytest
+
+import bench
+import poissonsolver_amd as P
+from oracle import oracle as O
+from tests.gpu_util import PREC, announce, gpu_solve, random_rhs
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp64": 1e-12, "fp32": 2e-6, "mixed": 1e-6}
+NAMES = ("A", "dx", "dy", "dz", "lap")
+
+
+def _potential(prec, n, bc, dx, reg, b, outputs=NAMES):
+    s = P.poisson_solver(PREC[prec])
+    s.setup3d(n, bc, dx, reg)
+    assert s.last_status == 0
+    s.push(b)
+    s.solve3d_potential(outputs)
+    assert s.last_status == 0
+    out = {k: np.full(b.shape, np.nan) for k in NAMES}
+    s.pull_potential(*[out[k] for k in NAMES])
+    assert s.last_status == 0
+    rb = np.full(b.shape, np.nan)
+    s.pull(None, None, None, rb)          # the held rhs is left as pushed
+    s.close()
+    return out, rb
+
+
+CASES = [((12, 10, 8), (0, 0, 0), (0.1, 0.12, 0.09), 2),
+         ((16, 12, 10), (1, 1, 0), (1 / 16, 1 / 12, 0.1), 6),
+         ((10, 9, 12), (0, 0, 1), (0.1, 0.1, 1 / 12), 0),
+         ((8, 10, 12), (1, 1, 1), (1 / 8, 0.1, 1 / 12), 4),
+         ((16, 12, 512), (0, 0, 0), (0.1, 0.1, 1 / 256), 2)]   # Mz = 1024: matrix-core z pass when mixed
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "%dx%dx%d-%d%d%d" % (c[0] + c[1]))
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "mixed"])
+def test_potential_vs_oracle_and_componentwise(case, prec):
+    n, bc, dx, reg = case
+    announce("potential", n, bc, reg, prec)
+    b = random_rhs(n, 77, 1)[0]
+    out, rb = _potential(prec, n, bc, dx, reg, b)
+    tol = TOL[prec]
+    lo_a, _ = O.solve(n, bc, dx, reg, [b])
+    lo_g, _ = O.solve(n, bc, dx, reg, [b, b, b], op="grad")
+    lo_l, _ = O.solve(n, bc, dx, reg, [b], op="lap")
+    assert O.rel_l2(out["A"], lo_a[0]) <= tol
+    assert O.rel_l2([out["dx"], out["dy"], out["dz"]], lo_g) <= tol
+    assert O.rel_l2(out["lap"], lo_l[0]) <= tol
+    assert np.array_equal(rb, b if prec == "fp64" else b.astype(np.float32).astype(np.float64))
+    # the component-wise solves it replaces
+    ga, _ = gpu_solve(n, bc, dx, reg, [b], prec=prec)
+    gg, _ = gpu_solve(n, bc, dx, reg, [b, b, b], op="grad", prec=prec)
+    assert O.rel_l2(out["A"], ga[0]) <= tol
+    assert O.rel_l2([out["dx"], out["dy"], out["dz"]], gg) <= tol
+
+
+def test_potential_subset_and_errors():
+    n, bc, dx, reg = CASES[0]
+    b = random_rhs(n, 78, 1)[0]
+    out, _ = _potential("fp64", n, bc, dx, reg, b, outputs=("dy", "lap"))
+    for k in ("A", "dx", "dz"):
+        assert np.isnan(out[k]).all(), k
+    lo_g, _ = O.solve(n, bc, dx, reg, [b, b, b], op="grad")
+    assert O.rel_l2(out["dy"], lo_g[1]) <= 1e-12
+    s = P.poisson_solver()
+    s.setup3d(n, bc, dx, reg)
+    s.solve3d_potential(("A",))
+    assert s.last_status == P.PS_ESTATE          # nothing pushed
+    s.push(None, b, None)
+    s.solve3d_potential(("A",))
+    assert s.last_status == P.PS_ESTATE          # the scalar is rhs X
+    s.close()
+
+
+def test_potential_laplacian_closed_form():
+    """Gaussian source, reg 2 (config 1): the Laplacian of the regularised
+    potential is minus the Gaussian of width s_eff = sqrt(s^2 + (2h)^2)."""
+    n = 64
+    announce("potential closed form", n)
+    b, h = bench.gaussian_source(n)
+    out, _ = _potential("fp64", (n, n, n), (0, 0, 0), (h, h, h), 2, b, outputs=("lap",))
+    se = math.sqrt(0.1 ** 2 + (2 * h) ** 2)
+    x = -1.0 + (np.arange(n) + 0.5) * h
+    r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+    g = np.exp(-r2 / (2 * se * se)) / ((2 * np.pi) ** 1.5 * se ** 3)
+    assert O.rel_l2(out["lap"], -g) <= 1e-10

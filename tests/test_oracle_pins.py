@@ -284,3 +284,22 @@ def test_smooth3d_gaussian_and_single_mode():
     want = (np.exp(-0.5 * h * h * kp2) * np.sin(2 * np.pi * X) * np.cos(4 * np.pi * Y)
             * (s / sz) * np.exp(-Z ** 2 / (2 * sz * sz)))
     assert O.rel_l2(o.rhs[0][:, :, :16].real, want) < 1e-9
+
+
+def test_oracle_laplacian_of_potential_closed_form():
+    """The oracle's Laplacian of the potential ((ik.ik) G r, the extension
+    output of the potential solve): for the config-1 Gaussian with reg 2 it is
+    minus the Gaussian of width s_eff = sqrt(s^2 + (2h)^2)."""
+    import math
+
+    import bench
+    n = 32
+    b, h = bench.gaussian_source(n)
+    lo, _ = O.solve((n,) * 3, (0, 0, 0), (h,) * 3, 2, [b], op="lap")
+    se = math.sqrt(0.1 ** 2 + (2 * h) ** 2)
+    x = -1.0 + (np.arange(n) + 0.5) * h
+    r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
+    g = np.exp(-r2 / (2 * se * se)) / ((2 * np.pi) ** 1.5 * se ** 3)
+    assert O.rel_l2(lo[0], -g) <= 1e-10
+    # a plausible slip (the Hermitian sign of ik, i.e. +|k|^2) fails it
+    assert O.rel_l2(-lo[0], -g) > 1.0
