@@ -61,6 +61,22 @@ def alg_bytes(n, cin, cout, eb):
     return dict(x=P1, y=P2, z=P3, y_inv=P4, x_inv=P5, solve=P1 + P2 + P3 + P4 + P5)
 
 
+def alg_bytes_grid(ncell, bc, cin, cout, eb):
+    """The pruned 5-pass byte model for any boundary tuple (M_d = 2 n_d for an
+    unbounded direction, n_d for a periodic one): cin spectra through the x,
+    y and z passes (the z pass reads each input once), cout outputs back."""
+    nx, ny, nz = ncell
+    M = [2 * n if b == 0 else n for n, b in zip(ncell, bc)]
+    H = M[0] // 2 + 1
+    er, ec, eg = eb, 2 * eb, eb
+    P1 = cin * (nx * ny * nz * er + H * ny * nz * ec)
+    P2 = cin * (H * ny * nz + H * M[1] * nz) * ec
+    P3 = cin * H * M[1] * nz * ec + (M[0] // 2 + 1) * (M[1] // 2 + 1) * (M[2] // 2 + 1) * eg + cout * H * M[1] * nz * ec
+    P4 = cout * (H * M[1] * nz + H * ny * nz) * ec
+    P5 = cout * (H * ny * nz * ec + nx * ny * nz * er)
+    return dict(x=P1, y=P2, z=P3, y_inv=P4, x_inv=P5, solve=P1 + P2 + P3 + P4 + P5)
+
+
 def cpu_baseline(n_target, cin, op):
     """The oracle (numpy FP64 restatement of the reference, one core) timed on
     a bounded sample: push+solve+pull at 128^3, scaled to n_target by
@@ -289,19 +305,25 @@ def bench_config(a):
       ppu256     x,y periodic / z unbounded 256^3, reg 6: A and grad A
       vortex512  512^3 unbounded vortex ring: streamfunction (3 components),
                  velocity by spectral curl, div of the streamfunction
-      large1024  1024^3 unbounded, reg 10, 256 blobs: A and grad A (FP32 and
-                 mixed only; FP64 check on every 4th point per direction)
-    A step is the workload's solves in sequence; each solve times its passes
-    with HIP events."""
+      large1024  1024^3 unbounded, reg 10, 256 blobs: A, grad A and the
+                 Laplacian of A (FP32 and mixed only; FP64 check on every 4th
+                 point per direction)
+    For ppu256 and large1024 the outputs come two ways: the reference API's
+    component-wise solves (A plain; grad of the scalar pushed three times),
+    and the potential extension (one pipeline from the one scalar).  `value`
+    is the rate of the potential form.  Each solve times its passes with HIP
+    events."""
     import poissonsolver_amd as P
     codes = {"fp32": P.PS_FP32, "mixed": P.PS_MIXED, "fp64": P.PS_FP64}
     precs = ["mixed", "fp32"] if a.prec == "mixed" else [a.prec]
+    pot = None          # outputs of the potential form
     if a.workload == "ppu256":
         n = a.n if a.n != 512 else 256
         src, dx = ppu_source(n)
         ncell, bc, reg = (n, n, n), (1, 1, 0), 6
         solves = [("A", "plain", [src]), ("gradA", "grad", [src, src, src])]
         cin_out = [(1, 1), (3, 3)]
+        pot = ("A", "dx", "dy", "dz")
         desc = "%d^3 x,y periodic z unbounded (doubled in z only), reg 6, sin*sin*gauss + random-phase source: A and grad A" % n
     elif a.workload == "vortex512":
         n = a.n
@@ -316,8 +338,10 @@ def bench_config(a):
         ncell, bc, reg, dx = (n, n, n), (0, 0, 0), 10, (h, h, h)
         solves = [("A", "plain", [src]), ("gradA", "grad", [src, src, src])]
         cin_out = [(1, 1), (3, 3)]
-        desc = "%d^3 fully unbounded, reg 10, 256 Gaussian blobs: A and grad A" % n
+        pot = ("A", "dx", "dy", "dz", "lap")
+        desc = "%d^3 fully unbounded, reg 10, 256 Gaussian blobs: A, grad A and the Laplacian of A" % n
     sub = (slice(None, None, 4),) * 3 if a.workload == "large1024" else (slice(None),) * 3
+    shape = (ncell[2], ncell[1], ncell[0])
     results = {}
     outs = {}
     for prec in precs + ["fp64"]:
@@ -335,21 +359,62 @@ def bench_config(a):
                              "z_kernel": s.zpass_variant()}
             else:
                 s.solve3d()
+            if s.last_status:
+                raise RuntimeError("solve %s %s failed: status %d" % (prec, name, s.last_status))
             nout = 1 if op == "div" else len(rhs) if op != "plain" else len(rhs)
-            lhs = [np.empty_like(rhs[0]) for _ in range(nout)]
+            lhs = [np.empty(shape) for _ in range(nout)]
             s.pull(*lhs)
             outs[(prec, name)] = [x[sub].copy() for x in lhs]
             del lhs
             s.close()
+        if pot is not None:
+            # the potential form: one pipeline from the one scalar.  The FP64
+            # reference runs the Laplacian alone, and at 1024^3 takes A and grad
+            # A from the component-wise FP64 solves (a four-output FP64 solve
+            # there would not fit in HBM next to its work buffers)
+            big = ncell[0] * ncell[1] * ncell[2] >= 512 ** 3 * 4
+            if prec != "fp64":
+                groups = [pot]
+            else:
+                groups = ([] if big else [tuple(k for k in pot if k != "lap")]) + ([("lap",)] if "lap" in pot else [])
+            got = {}
+            if prec == "fp64" and big:
+                for k, x in zip(("A", "dx", "dy", "dz"), outs[("fp64", "A")] + outs[("fp64", "gradA")]):
+                    got[k] = x
+            for grp in groups:
+                s = P.poisson_solver(codes[prec])
+                s.setup3d(ncell, bc, dx, reg)
+                s.push(src)
+                if prec != "fp64":
+                    if a.warmup > 0:
+                        s.profile_potential(grp, a.warmup)
+                    s.synchronize()
+                    prof = s.profile_potential(grp, a.steps)
+                    per["potential"] = {"ms": prof["solve"], "outputs": list(grp),
+                                        "pass_ms": {k: prof[k] for k in ("x", "y", "z", "y_inv", "x_inv")},
+                                        "z_kernel": s.zpass_variant()}
+                else:
+                    s.solve3d_potential(grp)
+                if s.last_status:
+                    raise RuntimeError("potential solve %s %s failed: status %d" % (prec, grp, s.last_status))
+                arrs = {k: (np.empty(shape) if k in grp else None) for k in P.poisson_solver.POT_NAMES}
+                s.pull_potential(*[arrs[k] for k in P.poisson_solver.POT_NAMES])
+                for k in grp:
+                    got[k] = arrs[k][sub].copy()
+                del arrs
+                s.close()
+            outs[(prec, "potential")] = [got[k] for k in pot]
         results[prec] = per
     line_prec = precs[0]
-    step_ms = sum(v["ms"] for v in results[line_prec].values())
+    comp_ms = sum(results[line_prec][nm]["ms"] for nm, _, _ in solves)
+    step_ms = results[line_prec]["potential"]["ms"] if pot is not None else comp_ms
+    eb = 8 if line_prec == "fp64" else 4
     out = {
         "metric": METRIC, "value": 1000.0 / step_ms, "unit": "steps/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": step_ms, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": line_prec, "data": "synthetic",
         "config": {"workload": a.workload + ": " + desc, "ncell": list(ncell), "bc": list(bc), "reg_order": reg,
-                   "step": [nm for nm, _, _ in solves]},
+                   "step": ["potential: " + ", ".join(pot)] if pot is not None else [nm for nm, _, _ in solves]},
         "solves": {}, "rel_l2_vs_fp64_gpu": {},
     }
     for prec in precs:
@@ -363,8 +428,32 @@ def bench_config(a):
                                                                      outs[("fp64", "u")])
             else:
                 out["rel_l2_vs_fp64_gpu"][key] = _rel(outs[(prec, name)], outs[("fp64", name)])
+        if pot is not None:
+            key = "%s/potential" % prec
+            out["solves"][key] = results[prec]["potential"]
+            for k, x, y in zip(pot, outs[(prec, "potential")], outs[("fp64", "potential")]):
+                out["rel_l2_vs_fp64_gpu"]["%s/%s" % (key, k)] = _rel([x], [y])
+    if pot is not None:
+        if not (ncell[0] * ncell[1] * ncell[2] >= 512 ** 3 * 4):
+            # the potential form against the component-wise solves (FP64: the same numbers)
+            ref = outs[("fp64", "A")] + outs[("fp64", "gradA")]
+            out["rel_l2_fp64_potential_vs_componentwise"] = _rel(outs[("fp64", "potential")][:4], ref)
+        out["componentwise_ms"] = comp_ms
+        out["speedup_potential_vs_componentwise"] = comp_ms / step_ms
+        B = alg_bytes_grid(ncell, bc, 1, len(pot), eb)
+        Bc = sum(alg_bytes_grid(ncell, bc, ci, co, eb)["solve"] for ci, co in cin_out)
+        pp = results[line_prec]["potential"]["pass_ms"]
+        out["alg_bytes_per_step"] = B["solve"]
+        out["alg_bytes_componentwise"] = Bc
+        out["step_hbm_gbs"] = B["solve"] / (step_ms * 1e-3) / 1e9
+        out["roofline"] = {"bound": "hbm", "achieved": B["z"] / (pp["z"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                           "unit": "GB/s", "frac": B["z"] / (pp["z"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                           "kernel": "z pass of the potential form (%s kernel: one spectrum in, %d quantities out)"
+                                     % (results[line_prec]["potential"]["z_kernel"], len(pot)),
+                           "alg_bytes_per_launch": B["z"], "avg_ms": pp["z"]}
     if len(precs) == 2:
-        out["speedup_vs_fp32"] = sum(v["ms"] for v in results["fp32"].values()) / step_ms
+        f32 = sum(v["ms"] for k, v in results["fp32"].items() if pot is None or k == "potential")
+        out["speedup_vs_fp32"] = f32 / step_ms
     if a.workload == "vortex512":
         # div psi relative to the velocity magnitude scale (L2 ratio, FP64)
         dv = outs[("fp64", "divpsi")]
@@ -541,6 +630,10 @@ def bench_batch(a):
         "gpu_ms_per_batch": gpu_ms,
         "alg_bytes_per_batch": alg,
         "batch_hbm_gbs": alg / (gpu_ms * 1e-3) / 1e9,
+        "roofline": {"bound": "hbm", "achieved": alg / (gpu_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": alg / (gpu_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "whole batch: one hipGraph launch of every problem's passes (grouped solves)",
+                     "alg_bytes_per_launch": alg, "avg_ms": gpu_ms},
         "setup_s": setup_s, "autotune_s": tune_s,
     }
     if not a.no_check:
@@ -578,6 +671,7 @@ def main(argv=None):
     ap.add_argument("--op", default="plain", choices=["plain", "curl"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-host", action="store_true", help="skip the host-pointer API end-to-end timing")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run a 1 GiB device-to-device copy (HBM counter calibration) and exit")
     ap.add_argument("--no-traffic", action="store_true",
@@ -712,6 +806,24 @@ def main(argv=None):
             se = math.sqrt(0.1 ** 2 + (2 * h) ** 2)
             A = erf(rr / (math.sqrt(2) * se)) / (4 * np.pi * rr)
             out["rel_l2_vs_analytic"] = float(np.sqrt(np.sum((lhs[0] - A) ** 2) / np.sum(A ** 2)))
+    if not (a.pmc_child or a.no_host):
+        # the host-pointer API end to end (not `value`): push from host FP64
+        # arrays, solve, pull to host FP64 arrays; PCIe transfers and the
+        # device-side FP64 <-> FP32 conversion included
+        lh = [np.empty_like(b) for _ in range(cout)]
+        s.push(*srcs)
+        s.solve3d()
+        s.pull(*lh)
+        reps = 3
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s.push(*srcs)
+            s.solve3d()
+            s.pull(*lh)
+        host_ms = 1000.0 * (time.perf_counter() - t0) / reps
+        out["host_api"] = {"value": 1000.0 / host_ms, "unit": "solves/s", "ms_per_solve": host_ms,
+                           "bytes_over_pcie": (cin + cout) * n ** 3 * 8,
+                           "what": "push(host FP64) + solve3d + pull(host FP64), %d repeats" % reps}
     s.close()
     # the plain FP32 GPU FFT path, timed beside the mixed form on the same input
     if a.prec == "mixed" and not (a.no_compare or a.pmc_child):

@@ -98,7 +98,8 @@ struct ps_ctx {
     bool ext_rl = false;    // rhs / lhs are slices of a batch's block (not freed here)
     void* rhsf = nullptr;   // 3 x n3 real, the held rhs as pull returns it (x < nx) when it is `spec`
     void* spec = nullptr;   // 3 x A1c complex: x spectra of the held rhs over the x-extended rows
-    void* pot = nullptr;    // 5 x n3 real: outputs of the last potential solve (compact, by bit)
+    void* pot = nullptr;    // pot_cap x n3 real: outputs of the last potential solve (compact, by bit)
+    int pot_cap = 0;
     int pot_mask = 0;       // quantities (PS_POT_*) that pot holds
     void* stage = nullptr;  // kStageReals doubles: host-pointer push / pull staging (FP32 forms)
     void* A1 = nullptr;     // ncomp_alloc x nz x ny x Hp complex
@@ -211,7 +212,7 @@ void release(ps_ctx* c)
     free_ptr(c, c->rhsf);
     free_ptr(c, c->spec);
     free_ptr(c, c->pot);
-    c->pot_mask = 0;
+    c->pot_mask = c->pot_cap = 0;
     free_ptr(c, c->stage);
     free_ptr(c, c->A1);
     free_ptr(c, c->A2);
@@ -1027,7 +1028,13 @@ template <typename T> int solve_potential_impl(ps_ctx* c, int mask, hipEvent_t* 
     if (!c->rhs || !c->rhs_present[0])
         return fail(c, PS_ESTATE, "solve3d_potential: the scalar rhs X has not been pushed");
     const int nout = __builtin_popcount((unsigned)mask);
-    if (!c->pot) CK(dev_alloc(c, &c->pot, 5 * (size_t)c->n3 * sizeof(T)));
+    if (nout > c->pot_cap) {
+        c->pot_mask = 0;
+        free_ptr(c, c->pot);
+        c->pot_cap = 0;
+        CK(dev_alloc(c, &c->pot, (size_t)nout * c->n3 * sizeof(T)));
+        c->pot_cap = nout;
+    }
     r = run_pipeline<T>(c, c->rhs, 0, 1, 1, OP_POTGRAD, 1, c->pot, 0, 1, nout, ev, mask, -1.0, -1,
                         PF_FANOUT | (c->src_spec ? PF_SRC_SPEC : 0));
     if (r) return r;
