@@ -660,6 +660,44 @@ def bench_batch(a):
     return 0
 
 
+def trace_gaps(path, per_launch, launches):
+    """Idle time inside and between graph launches from a rocprofv3
+    kernel_trace.csv: the last `launches` x `per_launch` dispatches, grouped
+    in start order into launches; for each, the span from the first kernel's
+    start to the last kernel's end against the union of the kernels' busy
+    intervals, and the gap to the next launch."""
+    import csv
+    rows = list(csv.DictReader(open(path)))
+    ks = sorted(((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in rows
+                 if "copyBuffer" not in r["Kernel_Name"] and "fillBuffer" not in r["Kernel_Name"]))
+    ks = ks[-per_launch * launches:]
+    spans, busys, gaps = [], [], []
+    for i in range(launches):
+        grp = ks[i * per_launch:(i + 1) * per_launch]
+        lo, hi = grp[0][0], max(e for _, e, _ in grp)
+        busy, cur_s, cur_e = 0, None, None
+        for st, en, _ in grp:
+            if cur_e is None or st > cur_e:
+                if cur_e is not None:
+                    busy += cur_e - cur_s
+                cur_s, cur_e = st, en
+            else:
+                cur_e = max(cur_e, en)
+        busy += cur_e - cur_s
+        spans.append(hi - lo)
+        busys.append(busy)
+        if i + 1 < launches:
+            gaps.append(ks[(i + 1) * per_launch][0] - hi)
+    span = sum(spans) / len(spans) / 1e6
+    busy = sum(busys) / len(busys) / 1e6
+    gap = (sum(gaps) / len(gaps) / 1e6) if gaps else 0.0
+    out = {"launches": launches, "kernels_per_launch": per_launch, "span_ms": span, "busy_ms": busy,
+           "idle_in_launch_ms": span - busy, "gap_between_launches_ms": gap,
+           "idle_fraction": (span - busy + gap) / (span + gap)}
+    print(json.dumps(out))
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -681,7 +719,13 @@ def main(argv=None):
                     choices=["single", "batch", "ppu256", "vortex512", "large1024", "uu2d"])
     ap.add_argument("--nprob", type=int, default=64)
     ap.add_argument("--autotune", action="store_true", help="batch: autotune each distinct size first")
+    ap.add_argument("--trace-gaps", default=None, metavar="KERNEL_TRACE_CSV",
+                    help="only summarise idle time of graph launches in a rocprofv3 kernel trace")
+    ap.add_argument("--per-launch", type=int, default=35, help="--trace-gaps: kernels per graph launch")
+    ap.add_argument("--launches", type=int, default=10, help="--trace-gaps: last launches to summarise")
     a = ap.parse_args(argv)
+    if a.trace_gaps:
+        return trace_gaps(a.trace_gaps, a.per_launch, a.launches)
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % a.gpus, file=sys.stderr)
         return 2
