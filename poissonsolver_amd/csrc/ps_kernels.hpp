@@ -81,6 +81,7 @@ template <typename T> struct PencilArgs {
     double zarg;     // 0.5 (2h)^2
     int reg;         // regularisation order (even), 0 = none
     double invN;     // 1/(Mx My Mz)
+    int col0;        // kx of the launch's column 0 (0: the launch covers every column)
     // matrix-core z pass: per-lane fp16 hi/lo fragments of the 32-point DFT
     // matrix (64 lanes x 8 fragments x 8 halves), built once at setup
     const void* mfrag;

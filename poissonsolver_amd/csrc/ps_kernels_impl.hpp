@@ -60,7 +60,7 @@ __device__ __forceinline__ TileK<T> tile_k(const PencilArgs<T>& a, int col, int 
 {
     TileK<T> k{T(0), T(0), T(0), T(0), 0.0, 0.0, nullptr};
     if (a.op == OP_NONE) return k;
-    const int kx = col, ky = slab;
+    const int kx = col + a.col0, ky = slab;
     k.kxf = kappa_full(kx, a.Mx, a.dk[0]);
     k.kyf = kappa_full(ky, a.My, a.dk[1]);
     const bool nyx = (2 * kx == a.Mx), nyy = (2 * ky == a.My);
