@@ -77,7 +77,7 @@ def alg_bytes_grid(ncell, bc, cin, cout, eb):
     return dict(x=P1, y=P2, z=P3, y_inv=P4, x_inv=P5, solve=P1 + P2 + P3 + P4 + P5)
 
 
-def cpu_baseline(n_target, cin, op):
+def cpu_baseline(n_target, cin, op, bc=(0, 0, 0)):
     """The oracle (numpy FP64 restatement of the reference, one core) timed on
     a bounded sample: push+solve+pull at 128^3, scaled to n_target by
     n^3 log n.  The reference itself cannot be built here (it needs MPI)."""
@@ -85,7 +85,7 @@ def cpu_baseline(n_target, cin, op):
     ns = 128
     b, h = gaussian_source(ns)
     s = O.OracleSolver()
-    s.setup3d((ns,) * 3, (0, 0, 0), (h,) * 3, 2)
+    s.setup3d((ns,) * 3, tuple(bc), (h,) * 3, 2)
     s.set_return_curl(op == "curl")
     reps, t = 0, 0.0
     while reps < 3 and t < 20.0:
@@ -99,8 +99,9 @@ def cpu_baseline(n_target, cin, op):
     per = t / reps
     scale = (n_target / ns) ** 3 * math.log2(2 * n_target) / math.log2(2 * ns)
     return {"value": 1.0 / (per * scale), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": "oracle (numpy FP64 C2C restatement) push+solve3d+pull at %d^3 UUU reg2, "
-                      "%d reps, %.2f s/solve, scaled to %d^3 by n^3 log n (x%.0f)" % (ns, reps, per, n_target, scale)}
+            "sample": "oracle (numpy FP64 C2C restatement) push+solve3d+pull at %d^3 bc %s reg2, "
+                      "%d reps, %.2f s/solve, scaled to %d^3 by n^3 log n (x%.0f)"
+                      % (ns, "".join(str(v) for v in bc), reps, per, n_target, scale)}
 
 
 CALIB_BYTES = 1 << 30
@@ -707,6 +708,8 @@ def main(argv=None):
     ap.add_argument("--prec", default="mixed", choices=["fp32", "mixed", "fp64"])
     ap.add_argument("--no-compare", action="store_true", help="mixed: do not time the FP32 path beside it")
     ap.add_argument("--op", default="plain", choices=["plain", "curl"])
+    ap.add_argument("--bc", default="0,0,0", choices=["0,0,0", "1,1,0", "0,0,1", "1,1,1"],
+                    help="boundary tuple of the default workload (0 unbounded, 1 periodic)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-host", action="store_true", help="skip the host-pointer API end-to-end timing")
@@ -744,6 +747,8 @@ def main(argv=None):
     import poissonsolver_amd as P
 
     n = a.n
+    bc = tuple(int(v) for v in a.bc.split(","))
+    uuu = bc == (0, 0, 0)
     b, h = gaussian_source(n)
     cin = 3 if a.op == "curl" else 1
     cout = 3 if a.op == "curl" else 1
@@ -754,7 +759,7 @@ def main(argv=None):
         """Time K solves of one precision (inputs resident in HBM); returns
         (per-pass ms, wall s, solver, z kernel family)."""
         s = P.poisson_solver(codes[prec])
-        s.setup3d((n,) * 3, (0, 0, 0), (h,) * 3, 2)
+        s.setup3d((n,) * 3, bc, (h,) * 3, 2)
         s.set_return_curl(a.op == "curl")
         s.push(*srcs)
         if s.last_status:
@@ -775,7 +780,7 @@ def main(argv=None):
         s.pull(*lhs)
         if ref is None:
             r = P.poisson_solver(P.PS_FP64)
-            r.setup3d((n,) * 3, (0, 0, 0), (h,) * 3, 2)
+            r.setup3d((n,) * 3, bc, (h,) * 3, 2)
             r.set_return_curl(a.op == "curl")
             r.push(*srcs)
             r.solve3d()
@@ -791,7 +796,7 @@ def main(argv=None):
         return 1
     ms = prof["solve"]
     eb = 8 if a.prec == "fp64" else 4
-    B = alg_bytes(n, cin, cout, eb)
+    B = alg_bytes_grid((n, n, n), bc, cin, cout, eb)
     dom = "z"
     achieved = B[dom] / (prof[dom] * 1e-3) / 1e9
     dtype = {"fp64": "f64", "fp32": "f32",
@@ -810,9 +815,14 @@ def main(argv=None):
         "dtype": dtype,
         "data": "synthetic",
         "config": {
-            "workload": "%d^3 fully unbounded %s Poisson solve, FFT domain doubled to %d^3, reg order 2, "
-                        "Gaussian source, %s" % (n, "vector curl" if a.op == "curl" else "scalar", 2 * n, a.prec),
-            "ncell": [n, n, n], "bc": [0, 0, 0], "reg_order": 2, "op": a.op, "precision": a.prec,
+            "workload": ("%d^3 fully unbounded %s Poisson solve, FFT domain doubled to %d^3, reg order 2, "
+                         "Gaussian source, %s" % (n, "vector curl" if a.op == "curl" else "scalar", 2 * n, a.prec))
+                        if uuu else
+                        ("%d^3 %s %s Poisson solve (bc %s: FFT lengths %s), reg order 2, Gaussian source, %s"
+                         % (n, {(1, 1, 0): "x,y periodic z unbounded", (0, 0, 1): "x,y unbounded z periodic",
+                                (1, 1, 1): "fully periodic"}[bc], "vector curl" if a.op == "curl" else "scalar",
+                            a.bc, "x".join(str(2 * n if v == 0 else n) for v in bc), a.prec)),
+            "ncell": [n, n, n], "bc": list(bc), "reg_order": 2, "op": a.op, "precision": a.prec,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -830,7 +840,7 @@ def main(argv=None):
     }
     if not (a.no_traffic or a.pmc_child):
         child = ["--pmc-child", "--no-traffic", "--no-cpu", "--no-check", "--no-compare", "--steps", "2",
-                 "--warmup", "1", "--n", str(n), "--prec", a.prec, "--op", a.op]
+                 "--warmup", "1", "--n", str(n), "--prec", a.prec, "--op", a.op, "--bc", a.bc]
         nc = 3 if a.op == "curl" else 1
         traffic, detail = measure_traffic(child, lambda name: is_zpass_kernel(name, nc))
         out["roofline"]["traffic"] = traffic
@@ -843,7 +853,7 @@ def main(argv=None):
         else:
             lhs = [np.empty_like(b) for _ in range(cout)]
             s.pull(*lhs)
-        if a.op == "plain":
+        if a.op == "plain" and uuu:
             x = -1.0 + (np.arange(n) + 0.5) * h
             r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
             rr = np.sqrt(r2)
@@ -883,7 +893,7 @@ def main(argv=None):
             out["z_speedup_vs_fp32"] = p32[dom] / prof[dom]
         s32.close()
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n, cin, a.op)
+        out["cpu_baseline"] = cpu_baseline(n, cin, a.op, bc)
     print(json.dumps(out), flush=True)
     return 0
 
