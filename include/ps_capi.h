@@ -212,7 +212,8 @@ int ps_zpass_variant(const ps_ctx* ctx);
 typedef struct ps_tuning {
     int zblk;         /* z blocking of the y/z-transposed spectrum: log2 block > 0,
                          0 = [z][ky][kx], -1 = [ky][z][kx], -2 = planner: 16-plane blocks,
-                         fewer for short z pencils, none from 1024 planes (PS_TUNE_ZBLK) */
+                         fewer for short z pencils, 32-plane blocks from 1024 planes, none
+                         for the matrix-core z pass of 512 planes (PS_TUNE_ZBLK) */
     int tile_x;       /* row pairs per workgroup, x passes; 0 = planner (PS_TUNE_PX) */
     int tile_y;       /* columns per workgroup, y passes; 0 = planner (PS_TUNE_BY) */
     int tile_z;       /* columns per workgroup, z pass; 0 = planner (PS_TUNE_BZ) */

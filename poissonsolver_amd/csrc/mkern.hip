@@ -104,12 +104,44 @@ hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s)
 
 bool zmfma_supported(int L, int half, int op)
 {
-    return L == MfmaZ::L && half &&
+    return (L == MfmaZ::L || L == 2048) && half &&
            (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF || op == OP_POTGRAD);
+}
+
+template <int OPK, int P> static hipError_t launch_zmfma2048_p(PencilArgs<float> a, hipStream_t s)
+{
+    using M = MfmaZ2048<P>;
+    const size_t shm = M::lds_bytes();
+    hipError_t e = allow_lds(k_zmfma2048<OPK, P>, shm);
+    if (e != hipSuccess) return e;
+    a.ncoltile = (a.ncol + M::P - 1) / M::P;
+    const long nwork = (long)a.ncoltile * a.nslab * a.ncomp;
+    hipLaunchKernelGGL((k_zmfma2048<OPK, P>), dim3((unsigned)nwork), dim3(64 * M::NW), shm, s, a);
+    return hipGetLastError();
+}
+
+// persistent (tuning mfma 2) selects the 8-pencil workgroups that hold the
+// pencil for both residues; otherwise 4-pencil workgroups that reload it
+template <int OPK> static hipError_t launch_zmfma2048_op(PencilArgs<float> a, int p8, hipStream_t s)
+{
+    return p8 ? launch_zmfma2048_p<OPK, 8>(a, s) : launch_zmfma2048_p<OPK, 4>(a, s);
 }
 
 hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t s)
 {
+    if (a.plan.L == 2048) {
+        // two 1024-point residues per pencil
+        if (a.n_in != 1024 || a.n_out != 1024) return hipErrorInvalidValue;
+        const int p8 = persistent ? 1 : 0;
+        switch (a.op) {
+        case OP_NONE: return launch_zmfma2048_op<OP_NONE>(a, p8, s);
+        case OP_PLAIN: return launch_zmfma2048_op<OP_PLAIN>(a, p8, s);
+        case OP_GRAD: return launch_zmfma2048_op<OP_GRAD>(a, p8, s);
+        case OP_RHSF: return launch_zmfma2048_op<OP_RHSF>(a, p8, s);
+        case OP_POTGRAD: return launch_zmfma2048_op<OP_POTGRAD>(a, p8, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (a.plan.L != MfmaZ::L || a.n_in != MfmaZ::L / 2 || a.n_out != MfmaZ::L / 2) return hipErrorInvalidValue;
     switch (a.op) {
     case OP_NONE: return launch_zmfma_op<OP_NONE>(a, persistent, s);

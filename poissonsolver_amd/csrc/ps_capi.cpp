@@ -694,11 +694,15 @@ template <typename T> int setup_impl(ps_ctx* c)
     // plane), zb = -1 is [ky][z][kx] (z pencils contiguous, y pass strided).
     int zb = tn.zblk;
     if (zb == -2) {
-        // 16-plane blocks (measured best at 512^3), fewer when the specialised
-        // z kernels' threads per pencil (Mz / E) are fewer than 16: a block
-        // must not split a thread's positions; from 1024 planes the plain
-        // [z][ky][kx] layout (z pass of 1024^3 26.8 -> 25.8 ms)
-        zb = (nz >= 1024) ? 0 : (nz >= 32) ? 4 : 0;
+        // 16-plane blocks (measured best at 512^3 on the FP32 kernels), fewer
+        // when the specialised z kernels' threads per pencil (Mz / E) are fewer
+        // than 16: a block must not split a thread's positions.  From 1024
+        // planes 32-plane blocks (1024^3: z pass 25.8 -> 24.9 ms on the FP32
+        // two-residue kernel, 26.0 -> 22.6 ms on the matrix cores; y + y^-1
+        // 15.3 -> 14.0 ms against the plain layout).  The matrix-core z pass
+        // of 512^3 takes the plain [z][ky][kx] layout (solve 4.34 -> 4.27 ms).
+        const bool mf = c->prec == PS_MIXED && tn.mfma && zmfma_supported(c->M[2], c->bc[2] == 0, OP_PLAIN);
+        zb = (nz >= 1024) ? 5 : (mf && !c->dim2) ? 0 : (nz >= 32) ? 4 : 0;
         int E1 = 0, B1 = 0, P1 = 0;
         if (zb > 0 && tn.specialised && static_policy(c->M[2], real_bytes(c), 1, &E1, &B1, &P1))
             while (zb > 0 && (c->M[2] / E1) % (1 << zb) != 0) --zb;
@@ -796,11 +800,13 @@ template <typename T> int setup_impl(ps_ctx* c)
             c->zwave = 0;
     }
     // matrix-core z pass (PS_MIXED): its lanes read Gk along kx; positions
-    // z0 + 32 u must be linear in u (z blocks of at most 32 planes, or plain)
-    // 32-bit byte offsets: one component of A2 and the Gk octant below 4 GB
+    // z0 + 32 u must be linear in u (z blocks of at most 32 planes, or plain).
+    // The 1024-point kernel uses 32-bit byte offsets: one component of A2 and
+    // the Gk octant below 4 GB; the 2048-point (two-residue) kernel 64-bit ones
     const long zcomp = c->dim2 ? c->A1c : c->A2c;
+    const bool off32 = zcomp * 8 < (1L << 32) && (long)c->O[0] * c->O[1] * c->O[2] * 4 < (1L << 32);
     c->zmfma = c->prec == PS_MIXED && tn.mfma && zmfma_supported(c->M[2], c->bc[2] == 0, OP_PLAIN) &&
-               c->zb <= 5 && zcomp * 8 < (1L << 32) && (long)c->O[0] * c->O[1] * c->O[2] * 4 < (1L << 32);
+               c->zb <= 5 && (c->M[2] == 2048 || off32);
     // two-residue z kernel: FP32 pencils of 2048 holding nz = 1024 unbounded
     // cells run as two 1024-point wavefront transforms (zwave = 1; zwave = 2
     // keeps one pencil of 2048 per wavefront with 32 elements per thread)

@@ -297,6 +297,24 @@ __device__ __forceinline__ MfTile mf_tile(const PencilArgs<float>& a, long lin, 
     return t;
 }
 
+// Multiplier of a derivative or potential quantity at one spectrum point in
+// single precision: kzw = 2 pi dk_z times the signed index of kz (its
+// Hermitian part kzh is 0 at the Nyquist index), kp2 = kx^2 + ky^2.
+template <int OPK>
+__device__ __forceinline__ float mf_kmul(const PencilArgs<float>& a, const TileK<float>& tk, int comp, int qn,
+                                         float kzw, float kzh, float kp2, float g)
+{
+    if constexpr (OPK == OP_GRAD) {
+        return kcomp<float>(a.kmap, comp, tk.kxh, tk.kyh, kzh) * g;
+    } else {
+        if (qn == 0) return g;
+        if (qn == 1) return tk.kxh * g;
+        if (qn == 2) return tk.kyh * g;
+        if (qn == 3) return kzh * g;
+        return -(kp2 + kzw * kzw) * g;
+    }
+}
+
 // k-space multipliers of the eight spectrum points kz = bp + 32 q' (q' = 4q + e
 // and 16 + 4q + e) that lane group q holds for butterfly bp, for the
 // one-component operators; computed before the DFT stages (with the input
@@ -325,13 +343,12 @@ __device__ __forceinline__ void mf_kfactors(float (&f)[8], const PencilArgs<floa
                 const float g = tk.grow ? *reinterpret_cast<const float*>(reinterpret_cast<const char*>(tk.grow) + go) : 0.0f;
                 if constexpr (OPK == OP_PLAIN) {
                     v = g;
-                } else if constexpr (OPK == OP_GRAD) {
-                    const float kzh = (2 * kz == a.Mz) ? 0.0f : (float)kappa_full(kz, a.Mz, a.dk[2]);
-                    v = kcomp<float>(a.kmap, comp, tk.kxh, tk.kyh, kzh) * g;
                 } else {
-                    const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
-                    const float kzh = (2 * kz == a.Mz) ? 0.0f : (float)kzf;
-                    v = pot_factor<float>(pot_quantity(a.kmap, comp), tk, kzh, kzf, g);
+                    const float kzw = (float)((2 * kz < a.Mz) ? kz : kz - a.Mz) * (float)(2.0 * kPi * a.dk[2]);
+                    const float kzh = (2 * kz == a.Mz) ? 0.0f : kzw;
+                    const float kp2 = (float)(tk.kxf * tk.kxf + tk.kyf * tk.kyf);
+                    v = mf_kmul<OPK>(a, tk, comp, (OPK == OP_POTGRAD) ? pot_quantity(a.kmap, comp) : 0, kzw, kzh,
+                                     kp2, g);
                 }
             } else {
                 const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
@@ -544,6 +561,299 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
 #pragma unroll
             for (int e = 0; e < 4; ++e)
                 *reinterpret_cast<float2*>(obase + (o0 + (unsigned)e * os32)) = float2{d.r0[e] * inv3, -d.i0[e] * inv3};
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transform length 2048 (Mz of a 1024-cell unbounded z direction): the
+// zero-padded 2048-point transform of a pencil x (1024 points) splits into
+// two 1024-point residues,
+//   X[2k']     = sum_n W1024^{n k'} x[n]                 (residue 0)
+//   X[2k' + 1] = sum_n W1024^{n k'} (W2048^n x[n])       (residue 1)
+// and the truncated inverse into
+//   y[n] = IDFT1024(Z_0)[n] + W2048^{-n} IDFT1024(Z_1)[n],   n < 1024,
+// so each residue runs the four DFT-32 stages of the 1024-point kernel, now
+// with 32 inputs in forward stage 1 and 32 outputs in inverse stage 2.  The
+// pencil data is loaded once and held in registers for both residues; the
+// residue-0 result is held until residue 1 adds to it.  One workgroup of 8
+// pencils per CU (2 wavefronts per SIMD, up to 256 registers); offsets are
+// 64-bit (a component of a 1024^3 grid's spectrum exceeds 4 GB).
+// ---------------------------------------------------------------------------
+// P pencils per workgroup, P wavefronts: an MFMA column is (pencil
+// p = j % P, butterfly b = (16 / P) blk + j / P).  P = 8 holds the pencil in
+// registers for both residues; P = 4 reloads it for residue 1 (fewer
+// registers, three workgroups per CU).
+template <int P_> struct MfmaZ2048 {
+    static constexpr int L = 2048, LH = 1024;
+    static constexpr int P = P_, NW = P_, NB = 16 / P_;
+    static constexpr bool HOLDX = P_ == 8;
+    static constexpr int RS = 33;
+    static constexpr int PS = 32 * 33 + (P_ == 8 ? 2 : 4);   // 2 or 4 (mod 16): p PS + bsub distinct
+    static constexpr size_t lds_bytes()
+    {
+        return (size_t)P * PS * 8 + 2 * (size_t)LH * 8 + 3 * P * 4;
+    }
+};
+
+// k-space multipliers of residue r: spectrum points kz = 2 (bp + 32 q') + r,
+// q' = 4q + e + 16h; Gk rows kzo = min(kz, 2048 - kz)
+template <int OPK>
+__device__ __forceinline__ void mf2_kfactors(float (&f)[8], const PencilArgs<float>& a, const TileK<float>& tk,
+                                             int comp, int bp, int q, int r)
+{
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int kz = 2 * (bp + 32 * (4 * q + e + 16 * h)) + r;
+            const int kzo = (kz <= 1024) ? kz : 2048 - kz;
+            float v;
+            if constexpr (OPK == OP_NONE) {
+                v = 1.0f;
+            } else if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_POTGRAD) {
+                const float g = tk.grow ? tk.grow[(long)kzo * a.g_kz] : 0.0f;
+                if constexpr (OPK == OP_PLAIN) {
+                    v = g;
+                } else {
+                    const float kzw = (float)((2 * kz < a.Mz) ? kz : kz - a.Mz) * (float)(2.0 * kPi * a.dk[2]);
+                    const float kzh = (2 * kz == a.Mz) ? 0.0f : kzw;
+                    const float kp2 = (float)(tk.kxf * tk.kxf + tk.kyf * tk.kyf);
+                    v = mf_kmul<OPK>(a, tk, comp, (OPK == OP_POTGRAD) ? pot_quantity(a.kmap, comp) : 0, kzw, kzh,
+                                     kp2, g);
+                }
+            } else {
+                const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
+                double zf = a.invN;
+                if (a.reg > 0) {
+                    const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
+                    double sum = 0.0, term = 1.0;
+                    for (int n = 0; n < a.reg / 2; ++n) {
+                        sum += term;
+                        term = term * arg / (double)(n + 1);
+                    }
+                    zf *= sum * exp(-arg);
+                }
+                v = (float)zf;
+            }
+            f[4 * h + e] = v;
+        }
+    }
+}
+
+template <int OPK, int P_>
+__global__ void __launch_bounds__(64 * P_, P_ == 8 ? 2 : 3) k_zmfma2048(PencilArgs<float> a)
+{
+    using M = MfmaZ2048<P_>;
+    constexpr int P = M::P, RS = M::RS, PSTR = M::PS, LH = M::LH, NB = M::NB, NW = M::NW;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* sd = reinterpret_cast<float2*>(smem_raw);
+    float2* stw = sd + P * PSTR;            // W1024^j, j < 1024
+    float2* stw2 = stw + LH;                 // W2048^n, n < 1024
+    unsigned* smax = reinterpret_cast<unsigned*>(stw2 + LH);   // [input | inverse r=0 | inverse r=1][P]
+
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, l = tid & 63, q = l >> 4, j = l & 15;
+    const int p = j % P, bsub = j / P;
+    const int bp32 = (l ^ 32) << 2;
+
+    const long nblk = gridDim.x;
+    const long lin = a.xcd_remap ? xcd_remap(blockIdx.x, nblk) : (long)blockIdx.x;
+    const long ntile = (long)a.ncoltile * a.nslab;
+    const int comp = (int)(lin / ntile);
+    const long rem = lin - (long)comp * ntile;
+    const int slab0 = (int)(rem / a.ncoltile);
+    const int ct = (int)(rem - (long)slab0 * a.ncoltile);
+    const int slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    const int col = ct * P + p;
+    const bool cv = col < a.ncol;
+    const int colr = cv ? col : a.ncol - 1;
+    const bool mimag = mf_imag_multiplier<OPK>(a, comp);
+
+    // the plan's table holds W2048^j, j < 2048: W1024^j = W2048^{2j}
+    for (int i = tid; i < LH; i += 64 * NW) {
+        stw[i] = a.plan.tw[2 * i];
+        stw2[i] = a.plan.tw[i];
+    }
+    if (tid < 3 * P) smax[tid] = 0u;
+    const MfDft32 F = mf_dft32_frags(a.mfrag, l);
+
+    const float2* ib = reinterpret_cast<const float2*>(a.in) + (long)comp * a.ia.comp + (long)colr * a.ia.col +
+                       boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
+    float2* ob = reinterpret_cast<float2*>(a.out) + (long)comp * a.oa.comp + (long)col * a.oa.col +
+                 boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi);
+    const long is32 = mf_pos_stride32(a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi);
+    const long os32 = mf_pos_stride32(a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
+    float2* myp = sd + p * PSTR;
+    const TileK<float> tk = tile_k(a, col, slab, cv);
+    int bb[2];
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) bb[bi] = NB * (w + NW * bi) + bsub;
+
+    // ---- load x[b + 32 m], m < 32: the whole nonzero pencil ----
+    float2 x[2][2][4];
+    float mx = 0.0f;
+    auto load_x = [&]() {
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const float2* ip = ib + boff(bb[bi], a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) + (long)(4 * q) * is32;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[bi][h][e] = ip[(long)(e + 16 * h) * is32];
+        }
+    };
+    load_x();
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) mx = mf_absmax(x[bi][h][e], mx);
+    __syncthreads();                       // tables and scale slots initialised
+    atomicMax(&smax[p], __float_as_uint(mx));
+    __syncthreads();
+    // one input scale per pencil for both residues (|W2048^n| = 1)
+    float s0, inv0;
+    mf_scale(__uint_as_float(smax[p]), s0, inv0);
+
+    float2 y0[2][8];                       // residue-0 result at n = n2 + 32 n1
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        // residue 1 without the held pencil: read it again (the output is
+        // stored only after both residues, so in-place passes read the input)
+        if constexpr (!M::HOLDX)
+            if (r == 1) load_x();
+        float gk[2][8];
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) mf2_kfactors<OPK>(gk[bi], a, tk, comp, bb[bi], q, r);
+
+        // ---- forward stage 1: DFT over m (32 inputs), 32 outputs k1 per column (p, b) ----
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int b = bb[bi];
+            MfFrag B[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                float2 v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float2 xv = x[bi][h][e];
+                    // residue 1: x[n] W2048^n, n = b + 32 (4q + e + 16h)
+                    v[e] = r ? cmul(xv, stw2[b + 32 * (4 * q + e + 16 * h)]) : xv;
+                }
+                B[h] = mf_split4(v, s0, s0, a.fzero);
+            }
+            const MfAcc d = mf_dft32_rows32i<2>(F, B);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int k1 = 4 * q + e + 16 * h;
+                    const float2 yv = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                    myp[k1 * RS + b] = cmul(yv, stw[(b * k1) & (LH - 1)]);
+                }
+            }
+        }
+        __syncthreads();
+
+        // ---- forward stage 2, k-space operator, inverse stage 1 (registers) ----
+        float m3 = 0.0f;
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int bp = bb[bi];
+            float2* row = myp + bp * RS;
+            float2 v0[4], v1[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v0[e] = row[4 * q + e];
+                v1[e] = row[16 + 4 * q + e];
+            }
+            MfFrag B[2];
+            B[0] = mf_split4u(v0, a.fone);
+            B[1] = mf_split4u(v1, a.fone);
+            MfAcc d = mf_dft32_rows32i<2>(F, B);
+            float2 z0[4], z1[4];
+            float mz = 0.0f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
+                if (mimag) {
+                    z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
+                    z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};
+                } else {
+                    z0[e] = float2{d.r0[e] * m0, d.i0[e] * m0};
+                    z1[e] = float2{d.r1[e] * m1, d.i1[e] * m1};
+                }
+                mz = mf_absmax(z0[e], mf_absmax(z1[e], mz));
+            }
+            float s2, inv2;
+            mf_scale(mf_colmax(mz, bp32), s2, inv2);
+            B[0] = mf_split4(z0, s2, -s2, a.fzero);
+            B[1] = mf_split4(z1, s2, -s2, a.fzero);
+            d = mf_dft32_rows32i<2>(F, B);
+            const float f = inv0 * inv2;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int n2 = 4 * q + e + 16 * h;
+                    const float2 u = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                    const float2 t = cmul(u, stw[(n2 * bp) & (LH - 1)]);
+                    const float2 rr = float2{t.x * f, -t.y * f};
+                    row[n2] = rr;
+                    m3 = mf_absmax(rr, m3);
+                }
+            }
+        }
+        atomicMax(&smax[P + r * P + p], __float_as_uint(m3));
+        __syncthreads();
+
+        // ---- inverse stage 2: column (p, n2), inputs over b', 32 outputs n1 ----
+        float s3, inv3;
+        mf_scale(__uint_as_float(smax[P + r * P + p]), s3, inv3);
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            const int n2 = bb[bi];
+            float2 v0[4], v1[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                v0[e] = myp[(4 * q + e) * RS + n2];
+                v1[e] = myp[(16 + 4 * q + e) * RS + n2];
+            }
+            MfFrag B[2];
+            B[0] = mf_split4(v0, s3, -s3, a.fzero);
+            B[1] = mf_split4(v1, s3, -s3, a.fzero);
+            const MfAcc d = mf_dft32_rows32i<2>(F, B);
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float2 yv = h ? float2{d.r1[e] * inv3, -d.i1[e] * inv3}
+                                        : float2{d.r0[e] * inv3, -d.i0[e] * inv3};
+                    if (r == 0) {
+                        y0[bi][4 * h + e] = yv;
+                    } else {
+                        // y[n] = y0[n] + W2048^{-n} y1[n], n = n2 + 32 n1
+                        const int n = n2 + 32 * (4 * q + e + 16 * h);
+                        const float2 wv = stw2[n];
+                        const float2 t = float2{yv.x * wv.x + yv.y * wv.y, yv.y * wv.x - yv.x * wv.y};
+                        y0[bi][4 * h + e] = float2{y0[bi][4 * h + e].x + t.x, y0[bi][4 * h + e].y + t.y};
+                    }
+                }
+            }
+        }
+        // the exchange buffer is rewritten by the next residue's stage 1
+        if (r == 0) __syncthreads();
+    }
+    if (cv) {
+#pragma unroll
+        for (int bi = 0; bi < 2; ++bi) {
+            float2* op_ = ob + boff(bb[bi], a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) + (long)(4 * q) * os32;
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) op_[(long)(e + 16 * h) * os32] = y0[bi][4 * h + e];
         }
     }
 }

@@ -1,6 +1,7 @@
-"""Mixed-precision form (PS_MIXED) vs the oracle: the z pass of a 512-cell
-unbounded z direction (transform length 1024) runs on the matrix cores as
-FP16 hi/lo DFT-32 products with FP32 accumulation; every other pass is FP32.
+"""Mixed-precision form (PS_MIXED) vs the oracle: the z pass of a 512- or
+1024-cell unbounded z direction (transform length 1024, or 2048 as two
+1024-point residues) runs on the matrix cores as FP16 hi/lo DFT-32 products
+with FP32 accumulation; every other pass is FP32.
 Bar: rel-L2 <= 1e-6 against the FP64 oracle (the north-star tolerance of the
 reduced-precision form).  x and y stay small so the oracle finishes in
 seconds."""
@@ -129,3 +130,45 @@ def test_mixed_curl_with_reprojection_and_filtered_rhs():
     lg, rg = gpu_solve(n, bc, dx, 2, b, op="curl", reproject=True, prec="mixed", want_rhs=True)
     assert O.rel_l2(lg, lo) <= MIXED_TOL
     assert O.rel_l2(rg, ro) <= 2e-6
+
+
+CASES2048 = [
+    ((8, 6, 1024), (0, 0, 0), (0.1, 0.12, 1 / 512)),
+    ((12, 10, 1024), (1, 1, 0), (1 / 12, 0.1, 1 / 512)),
+]
+
+
+@pytest.mark.parametrize("case", CASES2048, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("reg", [0, 2, 10])
+def test_mixed_2048_plain_grad_filtered_rhs(case, reg):
+    """nz = 1024 unbounded: z transforms of 2048 as two 1024-point residues
+    on the matrix cores (plain, component-wise grad, filtered rhs)."""
+    n, bc, dx = case
+    announce("mixed 2048", n, bc, reg)
+    assert _variant(n, bc, dx, reg) == "matrix-core"
+    b = random_rhs(n, 61 + reg)
+    _compare(n, bc, dx, reg, "plain", [b[0]], want_rhs=True)
+    _compare(n, bc, dx, reg, "grad", b, want_rhs=True)
+
+
+def test_mixed_2048_potential_and_range():
+    """The potential extension's five quantities on the 2048-point
+    matrix-core kernel, and a source spanning 2^-40 .. 2^40 in magnitude."""
+    n, bc, dx = CASES2048[0]
+    rng = np.random.default_rng(67)
+    b = rng.standard_normal((n[2], n[1], n[0])) * np.exp2(rng.uniform(-40, 40, (n[2], 1, 1)))
+    s = P.poisson_solver(P.PS_MIXED)
+    s.setup3d(n, bc, dx, 2)
+    assert s.zpass_variant() == "matrix-core"
+    s.push(b)
+    s.solve3d_potential(("A", "dx", "dy", "dz", "lap"))
+    got = [np.empty_like(b) for _ in range(5)]
+    s.pull_potential(*got)
+    assert s.last_status == 0
+    s.close()
+    lo_a, _ = O.solve(n, bc, dx, 2, [b])
+    lo_g, _ = O.solve(n, bc, dx, 2, [b, b, b], op="grad")
+    lo_l, _ = O.solve(n, bc, dx, 2, [b], op="lap")
+    assert O.rel_l2(got[0], lo_a[0]) <= MIXED_TOL
+    assert O.rel_l2(got[1:4], lo_g) <= MIXED_TOL
+    assert O.rel_l2(got[4], lo_l[0]) <= MIXED_TOL
