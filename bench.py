@@ -77,6 +77,20 @@ def alg_bytes_grid(ncell, bc, cin, cout, eb):
     return dict(x=P1, y=P2, z=P3, y_inv=P4, x_inv=P5, solve=P1 + P2 + P3 + P4 + P5)
 
 
+def alg_bytes_potential(ncell, bc, outputs, eb):
+    """Byte model of the potential form (ps_solve3d_potential): one scalar
+    through the x and y passes; one z transform per distinct z multiplier (A,
+    which d/dx and d/dy share, d/dz and the Laplacian), each reading the one
+    spectrum and Gk once and writing its own; the y^-1 and x^-1 passes once
+    per output (d/dx and d/dy read A's z output, times i k on load)."""
+    one = alg_bytes_grid(ncell, bc, 1, 1, eb)
+    back = alg_bytes_grid(ncell, bc, 1, len(outputs), eb)
+    nzq = int(any(k in outputs for k in ("A", "dx", "dy"))) + int("dz" in outputs) + int("lap" in outputs)
+    d = dict(x=one["x"], y=one["y"], z=nzq * one["z"], y_inv=back["y_inv"], x_inv=back["x_inv"])
+    d["solve"] = sum(d.values())
+    return d
+
+
 def cpu_baseline(n_target, cin, op, bc=(0, 0, 0)):
     """The oracle (numpy FP64 restatement of the reference, one core) timed on
     a bounded sample: push+solve+pull at 128^3, scaled to n_target by
@@ -375,7 +389,8 @@ def bench_config(a):
             # there would not fit in HBM next to its work buffers)
             big = ncell[0] * ncell[1] * ncell[2] >= 512 ** 3 * 4
             if prec != "fp64":
-                groups = [pot]
+                # with the Laplacian, also the four outputs of the component-wise solves
+                groups = [pot] + ([tuple(k for k in pot if k != "lap")] if "lap" in pot else [])
             else:
                 groups = ([] if big else [tuple(k for k in pot if k != "lap")]) + ([("lap",)] if "lap" in pot else [])
             got = {}
@@ -391,9 +406,10 @@ def bench_config(a):
                         s.profile_potential(grp, a.warmup)
                     s.synchronize()
                     prof = s.profile_potential(grp, a.steps)
-                    per["potential"] = {"ms": prof["solve"], "outputs": list(grp),
-                                        "pass_ms": {k: prof[k] for k in ("x", "y", "z", "y_inv", "x_inv")},
-                                        "z_kernel": s.zpass_variant()}
+                    per["potential" if grp == pot else "potential_agrad"] = {
+                        "ms": prof["solve"], "outputs": list(grp),
+                        "pass_ms": {k: prof[k] for k in ("x", "y", "z", "y_inv", "x_inv")},
+                        "z_kernel": s.zpass_variant()}
                 else:
                     s.solve3d_potential(grp)
                 if s.last_status:
@@ -401,7 +417,8 @@ def bench_config(a):
                 arrs = {k: (np.empty(shape) if k in grp else None) for k in P.poisson_solver.POT_NAMES}
                 s.pull_potential(*[arrs[k] for k in P.poisson_solver.POT_NAMES])
                 for k in grp:
-                    got[k] = arrs[k][sub].copy()
+                    if k not in got:
+                        got[k] = arrs[k][sub].copy()
                 del arrs
                 s.close()
             outs[(prec, "potential")] = [got[k] for k in pot]
@@ -432,6 +449,8 @@ def bench_config(a):
         if pot is not None:
             key = "%s/potential" % prec
             out["solves"][key] = results[prec]["potential"]
+            if "potential_agrad" in results[prec]:
+                out["solves"][key + "_agrad"] = results[prec]["potential_agrad"]
             for k, x, y in zip(pot, outs[(prec, "potential")], outs[("fp64", "potential")]):
                 out["rel_l2_vs_fp64_gpu"]["%s/%s" % (key, k)] = _rel([x], [y])
     if pot is not None:
@@ -440,17 +459,22 @@ def bench_config(a):
             ref = outs[("fp64", "A")] + outs[("fp64", "gradA")]
             out["rel_l2_fp64_potential_vs_componentwise"] = _rel(outs[("fp64", "potential")][:4], ref)
         out["componentwise_ms"] = comp_ms
-        out["speedup_potential_vs_componentwise"] = comp_ms / step_ms
-        B = alg_bytes_grid(ncell, bc, 1, len(pot), eb)
+        agrad = results[line_prec].get("potential_agrad")
+        # the same four outputs (A, grad A) as the component-wise solves
+        out["speedup_potential_vs_componentwise"] = comp_ms / (agrad["ms"] if agrad else step_ms)
+        B = alg_bytes_potential(ncell, bc, pot, eb)
         Bc = sum(alg_bytes_grid(ncell, bc, ci, co, eb)["solve"] for ci, co in cin_out)
         pp = results[line_prec]["potential"]["pass_ms"]
         out["alg_bytes_per_step"] = B["solve"]
+        if agrad:
+            out["alg_bytes_agrad"] = alg_bytes_potential(ncell, bc, agrad["outputs"], eb)["solve"]
         out["alg_bytes_componentwise"] = Bc
         out["step_hbm_gbs"] = B["solve"] / (step_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": B["z"] / (pp["z"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": B["z"] / (pp["z"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
-                           "kernel": "z pass of the potential form (%s kernel: one spectrum in, %d quantities out)"
-                                     % (results[line_prec]["potential"]["z_kernel"], len(pot)),
+                           "kernel": "z pass of the potential form (%s kernel: one spectrum in, %d quantities "
+                                     "out through %d z transforms)"
+                                     % (results[line_prec]["potential"]["z_kernel"], len(pot), B["z"] // alg_bytes_grid(ncell, bc, 1, 1, eb)["z"]),
                            "alg_bytes_per_launch": B["z"], "avg_ms": pp["z"]}
     if len(precs) == 2:
         f32 = sum(v["ms"] for k, v in results["fp32"].items() if pot is None or k == "potential")

@@ -105,7 +105,7 @@ hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s)
 bool zmfma_supported(int L, int half, int op)
 {
     return (L == MfmaZ::L || L == 2048) && half &&
-           (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF || op == OP_POTGRAD);
+           (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF || op == OP_LAPL);
 }
 
 template <int OPK, int P> static hipError_t launch_zmfma2048_p(PencilArgs<float> a, hipStream_t s)
@@ -138,7 +138,7 @@ hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t 
         case OP_PLAIN: return launch_zmfma2048_op<OP_PLAIN>(a, p8, s);
         case OP_GRAD: return launch_zmfma2048_op<OP_GRAD>(a, p8, s);
         case OP_RHSF: return launch_zmfma2048_op<OP_RHSF>(a, p8, s);
-        case OP_POTGRAD: return launch_zmfma2048_op<OP_POTGRAD>(a, p8, s);
+        case OP_LAPL: return launch_zmfma2048_op<OP_LAPL>(a, p8, s);
         default: return hipErrorInvalidValue;
         }
     }
@@ -148,7 +148,7 @@ hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t 
     case OP_PLAIN: return launch_zmfma_op<OP_PLAIN>(a, persistent, s);
     case OP_GRAD: return launch_zmfma_op<OP_GRAD>(a, persistent, s);
     case OP_RHSF: return launch_zmfma_op<OP_RHSF>(a, persistent, s);
-    case OP_POTGRAD: return launch_zmfma_op<OP_POTGRAD>(a, persistent, s);
+    case OP_LAPL: return launch_zmfma_op<OP_LAPL>(a, persistent, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -597,31 +597,61 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         za.out = A1;
         if (nc == 1) za.ncomp = nout;
     }
-    bool mfma_done = false;
-    if constexpr (sizeof(T) == 4) {
-        if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, op)) {
-            CK(launch_zmfma(za, c->tune.mfma == 2, s));
-            mfma_done = true;
-        } else if (c->zmfma && nc == 3 && kmap == KMAP_3D && zmfma3_supported(zp.p.L, c->bc[2] == 0, op)) {
-            CK(launch_zmfma3(za, s));
-            mfma_done = true;
+    auto zlaunch = [&](const PencilArgs<T>& zz) -> int {
+        bool mfma_done = false;
+        if constexpr (sizeof(T) == 4) {
+            if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, zz.op)) {
+                CK(launch_zmfma(zz, c->tune.mfma == 2, s));
+                mfma_done = true;
+            } else if (c->zmfma && nc == 3 && zz.kmap == KMAP_3D && zmfma3_supported(zp.p.L, c->bc[2] == 0, zz.op)) {
+                CK(launch_zmfma3(zz, s));
+                mfma_done = true;
+            }
         }
-    }
-    if (nc == 2 && zp.stat) return fail(c, PS_EINVAL, "one-input two-output z pass needs the runtime-plan kernel");
-    if constexpr (sizeof(T) == 4) {
-        if (!mfma_done && c->zres && nc == 1) {
-            CK((s_launch_zres<float, 1024>(za, c->tw_half, s)));
-            mfma_done = true;
+        if (nc == 2 && zp.stat) return fail(c, PS_EINVAL, "one-input two-output z pass needs the runtime-plan kernel");
+        if constexpr (sizeof(T) == 4) {
+            if (!mfma_done && c->zres && nc == 1) {
+                CK((s_launch_zres<float, 1024>(zz, c->tw_half, s)));
+                mfma_done = true;
+            }
         }
+        if (mfma_done) {
+        } else if (zp.stat) CK(static_launch_pencil<T>(zp.p.L, zz, nc, SM_FUSED, c->bc[2] == 0, s));
+        else CK(launch_pencil<T>(zp.p.E, nc, zz, s));
+        return PS_OK;
+    };
+    // potential fan-out: output c (the c-th set bit of kmap) has A2 slot c
+    // (component 1 + c).  kx and ky are constant along a z pencil, so d/dx A
+    // and d/dy A are i kx and i ky times the z-pass spectrum of A: the z pass
+    // transforms A (OP_PLAIN), d/dz A (OP_GRAD on k_z) and the Laplacian
+    // (OP_LAPL) only, and the y^-1 pass of d/dx and d/dy reads A's slot and
+    // multiplies by i k on load (PencilArgs::ik_in).
+    const bool potfan = fan && op == OP_POTGRAD;
+    if ((op == OP_POTGRAD) != fan || (fan && (nin != 1 || nc != 1 || c->dim2)))
+        return fail(c, PS_EINVAL, "the potential fan-out needs one input, one z component and a 3D context");
+    int slot[5] = {-1, -1, -1, -1, -1}, aslot = -1;
+    char* A2o = static_cast<char*>(A2) + (size_t)c->A2c * cbytes;
+    if (potfan) {
+        for (int qn = 0, k = 0; qn < 5; ++qn) slot[qn] = ((kmap >> qn) & 1) ? k++ : -1;
+        aslot = slot[0] >= 0 ? slot[0] : slot[1] >= 0 ? slot[1] : slot[2];
+        for (int qn : {0, 3, 4}) {
+            const int sl = qn == 0 ? aslot : slot[qn];
+            if (sl < 0) continue;
+            PencilArgs<T> zq = za;
+            zq.op = qn == 0 ? OP_PLAIN : qn == 3 ? OP_GRAD : OP_LAPL;
+            zq.kmap = qn == 3 ? KMAP_Z : KMAP_3D;
+            zq.ncomp = 1;
+            zq.out = A2o + (size_t)sl * c->A2c * cbytes;
+            if (const int zr = zlaunch(zq)) return zr;
+        }
+    } else if (const int zr = zlaunch(za)) {
+        return zr;
     }
-    if (mfma_done) {
-    } else if (zp.stat) CK(static_launch_pencil<T>(zp.p.L, za, nc, SM_FUSED, c->bc[2] == 0, s));
-    else CK(launch_pencil<T>(zp.p.E, nc, za, s));
     if (ev) CK(hipEventRecord(ev[3], s));
 
     if (!c->dim2) {
     PencilArgs<T> yi = ya;
-    yi.in = fan ? static_cast<void*>(static_cast<char*>(A2) + (size_t)c->A2c * cbytes) : A2;
+    yi.in = fan ? static_cast<void*>(A2o) : A2;
     yi.out = A1;
     yi.ia = c->a2_ypass;
     yi.oa = addr_plain(c->A1c, (long)ny * c->Hp, c->Hp, 1);
@@ -636,8 +666,28 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     yi.ncomp = nout;
     yi.do_fwd = 0;
     yi.do_inv = 1;
-    if (c->py.stat) CK(static_launch_pencil<T>(c->py.p.L, yi, 1, SM_INV, c->bc[1] == 0, s));
-    else CK(launch_pencil<T>(c->py.p.E, 1, yi, s));
+    auto ylaunch = [&](const PencilArgs<T>& y) -> int {
+        if (c->py.stat) CK(static_launch_pencil<T>(c->py.p.L, y, 1, SM_INV, c->bc[1] == 0, s));
+        else CK(launch_pencil<T>(c->py.p.E, 1, y, s));
+        return PS_OK;
+    };
+    if (potfan) {
+        // one launch per output: d/dx and d/dy read A's slot
+        yi.Mx = c->M[0];
+        yi.My = c->M[1];
+        for (int d = 0; d < 2; ++d) yi.dk[d] = 1.0 / ((double)c->M[d] * c->dx[d]);
+        for (int qn = 0; qn < 5; ++qn) {
+            if (slot[qn] < 0) continue;
+            PencilArgs<T> y1 = yi;
+            y1.ncomp = 1;
+            y1.ik_in = (qn == 1) ? 1 : (qn == 2) ? 2 : 0;
+            y1.in = A2o + (size_t)(y1.ik_in ? aslot : slot[qn]) * c->A2c * cbytes;
+            y1.out = static_cast<char*>(yi.out) + (size_t)slot[qn] * yi.oa.comp * cbytes;
+            if (const int yr = ylaunch(y1)) return yr;
+        }
+    } else if (const int yr = ylaunch(yi)) {
+        return yr;
+    }
     }
     if (flags & PF_DST_SPEC) {
         // y^-1 output (3D) or the fused pass's A1 (2D) times Mx: the

@@ -21,12 +21,11 @@ enum PencilOp : int {
     OP_CURL = 4,      // l = Gk i k x r
     OP_RHSF = 5,      // r_c * zeta / N               (filtered rhs, no reprojection)
     OP_RHSF_PROJ = 6, // r_d*zeta/N - Gk (kh_d kh.r + kn_d kn.r)  (filtered + reprojected)
-    OP_POTGRAD = 7,   // one scalar r, output c = the c-th quantity whose bit is set in kmap:
-                      //   bit 0 Gk r, bits 1..3 i k_d Gk r (d = x, y, z), bit 4 -|k|^2 Gk r
+    OP_POTGRAD = 7,   // pipeline only: one scalar r, outputs the quantities whose bits are set in
+                      //   kmap (bit 0 Gk r, bits 1..3 i k_d Gk r, bit 4 -|k|^2 Gk r); its z pass
+                      //   runs OP_PLAIN, OP_GRAD (KMAP_Z) and OP_LAPL
+    OP_LAPL = 8,      // l = -|k|^2 Gk r               (|k| with the Nyquist wavenumbers kept)
 };
-
-// Quantities of OP_POTGRAD (bits of PencilArgs::kmap).
-enum PotQuantity : int { POT_A = 1, POT_DX = 2, POT_DY = 4, POT_DZ = 8, POT_LAP = 16 };
 
 // Wavenumber axis of each vector component for OP_GRAD and OP_DIV.  A 2D
 // solve runs as a 3D one of shape (nx, 1, ny): its y direction is the pass
@@ -36,6 +35,7 @@ enum KMap : int {
     KMAP_2D_GRAD = 1,  // component 0 <-> k_x, 1 <-> k_z (both read the same scalar)
     KMAP_2D_CURL = 2,  // component 0 <-> +k_z, 1 <-> -k_x (curl of a scalar along z)
     KMAP_2D_DIV = 3,   // component 0 <-> k_x, 1 <-> k_z, component 2 unused
+    KMAP_Z = 4,        // component 0 <-> k_z (the z derivative of a scalar)
 };
 
 enum LoadMode : int { LOAD_COMPLEX = 0, LOAD_REAL_EVEN = 1 };
@@ -82,6 +82,10 @@ template <typename T> struct PencilArgs {
     int reg;         // regularisation order (even), 0 = none
     double invN;     // 1/(Mx My Mz)
     int col0;        // kx of the launch's column 0 (0: the launch covers every column)
+    // y^-1 pass of one component: multiply the loaded spectrum by i k (its
+    // Hermitian part, 0 at the Nyquist index) before the transform; 1: k = kx
+    // of the column (Mx, dk[0]), 2: k = ky of the position (My, dk[1]); 0: none
+    int ik_in;
     // matrix-core z pass: per-lane fp16 hi/lo fragments of the 32-point DFT
     // matrix (64 lanes x 8 fragments x 8 halves), built once at setup
     const void* mfrag;

@@ -73,6 +73,26 @@ __device__ __forceinline__ TileK<T> tile_k(const PencilArgs<T>& a, int col, int 
     return k;
 }
 
+// PencilArgs::ik_in on one thread's loaded values v[u] (positions t + TP u)
+template <typename T, int E>
+__device__ __forceinline__ void apply_ik_in(cplx<T> (&v)[E], const PencilArgs<T>& a, int col, int t, int TP)
+{
+    if (a.ik_in == 0) return;
+    if (a.ik_in == 1) {
+        const int kx = col + a.col0;
+        const T k = (2 * kx == a.Mx) ? T(0) : (T)kappa_full(kx, a.Mx, a.dk[0]);
+#pragma unroll
+        for (int u = 0; u < E; ++u) v[u] = cplx<T>{-k * v[u].y, k * v[u].x};
+    } else {
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const int ky = t + TP * u;
+            const T k = (2 * ky == a.My) ? T(0) : (T)kappa_full(ky, a.My, a.dk[1]);
+            v[u] = cplx<T>{-k * v[u].y, k * v[u].x};
+        }
+    }
+}
+
 // Signed wavenumber of vector component c under map `kmap` (KMap), from the
 // axis wavenumbers kx, ky, kz; 0 for a component the map does not use.
 template <typename T>
@@ -80,32 +100,8 @@ __device__ __forceinline__ T kcomp(int kmap, int c, T kx, T ky, T kz)
 {
     if (kmap == KMAP_3D) return (c == 0) ? kx : (c == 1) ? ky : kz;
     if (kmap == KMAP_2D_CURL) return (c == 0) ? kz : (c == 1) ? -kx : T(0);
+    if (kmap == KMAP_Z) return (c == 0) ? kz : T(0);
     return (c == 0) ? kx : (c == 1) ? kz : T(0);
-}
-
-// OP_POTGRAD: quantity of output component c, the c-th set bit of `mask`
-// (0 A, 1..3 d/dx, d/dy, d/dz, 4 Laplacian)
-__device__ __forceinline__ int pot_quantity(int mask, int c)
-{
-    for (int qn = 0; qn < 5; ++qn) {
-        if (!((mask >> qn) & 1)) continue;
-        if (c == 0) return qn;
-        --c;
-    }
-    return 0;
-}
-
-// OP_POTGRAD multiplier of quantity qn at one spectrum point: real m (A and
-// the Laplacian: Gk, -|k|^2 Gk) or imaginary i m (derivatives: i k_d Gk with
-// the Hermitian part of k_d, as OP_GRAD); kzh Hermitian, kzf full wavenumber
-template <typename T>
-__device__ __forceinline__ T pot_factor(int qn, const TileK<T>& tk, T kzh, double kzf, T g)
-{
-    if (qn == 0) return g;
-    if (qn == 1) return tk.kxh * g;
-    if (qn == 2) return tk.kyh * g;
-    if (qn == 3) return kzh * g;
-    return (T)(-(tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf)) * g;
 }
 
 // k-space operator of the fused z pass on one thread's E spectrum values
@@ -140,11 +136,9 @@ __device__ __forceinline__ void kspace_op(cplx<T> (&w)[E], const cplx<T> (&v)[NC
             const T kd = kcomp<T>(a.kmap, comp, tk.kxh, tk.kyh, kzh);
             const C s = v[0][u];
             r = C{-kd * g * s.y, kd * g * s.x};
-        } else if (op == OP_POTGRAD) {
-            const int qn = pot_quantity(a.kmap, comp);
-            const T m = pot_factor<T>(qn, tk, kzh, kzf, g);
-            const C s = v[0][u];
-            r = (qn >= 1 && qn <= 3) ? C{-m * s.y, m * s.x} : C{m * s.x, m * s.y};
+        } else if (op == OP_LAPL) {
+            const T m = (T)(-(tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf)) * g;
+            r = cscale(v[0][u], m);
         } else if (op == OP_DIV || op == OP_CURL) {
             const C sx = v[0][u], sy = v[vi][u], sz = v[wi][u];
             C q;
@@ -202,7 +196,6 @@ __device__ __forceinline__ void kspace_op_inplace(cplx<T> (&v)[E], const PencilA
     using C = cplx<T>;
     const int op = a.op;
     if (op == OP_NONE) return;
-    const int qn = (op == OP_POTGRAD) ? pot_quantity(a.kmap, comp) : 0;
 #pragma unroll
     for (int u = 0; u < E; ++u) {
         const int kz = kzm * (t + TP * u) + kza;
@@ -210,12 +203,9 @@ __device__ __forceinline__ void kspace_op_inplace(cplx<T> (&v)[E], const PencilA
         const T g = tk.grow ? tk.grow[kzo * a.g_kz] : T(0);
         if (op == OP_PLAIN) {
             v[u] = cscale(v[u], g);
-        } else if (op == OP_POTGRAD) {
+        } else if (op == OP_LAPL) {
             const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
-            const T kzh = (2 * kz == a.Mz) ? T(0) : (T)kzf;
-            const T m = pot_factor<T>(qn, tk, kzh, kzf, g);
-            const C s = v[u];
-            v[u] = (qn >= 1 && qn <= 3) ? C{-m * s.y, m * s.x} : C{m * s.x, m * s.y};
+            v[u] = cscale(v[u], (T)(-(tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf)) * g);
         } else if (op == OP_GRAD) {
             const T kzh = (2 * kz == a.Mz) ? T(0) : (T)kappa_full(kz, a.Mz, a.dk[2]);
             const T kd = kcomp<T>(a.kmap, comp, tk.kxh, tk.kyh, kzh);
@@ -291,6 +281,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_pencil(PencilArgs<T> a)
     if (a.do_fwd) {
 #pragma unroll
         for (int c = 0; c < NV; ++c) fft_pencil<E, -1, T>(v[c], a.plan, my, B, t);
+    } else if constexpr (NC == 1) {
+        apply_ik_in<T, E>(v[0], a, col, t, TP);
     }
 
     const int op = a.op;

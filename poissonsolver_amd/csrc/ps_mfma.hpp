@@ -297,22 +297,64 @@ __device__ __forceinline__ MfTile mf_tile(const PencilArgs<float>& a, long lin, 
     return t;
 }
 
-// Multiplier of a derivative or potential quantity at one spectrum point in
-// single precision: kzw = 2 pi dk_z times the signed index of kz (its
-// Hermitian part kzh is 0 at the Nyquist index), kp2 = kx^2 + ky^2.
+// The one-component multipliers from the loaded Gk value g, with c0, c1
+// fixed per pencil, so that the per-point work is branch-free:
+//   plain  g
+//   grad   (c0 + c1 kzh) g, applied as i k_d Gk: c0 = kx or ky, or c1 = 1 for kz
+//   Laplacian (OP_LAPL)  (c0 - kz^2) g, c0 = -(kx^2 + ky^2)
+// (Hermitian parts kzh: the Nyquist k_d is 0; kz^2 keeps the Nyquist kz.)  A
+// column beyond the grid reads Gk row 0 instead: every load is in bounds and
+// unconditional, and the column's results are never stored.
+struct MfKCoef {
+    float c0, c1, kst;
+    const float* grow;
+};
+
 template <int OPK>
-__device__ __forceinline__ float mf_kmul(const PencilArgs<float>& a, const TileK<float>& tk, int comp, int qn,
-                                         float kzw, float kzh, float kp2, float g)
+__device__ __forceinline__ MfKCoef mf_kcoef(const PencilArgs<float>& a, const TileK<float>& tk, int comp)
 {
+    MfKCoef k{0.0f, 0.0f, (float)(2.0 * kPi * a.dk[2]), tk.grow ? tk.grow : a.G};
     if constexpr (OPK == OP_GRAD) {
-        return kcomp<float>(a.kmap, comp, tk.kxh, tk.kyh, kzh) * g;
-    } else {
-        if (qn == 0) return g;
-        if (qn == 1) return tk.kxh * g;
-        if (qn == 2) return tk.kyh * g;
-        if (qn == 3) return kzh * g;
-        return -(kp2 + kzw * kzw) * g;
+        k.c0 = kcomp<float>(a.kmap, comp, tk.kxh, tk.kyh, 0.0f);
+        k.c1 = kcomp<float>(a.kmap, comp, 0.0f, 0.0f, 1.0f);
+    } else if constexpr (OPK == OP_LAPL) {
+        k.c0 = -(float)(tk.kxf * tk.kxf + tk.kyf * tk.kyf);
     }
+    return k;
+}
+
+// multiplier at spectrum index kz from the loaded Gk value g
+template <int OPK>
+__device__ __forceinline__ float mf_kval(const PencilArgs<float>& a, const MfKCoef& k, int kz, float g)
+{
+    if constexpr (OPK == OP_PLAIN) {
+        return g;
+    } else {
+        const float kzw = (float)((2 * kz < a.Mz) ? kz : kz - a.Mz) * k.kst;
+        if constexpr (OPK == OP_GRAD) {
+            const float kzh = (2 * kz == a.Mz) ? 0.0f : kzw;
+            return (k.c0 + k.c1 * kzh) * g;
+        } else {
+            return (k.c0 - kzw * kzw) * g;
+        }
+    }
+}
+
+// filtered-rhs multiplier zeta(|k|) / N at spectrum index kz
+__device__ __forceinline__ float mf_rhsf(const PencilArgs<float>& a, const TileK<float>& tk, int kz)
+{
+    const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
+    double zf = a.invN;
+    if (a.reg > 0) {
+        const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
+        double sum = 0.0, term = 1.0;
+        for (int n = 0; n < a.reg / 2; ++n) {
+            sum += term;
+            term = term * arg / (double)(n + 1);
+        }
+        zf *= sum * exp(-arg);
+    }
+    return (float)zf;
 }
 
 // k-space multipliers of the eight spectrum points kz = bp + 32 q' (q' = 4q + e
@@ -329,56 +371,34 @@ template <int OPK>
 __device__ __forceinline__ void mf_kfactors(float (&f)[8], const PencilArgs<float>& a, const TileK<float>& tk,
                                             int comp, int bp, int q, int goff0, int goff1, int gstep)
 {
+    if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_LAPL) {
+        const MfKCoef k = mf_kcoef<OPK>(a, tk, comp);
+        float g[8];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int kz = bp + 32 * (4 * q + e + 16 * h);
-            float v;
-            if constexpr (OPK == OP_NONE) {
-                v = 1.0f;
-            } else if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_POTGRAD) {
+            for (int e = 0; e < 4; ++e) {
                 const unsigned go = h ? (unsigned)goff1 - (unsigned)e * (unsigned)gstep
                                       : (unsigned)goff0 + (unsigned)e * (unsigned)gstep;
-                const float g = tk.grow ? *reinterpret_cast<const float*>(reinterpret_cast<const char*>(tk.grow) + go) : 0.0f;
-                if constexpr (OPK == OP_PLAIN) {
-                    v = g;
-                } else {
-                    const float kzw = (float)((2 * kz < a.Mz) ? kz : kz - a.Mz) * (float)(2.0 * kPi * a.dk[2]);
-                    const float kzh = (2 * kz == a.Mz) ? 0.0f : kzw;
-                    const float kp2 = (float)(tk.kxf * tk.kxf + tk.kyf * tk.kyf);
-                    v = mf_kmul<OPK>(a, tk, comp, (OPK == OP_POTGRAD) ? pot_quantity(a.kmap, comp) : 0, kzw, kzh,
-                                     kp2, g);
-                }
-            } else {
-                const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
-                double zf = a.invN;
-                if (a.reg > 0) {
-                    const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
-                    double sum = 0.0, term = 1.0;
-                    for (int n = 0; n < a.reg / 2; ++n) {
-                        sum += term;
-                        term = term * arg / (double)(n + 1);
-                    }
-                    zf *= sum * exp(-arg);
-                }
-                v = (float)zf;
+                g[4 * h + e] = *reinterpret_cast<const float*>(reinterpret_cast<const char*>(k.grow) + go);
             }
-            f[4 * h + e] = v;
-        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) f[4 * h + e] = mf_kval<OPK>(a, k, bp + 32 * (4 * q + e + 16 * h), g[4 * h + e]);
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                f[4 * h + e] = (OPK == OP_NONE) ? 1.0f : mf_rhsf(a, tk, bp + 32 * (4 * q + e + 16 * h));
     }
 }
 
-// whether the k-space multiplier of component `comp` is imaginary (i k_d Gk:
-// grad, and the derivative quantities of OP_POTGRAD); uniform per tile
-template <int OPK> __device__ __forceinline__ bool mf_imag_multiplier(const PencilArgs<float>& a, int comp)
+// whether the k-space multiplier is imaginary (i k_d Gk: grad)
+template <int OPK> constexpr bool mf_imag_multiplier()
 {
-    if constexpr (OPK == OP_GRAD) return true;
-    if constexpr (OPK == OP_POTGRAD) {
-        const int qn = pot_quantity(a.kmap, comp);
-        return qn >= 1 && qn <= 3;
-    }
-    return false;
+    return OPK == OP_GRAD;
 }
 
 // One tile per workgroup (4 wavefronts per SIMD, two workgroups per CU).
@@ -410,7 +430,7 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
     // columns past the grid repeat the last one: loads stay in bounds, their
     // results are never stored, and no pencil's data mixes with another's
     const int colr = cv ? col : a.ncol - 1;
-    const bool mimag = mf_imag_multiplier<OPK>(a, comp);
+    constexpr bool mimag = mf_imag_multiplier<OPK>();
 
     for (int i = tid; i < L; i += 512) stw[i] = a.plan.tw[i];
     if (tid < 2 * P) smax[tid] = 0u;
@@ -504,7 +524,7 @@ __global__ void __launch_bounds__(512, 4) k_zmfma1024(PencilArgs<float> a)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
-            if (mimag) {
+            if constexpr (mimag) {
                 // i k_d Gk: purely imaginary
                 z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
                 z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};
@@ -600,44 +620,29 @@ template <int P_> struct MfmaZ2048 {
 // q' = 4q + e + 16h; Gk rows kzo = min(kz, 2048 - kz)
 template <int OPK>
 __device__ __forceinline__ void mf2_kfactors(float (&f)[8], const PencilArgs<float>& a, const TileK<float>& tk,
-                                             int comp, int bp, int q, int r)
+                                             const MfKCoef& k, int bp, int q, int r)
 {
+    if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_LAPL) {
+        float g[8];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int kz = 2 * (bp + 32 * (4 * q + e + 16 * h)) + r;
-            const int kzo = (kz <= 1024) ? kz : 2048 - kz;
-            float v;
-            if constexpr (OPK == OP_NONE) {
-                v = 1.0f;
-            } else if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_POTGRAD) {
-                const float g = tk.grow ? tk.grow[(long)kzo * a.g_kz] : 0.0f;
-                if constexpr (OPK == OP_PLAIN) {
-                    v = g;
-                } else {
-                    const float kzw = (float)((2 * kz < a.Mz) ? kz : kz - a.Mz) * (float)(2.0 * kPi * a.dk[2]);
-                    const float kzh = (2 * kz == a.Mz) ? 0.0f : kzw;
-                    const float kp2 = (float)(tk.kxf * tk.kxf + tk.kyf * tk.kyf);
-                    v = mf_kmul<OPK>(a, tk, comp, (OPK == OP_POTGRAD) ? pot_quantity(a.kmap, comp) : 0, kzw, kzh,
-                                     kp2, g);
-                }
-            } else {
-                const double kzf = kappa_full(kz, a.Mz, a.dk[2]);
-                double zf = a.invN;
-                if (a.reg > 0) {
-                    const double arg = a.zarg * (tk.kxf * tk.kxf + tk.kyf * tk.kyf + kzf * kzf);
-                    double sum = 0.0, term = 1.0;
-                    for (int n = 0; n < a.reg / 2; ++n) {
-                        sum += term;
-                        term = term * arg / (double)(n + 1);
-                    }
-                    zf *= sum * exp(-arg);
-                }
-                v = (float)zf;
+            for (int e = 0; e < 4; ++e) {
+                const int kz = 2 * (bp + 32 * (4 * q + e + 16 * h)) + r;
+                const int kzo = (kz <= 1024) ? kz : 2048 - kz;
+                g[4 * h + e] = k.grow[(long)kzo * a.g_kz];
             }
-            f[4 * h + e] = v;
-        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                f[4 * h + e] = mf_kval<OPK>(a, k, 2 * (bp + 32 * (4 * q + e + 16 * h)) + r, g[4 * h + e]);
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                f[4 * h + e] = (OPK == OP_NONE) ? 1.0f : mf_rhsf(a, tk, 2 * (bp + 32 * (4 * q + e + 16 * h)) + r);
     }
 }
 
@@ -668,7 +673,7 @@ __global__ void __launch_bounds__(64 * P_, P_ == 8 ? 2 : 3) k_zmfma2048(PencilAr
     const int col = ct * P + p;
     const bool cv = col < a.ncol;
     const int colr = cv ? col : a.ncol - 1;
-    const bool mimag = mf_imag_multiplier<OPK>(a, comp);
+    constexpr bool mimag = mf_imag_multiplier<OPK>();
 
     // the plan's table holds W2048^j, j < 2048: W1024^j = W2048^{2j}
     for (int i = tid; i < LH; i += 64 * NW) {
@@ -686,6 +691,7 @@ __global__ void __launch_bounds__(64 * P_, P_ == 8 ? 2 : 3) k_zmfma2048(PencilAr
     const long os32 = mf_pos_stride32(a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
     float2* myp = sd + p * PSTR;
     const TileK<float> tk = tile_k(a, col, slab, cv);
+    const MfKCoef kc = mf_kcoef<OPK>(a, tk, comp);
     int bb[2];
 #pragma unroll
     for (int bi = 0; bi < 2; ++bi) bb[bi] = NB * (w + NW * bi) + bsub;
@@ -726,7 +732,7 @@ __global__ void __launch_bounds__(64 * P_, P_ == 8 ? 2 : 3) k_zmfma2048(PencilAr
             if (r == 1) load_x();
         float gk[2][8];
 #pragma unroll
-        for (int bi = 0; bi < 2; ++bi) mf2_kfactors<OPK>(gk[bi], a, tk, comp, bb[bi], q, r);
+        for (int bi = 0; bi < 2; ++bi) mf2_kfactors<OPK>(gk[bi], a, tk, kc, bb[bi], q, r);
 
         // ---- forward stage 1: DFT over m (32 inputs), 32 outputs k1 per column (p, b) ----
 #pragma unroll
@@ -778,7 +784,7 @@ __global__ void __launch_bounds__(64 * P_, P_ == 8 ? 2 : 3) k_zmfma2048(PencilAr
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
-                if (mimag) {
+                if constexpr (mimag) {
                     z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
                     z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};
                 } else {
@@ -998,11 +1004,11 @@ __global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
             // the k-space operator keeps the factor s0
             float2 z0[4], z1[4];
             float mz = 0.0f;
-            const bool mimag = mf_imag_multiplier<OPK>(a, T.comp);
+            constexpr bool mimag = mf_imag_multiplier<OPK>();
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
-                if (mimag) {
+                if constexpr (mimag) {
                     // i k_d Gk: purely imaginary
                     z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
                     z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};

@@ -211,6 +211,8 @@ __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
             if constexpr (TWREG) sfft<L, E, B, -1, T>(v[c], W, my, t);
             else sfft_g<L, E, B, -1, T>(v[c], a.plan.tw, my, t);
         }
+    } else if constexpr (NC == 1) {
+        apply_ik_in<T, E>(v[0], a, col, t, TP);
     }
 
     const int nout = (NC == 1 || a.op == OP_DIV) ? 1 : 3;

@@ -20,8 +20,10 @@ TOL = {"fp64": 1e-12, "fp32": 2e-6, "mixed": 1e-6}
 NAMES = ("A", "dx", "dy", "dz", "lap")
 
 
-def _potential(prec, n, bc, dx, reg, b, outputs=NAMES):
+def _potential(prec, n, bc, dx, reg, b, outputs=NAMES, tuning=None):
     s = P.poisson_solver(PREC[prec])
+    if tuning:
+        s.set_tuning(**tuning)
     s.setup3d(n, bc, dx, reg)
     assert s.last_status == 0
     s.push(b)
@@ -95,3 +97,44 @@ def test_potential_laplacian_closed_form():
     r2 = x[:, None, None] ** 2 + x[None, :, None] ** 2 + x[None, None, :] ** 2
     g = np.exp(-r2 / (2 * se * se)) / ((2 * np.pi) ** 1.5 * se ** 3)
     assert O.rel_l2(out["lap"], -g) <= 1e-10
+
+
+SUBSETS = [("dx",), ("dy",), ("dx", "dy"), ("A", "dy"), ("dx", "dz", "lap"), NAMES]
+
+
+@pytest.mark.parametrize("zblk", [-1, 0, 3, 5], ids=lambda z: "zblk%d" % z)
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_potential_subsets_in_every_spectrum_layout(zblk, prec):
+    """d/dx A and d/dy A are derived from the z-pass spectrum of A in the
+    y-pass layout of the spectrum buffer: every output subset (with and
+    without A itself) in the three layouts (ky-major, z-major, z blocks)."""
+    n, bc, dx, reg = (12, 10, 40), (0, 0, 0), (0.1, 0.12, 0.03), 2
+    b = random_rhs(n, 79, 1)[0]
+    lo_a, _ = O.solve(n, bc, dx, reg, [b])
+    lo_g, _ = O.solve(n, bc, dx, reg, [b, b, b], op="grad")
+    lo_l, _ = O.solve(n, bc, dx, reg, [b], op="lap")
+    ref = {"A": lo_a[0], "dx": lo_g[0], "dy": lo_g[1], "dz": lo_g[2], "lap": lo_l[0]}
+    for outs in SUBSETS:
+        announce("potential subset", n, prec, zblk, outs)
+        out, _ = _potential(prec, n, bc, dx, reg, b, outputs=outs, tuning={"zblk": zblk})
+        for k in NAMES:
+            if k in outs:
+                assert O.rel_l2(out[k], ref[k]) <= TOL[prec], (outs, k)
+            else:
+                assert np.isnan(out[k]).all(), (outs, k)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "mixed"])
+def test_potential_long_z(prec):
+    """z transforms of 2048 (nz = 1024 unbounded; the matrix-core kernel of
+    two 1024-point residues when mixed), 32-plane z blocks."""
+    n, bc, dx, reg = (8, 6, 1024), (0, 0, 0), (0.1, 0.12, 1 / 512), 4
+    announce("potential long z", n, prec)
+    b = random_rhs(n, 80, 1)[0]
+    out, _ = _potential(prec, n, bc, dx, reg, b)
+    lo_a, _ = O.solve(n, bc, dx, reg, [b])
+    lo_g, _ = O.solve(n, bc, dx, reg, [b, b, b], op="grad")
+    lo_l, _ = O.solve(n, bc, dx, reg, [b], op="lap")
+    assert O.rel_l2(out["A"], lo_a[0]) <= 2e-6
+    assert O.rel_l2([out["dx"], out["dy"], out["dz"]], lo_g) <= 2e-6
+    assert O.rel_l2(out["lap"], lo_l[0]) <= 2e-6
