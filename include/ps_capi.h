@@ -150,13 +150,16 @@ int ps_eval_special(const double* x, double* y, long n, int which);
    No reference counterpart: the reference's grad is component-wise
    (solve3d.cpp:348-353), so a caller wanting A and grad A pushes the scalar
    three times and solves twice (c++/test/mixed_ffp3d/sources/main.cpp:164-178).
-   ps_solve3d_potential transforms the held scalar rhs X once in x and y; its
-   z pass writes every selected quantity from that one spectrum:
+   ps_solve3d_potential transforms the held scalar rhs X once in x and y and
+   computes every selected quantity from that one spectrum:
      PS_POT_A    A = G * B                  (the plain solve of X)
      PS_POT_DX.. dA/dx, dA/dy, dA/dz         (grad of X, as solve3d's grad)
      PS_POT_LAP  Laplacian of A, -|k|^2 Ghat  (div of grad A)
-   with the context's regularisation; results stay in the context until the
-   next potential solve.  The held rhs is left as it was.  3D contexts. */
+   with the context's regularisation.  The z pass transforms A, dA/dz and the
+   Laplacian (one z transform each); dA/dx and dA/dy are i kx and i ky times
+   A's z-pass output, applied as their y^-1 pass loads it.  Results stay in
+   the context until the next potential solve.  The held rhs is left as it
+   was.  3D contexts. */
 enum { PS_POT_A = 1, PS_POT_DX = 2, PS_POT_DY = 4, PS_POT_DZ = 8, PS_POT_LAP = 16 };
 int ps_solve3d_potential(ps_ctx* ctx, int outputs);
 /* host arrays (x fastest) of the quantities computed; NULL skips one */

@@ -102,10 +102,10 @@ struct ps_ctx {
     int pot_cap = 0;
     int pot_mask = 0;       // quantities (PS_POT_*) that pot holds
     void* stage = nullptr;  // kStageReals doubles: host-pointer push / pull staging (FP32 forms)
-    void* A1 = nullptr;     // ncomp_alloc x nz x ny x Hp complex
-    void* A2 = nullptr;     // ncomp_alloc x nz x My x Hp complex (not used by 2D contexts)
+    void* A1 = nullptr;     // ncomp_a1 x nz x ny x Hp complex
+    void* A2 = nullptr;     // ncomp_a2 x nz x My x Hp complex (not used by 2D contexts)
     long A1c = 0, A2c = 0;  // complex elements per component
-    int ncomp_alloc = 0;
+    int ncomp_a1 = 0, ncomp_a2 = 0;
     void* G = nullptr;      // octant of Ghat*zeta/N, real
     void* mfrag = nullptr;  // matrix-core DFT fragments (PS_MIXED)
 
@@ -221,7 +221,7 @@ void release(ps_ctx* c)
     free_ptr(c, c->tw_half);
     c->zres = false;
     for (DevPlan* d : {&c->px, &c->py, &c->pz1, &c->pz3}) free_ptr(c, d->tw);
-    c->ncomp_alloc = 0;
+    c->ncomp_a1 = c->ncomp_a2 = 0;
     c->ready = false;
     c->src_spec = c->filt_pending = c->filt_reproj = c->rhsf_valid = false;
 }
@@ -251,15 +251,22 @@ int plan_dim(ps_ctx* c, DevPlan* d, int L, int ncomp, const char* what)
     return fail(c, PS_EUNSUPPORTED, std::string(what) + ": " + why);
 }
 
-int ensure_comp_buffers(ps_ctx* c, int ncomp)
+// A1 of n1 and A2 of n2 components (kept when already as large)
+int ensure_comp_buffers(ps_ctx* c, int n1, int n2)
 {
-    if (ncomp <= c->ncomp_alloc) return PS_OK;
-    free_ptr(c, c->A1);
-    free_ptr(c, c->A2);
     const size_t cb = 2 * (size_t)real_bytes(c);
-    CK(dev_alloc(c, &c->A1, (size_t)ncomp * c->A1c * cb));
-    if (c->A2c > 0) CK(dev_alloc(c, &c->A2, (size_t)ncomp * c->A2c * cb));
-    c->ncomp_alloc = ncomp;
+    if (n1 > c->ncomp_a1) {
+        free_ptr(c, c->A1);
+        c->ncomp_a1 = 0;
+        CK(dev_alloc(c, &c->A1, (size_t)n1 * c->A1c * cb));
+        c->ncomp_a1 = n1;
+    }
+    if (n2 > c->ncomp_a2 && c->A2c > 0) {
+        free_ptr(c, c->A2);
+        c->ncomp_a2 = 0;
+        CK(dev_alloc(c, &c->A2, (size_t)n2 * c->A2c * cb));
+        c->ncomp_a2 = n2;
+    }
     return PS_OK;
 }
 
@@ -504,8 +511,25 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     void* A1 = wb ? wb->A1 : nullptr;
     void* A2 = wb ? wb->A2 : nullptr;
     const bool fan = (flags & PF_FANOUT) != 0;
+    // potential fan-out: output c (the c-th set bit of kmap) goes to A1 /
+    // dst component c.  kx and ky are constant along a z pencil, so d/dx A
+    // and d/dy A are i kx and i ky times the z-pass spectrum of A: the z pass
+    // transforms A (OP_PLAIN), d/dz A (OP_GRAD on k_z) and the Laplacian
+    // (OP_LAPL) only, into A2 components 1 + zslot, and the y^-1 pass of
+    // d/dx and d/dy reads A's and multiplies by i k on load (PencilArgs::ik_in).
+    const bool potfan = fan && op == OP_POTGRAD;
+    if ((op == OP_POTGRAD) != fan || (fan && (nin != 1 || nc != 1 || c->dim2)))
+        return fail(c, PS_EINVAL, "the potential fan-out needs one input, one z component and a 3D context");
+    int slot[5] = {-1, -1, -1, -1, -1}, zslot[5] = {-1, -1, -1, -1, -1}, nzq = 0;
+    if (potfan) {
+        for (int qn = 0, k = 0; qn < 5; ++qn) slot[qn] = ((kmap >> qn) & 1) ? k++ : -1;
+        for (int qn : {0, 3, 4})
+            if (qn == 0 ? (kmap & 7) != 0 : ((kmap >> qn) & 1) != 0) zslot[qn] = nzq++;
+        zslot[1] = zslot[2] = zslot[0];
+    }
     if (!wb) {
-        int r = ensure_comp_buffers(c, fan ? nout + 1 : std::max(std::max(nin, nc), nout));
+        const int n = std::max(std::max(nin, nc), nout);
+        int r = ensure_comp_buffers(c, potfan ? nout : n, potfan ? 1 + nzq : n);
         if (r) return r;
         A1 = c->A1;
         A2 = c->A2;
@@ -579,11 +603,9 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     if (reg >= 0) za.reg = reg;
     za.ncomp = (nc == 3) ? 1 : nin;     // nc 2: one input, outputs 0 and 1
     if (fan) {
-        // outputs 0..nout-1 of the one input spectrum (A2 component 0) go to
-        // A2 components 1..nout
+        // the one input spectrum (A2 component 0) is read by each z launch below
         za.ia.comp = 0;
-        za.out = static_cast<char*>(A2) + (size_t)c->A2c * cbytes;
-        za.ncomp = nout;
+        za.ncomp = 1;
     }
     if (c->t2d) {
         // 2D transposed path: A1 [c][y][Hp] -> A2 [c][kx][nyp], then the fused
@@ -620,28 +642,15 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         else CK(launch_pencil<T>(zp.p.E, nc, zz, s));
         return PS_OK;
     };
-    // potential fan-out: output c (the c-th set bit of kmap) has A2 slot c
-    // (component 1 + c).  kx and ky are constant along a z pencil, so d/dx A
-    // and d/dy A are i kx and i ky times the z-pass spectrum of A: the z pass
-    // transforms A (OP_PLAIN), d/dz A (OP_GRAD on k_z) and the Laplacian
-    // (OP_LAPL) only, and the y^-1 pass of d/dx and d/dy reads A's slot and
-    // multiplies by i k on load (PencilArgs::ik_in).
-    const bool potfan = fan && op == OP_POTGRAD;
-    if ((op == OP_POTGRAD) != fan || (fan && (nin != 1 || nc != 1 || c->dim2)))
-        return fail(c, PS_EINVAL, "the potential fan-out needs one input, one z component and a 3D context");
-    int slot[5] = {-1, -1, -1, -1, -1}, aslot = -1;
     char* A2o = static_cast<char*>(A2) + (size_t)c->A2c * cbytes;
     if (potfan) {
-        for (int qn = 0, k = 0; qn < 5; ++qn) slot[qn] = ((kmap >> qn) & 1) ? k++ : -1;
-        aslot = slot[0] >= 0 ? slot[0] : slot[1] >= 0 ? slot[1] : slot[2];
         for (int qn : {0, 3, 4}) {
-            const int sl = qn == 0 ? aslot : slot[qn];
-            if (sl < 0) continue;
+            if (zslot[qn] < 0) continue;
             PencilArgs<T> zq = za;
             zq.op = qn == 0 ? OP_PLAIN : qn == 3 ? OP_GRAD : OP_LAPL;
             zq.kmap = qn == 3 ? KMAP_Z : KMAP_3D;
             zq.ncomp = 1;
-            zq.out = A2o + (size_t)sl * c->A2c * cbytes;
+            zq.out = A2o + (size_t)zslot[qn] * c->A2c * cbytes;
             if (const int zr = zlaunch(zq)) return zr;
         }
     } else if (const int zr = zlaunch(za)) {
@@ -681,7 +690,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
             PencilArgs<T> y1 = yi;
             y1.ncomp = 1;
             y1.ik_in = (qn == 1) ? 1 : (qn == 2) ? 2 : 0;
-            y1.in = A2o + (size_t)(y1.ik_in ? aslot : slot[qn]) * c->A2c * cbytes;
+            y1.in = A2o + (size_t)zslot[qn] * c->A2c * cbytes;
             y1.out = static_cast<char*>(yi.out) + (size_t)slot[qn] * yi.oa.comp * cbytes;
             if (const int yr = ylaunch(y1)) return yr;
         }
