@@ -202,6 +202,9 @@ int ps_is_2d(const ps_ctx* ctx);
    kernels multiply by (binary16 bit patterns; n >= 4096 entries, layout
    [lane 64][K chunk 2][Yr hi, Yr lo, Yi hi, Yi lo][8]) */
 int ps_dft32_fragments(unsigned short* out, int n);
+/* the same for the 16-point DFT matrix (transforms of 512; n >= 2048
+   entries, layout [lane 64][Yr hi, Yr lo, Yi hi, Yi lo][8]) */
+int ps_dft16_fragments(unsigned short* out, int n);
 /* kernel family of the one-component z pass chosen at setup: 0 runtime-plan,
    1 compile-time specialised, 2 pencil-per-wavefront, 3 matrix-core (PS_MIXED),
    4 two-residue (FP32 z pencils of 2048 as two 1024-point transforms);

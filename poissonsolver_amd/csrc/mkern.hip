@@ -57,6 +57,39 @@ void zmfma_fragments(unsigned short* out)
     }
 }
 
+void zmfma_fragments16(unsigned short* out)
+{
+    for (int lane = 0; lane < 64; ++lane) {
+        const int i = lane & 15, qa = lane >> 4;
+        for (int e8 = 0; e8 < 8; ++e8) {
+            const int m = 4 * qa + (e8 >> 1);
+            const int jj = (2 * i * m) & 31;                  // 2 pi i m / 16 = 2 pi (2 i m) / 32
+            const double cs = Cos32::c[jj];
+            const double sn = Cos32::c[(jj + 24) & 31];
+            const double vr = (e8 & 1) ? sn : cs;
+            const double vi = (e8 & 1) ? cs : -sn;
+            const unsigned short rh = f16_bits(vr), ih = f16_bits(vi);
+            unsigned short* o = out + lane * 32 + e8;
+            o[0] = rh;
+            o[8] = f16_bits(vr - f16_value(rh));
+            o[16] = ih;
+            o[24] = f16_bits(vi - f16_value(ih));
+        }
+    }
+}
+
+template <int OPK> static hipError_t launch_zmfma512_op(PencilArgs<float> a, hipStream_t s)
+{
+    using M = MfmaZ512;
+    const size_t shm = M::lds_bytes();
+    hipError_t e = allow_lds(k_zmfma512<OPK>, shm);
+    if (e != hipSuccess) return e;
+    a.ncoltile = (a.ncol + M::P - 1) / M::P;
+    const long nwork = (long)a.ncoltile * a.nslab * a.ncomp;
+    hipLaunchKernelGGL((k_zmfma512<OPK>), dim3((unsigned)nwork), dim3(64 * M::NW), shm, s, a);
+    return hipGetLastError();
+}
+
 template <int OPK> static hipError_t launch_zmfma_op(PencilArgs<float> a, int persistent, hipStream_t s)
 {
     const size_t shm = MfmaZ::lds_bytes();
@@ -104,7 +137,7 @@ hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s)
 
 bool zmfma_supported(int L, int half, int op)
 {
-    return (L == MfmaZ::L || L == 2048) && half &&
+    return (L == 512 || L == MfmaZ::L || L == 2048) && half &&
            (op == OP_NONE || op == OP_PLAIN || op == OP_GRAD || op == OP_RHSF || op == OP_LAPL);
 }
 
@@ -129,6 +162,17 @@ template <int OPK> static hipError_t launch_zmfma2048_op(PencilArgs<float> a, in
 
 hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t s)
 {
+    if (a.plan.L == 512) {
+        if (a.n_in != 256 || a.n_out != 256 || !a.mfrag16) return hipErrorInvalidValue;
+        switch (a.op) {
+        case OP_NONE: return launch_zmfma512_op<OP_NONE>(a, s);
+        case OP_PLAIN: return launch_zmfma512_op<OP_PLAIN>(a, s);
+        case OP_GRAD: return launch_zmfma512_op<OP_GRAD>(a, s);
+        case OP_RHSF: return launch_zmfma512_op<OP_RHSF>(a, s);
+        case OP_LAPL: return launch_zmfma512_op<OP_LAPL>(a, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (a.plan.L == 2048) {
         // two 1024-point residues per pencil
         if (a.n_in != 1024 || a.n_out != 1024) return hipErrorInvalidValue;

@@ -433,6 +433,7 @@ void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d, void* A1, void* A
     a.reg = regm > 0 ? regm : 0;
     a.invN = 1.0 / ((double)c->M[0] * c->M[1] * c->M[2]);
     a.mfrag = c->mfrag;
+    a.mfrag16 = c->mfrag ? static_cast<const unsigned short*>(c->mfrag) + kZmfmaFragHalves : nullptr;
     a.fone = 1.0f;
     a.fzero = 0.0f;
 }
@@ -865,7 +866,7 @@ template <typename T> int setup_impl(ps_ctx* c)
     const long zcomp = c->dim2 ? c->A1c : c->A2c;
     const bool off32 = zcomp * 8 < (1L << 32) && (long)c->O[0] * c->O[1] * c->O[2] * 4 < (1L << 32);
     c->zmfma = c->prec == PS_MIXED && tn.mfma && zmfma_supported(c->M[2], c->bc[2] == 0, OP_PLAIN) &&
-               c->zb <= 5 && (c->M[2] == 2048 || off32);
+               c->zb <= 5 && (c->M[2] != 1024 || off32);     // the 1024 kernel's offsets are 32-bit
     // two-residue z kernel: FP32 pencils of 2048 holding nz = 1024 unbounded
     // cells run as two 1024-point wavefront transforms (zwave = 1; zwave = 2
     // keeps one pencil of 2048 per wavefront with 32 elements per thread)
@@ -898,8 +899,9 @@ template <typename T> int setup_impl(ps_ctx* c)
     }
     c->g_layout = ((c->zwave || c->zres || c->t2d) && !c->zmfma) ? 1 : 0;
     if (c->zmfma) {
-        std::vector<unsigned short> fr(kZmfmaFragHalves);
+        std::vector<unsigned short> fr(kZmfmaFragHalves + kZmfma16FragHalves);
         zmfma_fragments(fr.data());
+        zmfma_fragments16(fr.data() + kZmfmaFragHalves);
         CK(dev_alloc(c, &c->mfrag, fr.size() * sizeof(unsigned short)));
         CK(hipMemcpy(c->mfrag, fr.data(), fr.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
     }
@@ -1393,6 +1395,13 @@ int ps_solve3d(ps_ctx* c)
     r = (c->prec == PS_FP64) ? solve_impl<double>(c, nullptr) : solve_impl<float>(c, nullptr);
     if (r) return r;
     CK(hipStreamSynchronize(c->stream));
+    return PS_OK;
+}
+
+int ps_dft16_fragments(unsigned short* out, int n)
+{
+    if (!out || n < kZmfma16FragHalves) return PS_EINVAL;
+    zmfma_fragments16(out);
     return PS_OK;
 }
 

@@ -89,6 +89,7 @@ template <typename T> struct PencilArgs {
     // matrix-core z pass: per-lane fp16 hi/lo fragments of the 32-point DFT
     // matrix (64 lanes x 8 fragments x 8 halves), built once at setup
     const void* mfrag;
+    const void* mfrag16;   // the same for the 16-point DFT (64 lanes x 4 fragments x 8 halves)
     // 1 and 0 supplied at run time: the compiler cannot fold them, so the
     // FP16 split forms each conversion as one mixed-precision FMA
     float fone, fzero;

@@ -864,6 +864,241 @@ __global__ void __launch_bounds__(64 * P_, P_ == 8 ? 2 : 3) k_zmfma2048(PencilAr
     }
 }
 
+// ---------------------------------------------------------------------------
+// Transform length 512 = 32 x 16 (Mz of a 256-cell unbounded z direction):
+//   X[k1 + 32 k2] = sum_b W16^{b k2} W512^{b k1} sum_{m<16} W32^{m k1} x[b + 16 m]
+//   x[b + 16 m]   = sum_k1 W32^{-m k1} W512^{-b k1} sum_k2 W16^{-b k2} Z[k1 + 32 k2]
+// (b < 16, k1 < 32, k2 < 16; inputs and kept outputs m < 16).  Forward stage
+// 1 (16 inputs) and inverse stage 2 (16 outputs) are the 32-point stages of
+// the 1024-point kernel; the two middle stages are 16-point DFTs of one K
+// chunk each, with the k-space operator between them in registers.
+// Workgroup: P = 16 pencils (adjacent kx columns: 128-byte rows per z
+// position) x the whole z extent, 8 wavefronts; an MFMA column is one pencil
+// (p = lane & 15).  Wavefront w owns b = w, w + 8 in the 32-point stages and
+// k1 = w + 8 bi (bi < 4) in the 16-point ones.  Offsets are 64-bit.
+// ---------------------------------------------------------------------------
+struct MfmaZ512 {
+    static constexpr int L = 512;
+    static constexpr int P = 16, NW = 8;
+    // LDS [p][k1 < 32][b < 16]: in every exchange the 16 lanes of a group
+    // differ in p only, and PS = 1 (mod 16) puts their 64-bit accesses on 16
+    // distinct bank pairs
+    static constexpr int RS = 16;
+    static constexpr int PS = 32 * 16 + 1;
+    static constexpr size_t lds_bytes()
+    {
+        return (size_t)P * PS * 8 + (size_t)L * 8 + 2 * P * 4;
+    }
+};
+
+// A fragments of the 16-point forward DFT in real form (one K chunk: points
+// m = 0..15, re/im interleaved; rows 0..15):
+//   Yr row k: ( cos t,  sin t),  Yi row k: (-sin t, cos t),  t = 2 pi k m / 16
+// Table layout (built by zmfma_fragments16): [lane][rh, rl, ih, il][8 halves].
+struct MfDft16 {
+    mf_h8 rh, rl, ih, il;
+};
+
+__device__ __forceinline__ MfDft16 mf_dft16_frags(const void* table, int lane)
+{
+    const mf_h8* t = reinterpret_cast<const mf_h8*>(table) + lane * 4;
+    return MfDft16{t[0], t[1], t[2], t[3]};
+}
+
+struct MfAcc16 {
+    mf_f4 r, i;
+};
+
+// the 16-point DFT of 16 columns (lane group q holds points, and receives
+// outputs, 4q..4q+3)
+__device__ __forceinline__ MfAcc16 mf_dft16(const MfDft16& F, const MfFrag& B)
+{
+    MfAcc16 d;
+    d.r = d.i = mf_f4{0, 0, 0, 0};
+    d.r = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh, B.hi, d.r, 0, 0, 0);
+    d.i = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih, B.hi, d.i, 0, 0, 0);
+    d.r = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh, B.lo, d.r, 0, 0, 0);
+    d.i = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih, B.lo, d.i, 0, 0, 0);
+    d.r = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rl, B.hi, d.r, 0, 0, 0);
+    d.i = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.il, B.hi, d.i, 0, 0, 0);
+    return d;
+}
+
+// k-space multipliers of spectrum points kz = k1 + 32 (4q + e), e < 4;
+// Gk rows kzo = min(kz, 512 - kz)
+template <int OPK>
+__device__ __forceinline__ void mf5_kfactors(float (&f)[4], const PencilArgs<float>& a, const TileK<float>& tk,
+                                             const MfKCoef& k, int k1, int q)
+{
+    if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_LAPL) {
+        float g[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int kz = k1 + 32 * (4 * q + e);
+            const int kzo = (kz <= 256) ? kz : 512 - kz;
+            g[e] = k.grow[(long)kzo * a.g_kz];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) f[e] = mf_kval<OPK>(a, k, k1 + 32 * (4 * q + e), g[e]);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) f[e] = (OPK == OP_NONE) ? 1.0f : mf_rhsf(a, tk, k1 + 32 * (4 * q + e));
+    }
+}
+
+template <int OPK>
+__global__ void __launch_bounds__(512, 4) k_zmfma512(PencilArgs<float> a)
+{
+    using M = MfmaZ512;
+    constexpr int P = M::P, RS = M::RS, PSTR = M::PS, L = M::L;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    float2* sd = reinterpret_cast<float2*>(smem_raw);
+    float2* stw = sd + P * PSTR;            // W512^j, j < 512
+    unsigned* smax = reinterpret_cast<unsigned*>(stw + L);   // [input | inverse stage-2 input][P]
+
+    const int tid = threadIdx.x;
+    const int w = tid >> 6, l = tid & 63, q = l >> 4, p = l & 15;
+    const int bp32 = (l ^ 32) << 2;
+
+    const long nblk = gridDim.x;
+    const long lin = a.xcd_remap ? xcd_remap(blockIdx.x, nblk) : (long)blockIdx.x;
+    const long ntile = (long)a.ncoltile * a.nslab;
+    const int comp = (int)(lin / ntile);
+    const long rem = lin - (long)comp * ntile;
+    const int slab0 = (int)(rem / a.ncoltile);
+    const int ct = (int)(rem - (long)slab0 * a.ncoltile);
+    const int slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    const int col = ct * P + p;
+    const bool cv = col < a.ncol;
+    // columns past the grid repeat the last one: loads stay in bounds, their
+    // results are never stored, and no pencil's data mixes with another's
+    const int colr = cv ? col : a.ncol - 1;
+    constexpr bool mimag = mf_imag_multiplier<OPK>();
+
+    for (int i = tid; i < L; i += 64 * M::NW) stw[i] = a.plan.tw[i];
+    if (tid < 2 * P) smax[tid] = 0u;
+    const MfDft32 F = mf_dft32_frags(a.mfrag, l);
+    const MfDft16 F16 = mf_dft16_frags(a.mfrag16, l);
+
+    const float2* ib = reinterpret_cast<const float2*>(a.in) + (long)comp * a.ia.comp + (long)colr * a.ia.col +
+                       boff(slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
+    float2* ob = reinterpret_cast<float2*>(a.out) + (long)comp * a.oa.comp + (long)col * a.oa.col +
+                 boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi);
+    float2* myp = sd + p * PSTR;
+
+    // k-space multipliers of the four 16-point columns k1 = w + 8 bi
+    const TileK<float> tk = tile_k(a, col, slab, cv);
+    const MfKCoef kc = mf_kcoef<OPK>(a, tk, comp);
+    float gk[4][4];
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) mf5_kfactors<OPK>(gk[bi], a, tk, kc, w + 8 * bi, q);
+
+    // ---- load x[b + 16 m], m = 4q + e < 16 (the rest is zero padding) ----
+    float2 x[2][4];
+    float mx = 0.0f;
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int b = w + 8 * bi;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            x[bi][e] = ib[boff(b + 16 * (4 * q + e), a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi)];
+            mx = mf_absmax(x[bi][e], mx);
+        }
+    }
+    __syncthreads();                       // twiddle table and max slots initialised
+    atomicMax(&smax[p], __float_as_uint(mx));
+    __syncthreads();
+    // one scale per pencil: the forward stage-1 output (<= 16 sqrt 2 2^10)
+    // is the stage-2 input without rescaling
+    float s0, inv0;
+    mf_scale(__uint_as_float(smax[p]), s0, inv0);
+
+    // ---- forward stage 1: column (p, b), DFT-32 over m, 32 outputs k1 ----
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int b = w + 8 * bi;
+        MfFrag B[2];
+        B[0] = mf_split4(x[bi], s0, s0, a.fzero);
+        const MfAcc d = mf_dft32_rows32i<1>(F, B);
+        // twiddle W512^{b k1}; stored as row k1, column b
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int k1 = 4 * q + e + 16 * h;
+                const float2 y = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
+                myp[k1 * RS + b] = cmul(y, stw[(b * k1) & (L - 1)]);
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- forward stage 2 (DFT-16 over b), k-space operator, inverse stage 1
+    //      (DFT-16 over k2), column (p, k1) ----
+    float m3 = 0.0f;
+#pragma unroll
+    for (int bi = 0; bi < 4; ++bi) {
+        const int k1 = w + 8 * bi;
+        float2* row = myp + k1 * RS;
+        float2 v[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = row[4 * q + e];
+        MfFrag B = mf_split4u(v, a.fone);
+        MfAcc16 d = mf_dft16(F16, B);
+        // s0 X[k1 + 32 k2], k2 = 4q + e; the k-space operator keeps the factor s0
+        float2 z[4];
+        float mz = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float m0 = gk[bi][e];
+            if constexpr (mimag) z[e] = float2{-d.i[e] * m0, d.r[e] * m0};
+            else z[e] = float2{d.r[e] * m0, d.i[e] * m0};
+            mz = mf_absmax(z[e], mz);
+        }
+        float s2, inv2;
+        mf_scale(mf_colmax(mz, bp32), s2, inv2);
+        // inverse: conj(F) z = conj(F conj(z))
+        B = mf_split4(z, s2, -s2, a.fzero);
+        d = mf_dft16(F16, B);
+        // output b = 4q + e: conj(d W512^{b k1}) inv0 inv2, in place as row k1, column b
+        const float f = inv0 * inv2;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int b = 4 * q + e;
+            const float2 t = cmul(float2{d.r[e], d.i[e]}, stw[(b * k1) & (L - 1)]);
+            const float2 r = float2{t.x * f, -t.y * f};
+            row[b] = r;
+            m3 = mf_absmax(r, m3);
+        }
+    }
+    atomicMax(&smax[P + p], __float_as_uint(m3));
+    __syncthreads();
+
+    // ---- inverse stage 2: column (p, b), inputs over k1, outputs m < 16 ----
+    float s3, inv3;
+    mf_scale(__uint_as_float(smax[P + p]), s3, inv3);
+#pragma unroll
+    for (int bi = 0; bi < 2; ++bi) {
+        const int b = w + 8 * bi;
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = myp[(4 * q + e) * RS + b];
+            v1[e] = myp[(16 + 4 * q + e) * RS + b];
+        }
+        MfFrag B[2];
+        B[0] = mf_split4(v0, s3, -s3, a.fzero);
+        B[1] = mf_split4(v1, s3, -s3, a.fzero);
+        const MfAcc d = mf_dft32_rows16(F, B);
+        if (cv) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                ob[boff(b + 16 * (4 * q + e), a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi)] =
+                    float2{d.r0[e] * inv3, -d.i0[e] * inv3};
+        }
+    }
+}
+
 // Persistent form (2 wavefronts per SIMD, one workgroup per CU, 256
 // registers): workgroup g handles tiles g, g + G, g + 2G, ... (in
 // XCD-contiguous order).  The next tile's input and k-space multipliers are

@@ -27,6 +27,10 @@ hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s);
 // entries; PencilArgs::mfrag points at a device copy)
 constexpr int kZmfmaFragHalves = 64 * 8 * 8;
 void zmfma_fragments(unsigned short* out);
+// the same for the 16-point DFT (transforms of 512): kZmfma16FragHalves
+// entries, [lane][rh, rl, ih, il][8 halves]
+constexpr int kZmfma16FragHalves = 64 * 4 * 8;
+void zmfma_fragments16(unsigned short* out);
 // round-to-nearest-even binary16 bit pattern of a finite double below 65504
 // in magnitude, and its value
 unsigned short f16_bits(double v);

@@ -1,7 +1,8 @@
-"""Mixed-precision form (PS_MIXED) vs the oracle: the z pass of a 512- or
-1024-cell unbounded z direction (transform length 1024, or 2048 as two
-1024-point residues) runs on the matrix cores as FP16 hi/lo DFT-32 products
-with FP32 accumulation; every other pass is FP32.
+"""Mixed-precision form (PS_MIXED) vs the oracle: the z pass of a 256-, 512-
+or 1024-cell unbounded z direction (transform length 512 = 32 x 16, 1024 =
+32 x 32, or 2048 as two 1024-point residues) runs on the matrix cores as
+FP16 hi/lo DFT-32 and DFT-16 products with FP32 accumulation; every other
+pass is FP32.
 Bar: rel-L2 <= 1e-6 against the FP64 oracle (the north-star tolerance of the
 reduced-precision form).  x and y stay small so the oracle finishes in
 seconds."""
@@ -10,7 +11,7 @@ import pytest
 
 import poissonsolver_amd as P
 from oracle import oracle as O
-from tests.gpu_util import announce, gpu_solve, random_rhs
+from tests.gpu_util import announce, gpu_solve, gpu_solve2d, random_rhs
 
 pytestmark = pytest.mark.gpu
 
@@ -172,3 +173,64 @@ def test_mixed_2048_potential_and_range():
     assert O.rel_l2(got[0], lo_a[0]) <= MIXED_TOL
     assert O.rel_l2(got[1:4], lo_g) <= MIXED_TOL
     assert O.rel_l2(got[4], lo_l[0]) <= MIXED_TOL
+
+
+CASES512 = [
+    ((8, 6, 256), (0, 0, 0), (0.1, 0.12, 1 / 128)),
+    ((12, 10, 256), (1, 1, 0), (1 / 12, 0.1, 1 / 128)),
+    ((9, 7, 256), (0, 0, 0), (0.1, 0.1, 0.008)),
+]
+
+
+@pytest.mark.parametrize("case", CASES512, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
+@pytest.mark.parametrize("reg", [0, 2, 10])
+def test_mixed_512_plain_grad_filtered_rhs(case, reg):
+    """nz = 256 unbounded: z transforms of 512 = 32 x 16 on the matrix cores
+    (plain, component-wise grad, filtered rhs)."""
+    n, bc, dx = case
+    announce("mixed 512", n, bc, reg)
+    assert _variant(n, bc, dx, reg) == "matrix-core"
+    b = random_rhs(n, 71 + reg)
+    _compare(n, bc, dx, reg, "plain", [b[0]], want_rhs=True)
+    _compare(n, bc, dx, reg, "grad", b, want_rhs=True)
+
+
+@pytest.mark.parametrize("zblk", [-1, 0, 3, 4, 5], ids=lambda z: "zblk%d" % z)
+def test_mixed_512_every_layout_and_range(zblk):
+    """The 512-point kernel in every z layout of the spectrum buffer, on a
+    source spanning 2^-40 .. 2^40 in magnitude, with the potential form's
+    quantities (plain, d/dz, Laplacian; d/dx and d/dy from A)."""
+    n, bc, dx = (10, 8, 256), (0, 0, 0), (0.1, 0.12, 1 / 128)
+    announce("mixed 512 layout", n, zblk)
+    rng = np.random.default_rng(73)
+    b = rng.standard_normal((n[2], n[1], n[0])) * np.exp2(rng.uniform(-40, 40, (n[2], 1, 1)))
+    s = P.poisson_solver(P.PS_MIXED)
+    s.set_tuning(zblk=zblk)
+    s.setup3d(n, bc, dx, 2)
+    assert s.zpass_variant() == "matrix-core"
+    s.push(b)
+    s.solve3d_potential(("A", "dx", "dy", "dz", "lap"))
+    got = [np.empty_like(b) for _ in range(5)]
+    s.pull_potential(*got)
+    assert s.last_status == 0
+    s.close()
+    lo_a, _ = O.solve(n, bc, dx, 2, [b])
+    lo_g, _ = O.solve(n, bc, dx, 2, [b, b, b], op="grad")
+    lo_l, _ = O.solve(n, bc, dx, 2, [b], op="lap")
+    assert O.rel_l2(got[0], lo_a[0]) <= MIXED_TOL
+    assert O.rel_l2(got[1:4], lo_g) <= MIXED_TOL
+    assert O.rel_l2(got[4], lo_l[0]) <= MIXED_TOL
+
+
+def test_mixed_512_2d():
+    """2D: y transforms of 512 (ny = 256 unbounded) on the 512-point kernel."""
+    n, bc, dx = (24, 256), (0, 0), (0.05, 1 / 128)
+    announce("mixed 512 2D", n)
+    b = np.random.default_rng(79).standard_normal((n[1], n[0]))
+    s = P.poisson_solver(P.PS_MIXED)
+    s.setup2d(n, bc, dx, 2)
+    assert s.zpass_variant() == "matrix-core"
+    lg, _ = gpu_solve2d(n, bc, dx, 2, [b], op="grad", prec="mixed", solver=s)
+    s.close()
+    lo, _ = O.solve2d(n, bc, dx, 2, [b], op="grad")
+    assert O.rel_l2([lg[0], lg[1]], [lo[0], lo[1]]) <= MIXED_TOL

@@ -59,3 +59,49 @@ def test_split_product_reproduces_a_dft_to_fp32_accuracy():
     got_i = A[1, 0] @ xh + A[1, 0] @ xl + A[1, 1] @ xh
     err = np.linalg.norm((got_r + 1j * got_i) - y) / np.linalg.norm(y)
     assert err < 1e-6, err
+
+
+def fragments16():
+    lib = C.lib()
+    buf = (ctypes.c_ushort * 2048)()
+    assert lib.ps_dft16_fragments(buf, 2048) == 0
+    assert lib.ps_dft16_fragments(buf, 100) != 0        # too small: error, not a write
+    return np.frombuffer(bytes(buf), dtype=np.uint16).view(np.float16).reshape(64, 4, 8)
+
+
+def test_fragments16_are_the_split_dft16_matrix():
+    # the 16-point DFT of the transforms of 512: one K chunk, rows 0..15
+    f = fragments16().astype(np.float64)
+    for lane in range(64):
+        i, q = lane % 16, lane // 16
+        for e8 in range(8):
+            m = 4 * q + (e8 >> 1)
+            t = 2 * np.pi * i * m / 16
+            vr = np.sin(t) if e8 & 1 else np.cos(t)
+            vi = np.cos(t) if e8 & 1 else -np.sin(t)
+            rh, rl, ih, il = f[lane, :, e8]
+            assert abs(rh + rl - vr) <= 2.0 ** -22 * max(abs(vr), 2.0 ** -3)
+            assert abs(ih + il - vi) <= 2.0 ** -22 * max(abs(vi), 2.0 ** -3)
+            assert rh == np.float64(np.float16(vr)) and ih == np.float64(np.float16(vi))
+            assert rl == np.float64(np.float16(vr - rh)) and il == np.float64(np.float16(vi - ih))
+
+
+def test_split_product16_reproduces_a_dft_to_fp32_accuracy():
+    f = fragments16().astype(np.float64)
+    rng = np.random.default_rng(5)
+    x = (rng.standard_normal(16) + 1j * rng.standard_normal(16)) * 300.0
+    xr = np.empty(32)
+    xr[0::2], xr[1::2] = x.real, x.imag
+    xh = xr.astype(np.float16).astype(np.float64)
+    xl = (xr - xh).astype(np.float16).astype(np.float64)
+    A = np.zeros((2, 2, 16, 32))
+    for lane in range(64):
+        i, q = lane % 16, lane // 16
+        for e8 in range(8):
+            K = 8 * q + e8
+            A[0, 0, i, K], A[0, 1, i, K], A[1, 0, i, K], A[1, 1, i, K] = f[lane, :, e8]
+    y = np.fft.fft(x)
+    got_r = A[0, 0] @ xh + A[0, 0] @ xl + A[0, 1] @ xh
+    got_i = A[1, 0] @ xh + A[1, 0] @ xl + A[1, 1] @ xh
+    err = np.linalg.norm((got_r + 1j * got_i) - y) / np.linalg.norm(y)
+    assert err < 1e-6, err
