@@ -3,7 +3,7 @@ or 1024-cell unbounded z direction (transform length 512 = 32 x 16, 1024 =
 32 x 32, or 2048 as two 1024-point residues) runs on the matrix cores as
 FP16 hi/lo DFT-32 and DFT-16 products with FP32 accumulation; every other
 pass is FP32.
-Bar: rel-L2 <= 1e-6 against the FP64 oracle (the north-star tolerance of the
+Bar: rel-L2 <= 1e-6 against the FP64 oracle (the tolerance of the
 reduced-precision form).  x and y stay small so the oracle finishes in
 seconds."""
 import numpy as np

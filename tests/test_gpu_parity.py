@@ -1,6 +1,6 @@
 """HIP path vs the oracle (FP64 restatement of the reference) on the same
-seeded inputs.  FP64: rel-L2 <= 1e-12 (the north-star bar for the FP64 GPU
-path).  FP32: rel-L2 <= 2e-6 against the FP64 oracle."""
+seeded inputs.  FP64: rel-L2 <= 1e-12 (the bar for the FP64 GPU path).
+FP32: rel-L2 <= 2e-6 against the FP64 oracle."""
 import numpy as np
 import pytest
 
