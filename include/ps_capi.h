@@ -205,6 +205,10 @@ int ps_dft32_fragments(unsigned short* out, int n);
 /* the same for the 16-point DFT matrix (transforms of 512; n >= 2048
    entries, layout [lane 64][Yr hi, Yr lo, Yi hi, Yi lo][8]) */
 int ps_dft16_fragments(unsigned short* out, int n);
+/* 16 x 16 tables W32^f(i,k) (W32 = exp(-2 pi i / 32)) in the same layout, which the
+   matrix-core z pass of 1024 uses for its even/odd 16-point products: kind 0 the
+   16-point DFT (f = 2ik), 1 P (f = k(2i+1)), 2 Q (f = i(2k+1)); n >= 2048 */
+int ps_dft16_table(unsigned short* out, int n, int kind);
 /* kernel family of the one-component z pass chosen at setup: 0 runtime-plan,
    1 compile-time specialised, 2 pencil-per-wavefront, 3 matrix-core (PS_MIXED),
    4 two-residue (FP32 z pencils of 2048 as two 1024-point transforms);
@@ -240,9 +244,11 @@ typedef struct ps_tuning {
     int halfb_z;      /* z tile-width divisor 0/1/2 -> B, B/2, B/4; -1 = planner: B/2 for z
                          transforms of 512 or shorter, else B (PS_TUNE_HALFB_Z, -1) */
     int halfb_y;      /* y tile-width divisor 0/1 -> B, B/2 (PS_TUNE_HALFB_Y, 0) */
-    int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT-32) kernels where the length
-                         has them: 1 one tile per workgroup, 2 persistent workgroups that
-                         prefetch; 0 the FP32 kernels (PS_TUNE_MFMA, 1) */
+    int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT) kernels where the length
+                         has them: z transforms of 1024: 1 the persistent register-pencil
+                         kernel (one pencil per wavefront, 16-column tiles), 2 one 8-pencil
+                         tile per workgroup; of 2048: 1 four pencils per workgroup, 2 eight;
+                         0 the FP32 kernels (PS_TUNE_MFMA, 1) */
     int t2d;          /* 2D contexts whose fused y pass holds one or two columns per workgroup
                          (long pencils): the x pass output is transposed so that y pencils are
                          contiguous, and grad / curl transform their scalar once (PS_TUNE_T2D, 1) */

@@ -81,6 +81,7 @@ SIGNATURES = {
     "ps_zpass_variant": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_dft32_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
     "ps_dft16_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
+    "ps_dft16_table": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int, ctypes.c_int]),
     "ps_tuning_default": (ctypes.c_int, [_tun_p]),
     "ps_set_tuning": (ctypes.c_int, [ctypes.c_void_p, _tun_p]),
     "ps_get_tuning": (ctypes.c_int, [ctypes.c_void_p, _tun_p]),

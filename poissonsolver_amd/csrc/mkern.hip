@@ -1,6 +1,7 @@
 // Mixed-precision (matrix-core) pass kernels: instantiation and launchers.
 #include "ps_mfma.hpp"
 #include "ps_mfma_launch.hpp"
+#include "ps_mfma_reg.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -57,13 +58,17 @@ void zmfma_fragments(unsigned short* out)
     }
 }
 
-void zmfma_fragments16(unsigned short* out)
+// 16 x 16 tables W32^f(i, k) in the layout of zmfma_fragments (row i = lane &
+// 15, K points k = 4 (lane >> 4) + e8 / 2): kind 0 the 16-point DFT
+// W16^(ik) (f = 2ik), kind 1 P (f = k (2i + 1)), kind 2 Q (f = i (2k + 1))
+void zmfma_fragments16k(unsigned short* out, int kind)
 {
     for (int lane = 0; lane < 64; ++lane) {
         const int i = lane & 15, qa = lane >> 4;
         for (int e8 = 0; e8 < 8; ++e8) {
             const int m = 4 * qa + (e8 >> 1);
-            const int jj = (2 * i * m) & 31;                  // 2 pi i m / 16 = 2 pi (2 i m) / 32
+            const int f = (kind == 0) ? 2 * i * m : (kind == 1) ? m * (2 * i + 1) : i * (2 * m + 1);
+            const int jj = f & 31;
             const double cs = Cos32::c[jj];
             const double sn = Cos32::c[(jj + 24) & 31];
             const double vr = (e8 & 1) ? sn : cs;
@@ -78,6 +83,11 @@ void zmfma_fragments16(unsigned short* out)
     }
 }
 
+void zmfma_fragments16(unsigned short* out)
+{
+    zmfma_fragments16k(out, 0);
+}
+
 template <int OPK> static hipError_t launch_zmfma512_op(PencilArgs<float> a, hipStream_t s)
 {
     using M = MfmaZ512;
@@ -90,24 +100,44 @@ template <int OPK> static hipError_t launch_zmfma512_op(PencilArgs<float> a, hip
     return hipGetLastError();
 }
 
-template <int OPK> static hipError_t launch_zmfma_op(PencilArgs<float> a, int persistent, hipStream_t s)
+// whether the register-pencil kernel can run these arguments: kx-contiguous
+// rows (16-column tiles of 128 bytes), the octant in the [ky][kx][kz] layout
+// and the E16/P/Q tables after the DFT-16 one
+static bool zmfma_reg_ok(const PencilArgs<float>& a)
 {
+    return a.ia.col == 1 && a.oa.col == 1 && a.g_kz == 1 && a.mfrag16 && a.ncol > 0;
+}
+
+static int device_cus()
+{
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    return ncu;
+}
+
+template <int OPK> static hipError_t launch_zmfma_op(PencilArgs<float> a, int variant, hipStream_t s)
+{
+    hipError_t e;
+    if (variant == 1 && zmfma_reg_ok(a)) {
+        // persistent register-pencil kernel: 16-column tiles, one workgroup of
+        // 16 wavefronts per CU; four workgroups per CU over the launch so that
+        // CUs finishing early take the remaining ones
+        const size_t shm = MfmaZR::lds_bytes();
+        if ((e = allow_lds(k_zmfma1024r<OPK>, shm)) != hipSuccess) return e;
+        a.ncoltile = (a.ncol + MfmaZR::P - 1) / MfmaZR::P;
+        const long ntile = (long)a.ncoltile * a.nslab * a.ncomp;
+        if (ntile >= (1L << 31)) return hipErrorInvalidValue;
+        const long ncu = std::max(device_cus(), 8);
+        const long nwg = std::min<long>(ntile, 4 * ncu);
+        hipLaunchKernelGGL((k_zmfma1024r<OPK>), dim3((unsigned)nwg), dim3(64 * MfmaZR::NW), shm, s, a);
+        return hipGetLastError();
+    }
     const size_t shm = MfmaZ::lds_bytes();
     a.ncoltile = (a.ncol + MfmaZ::P - 1) / MfmaZ::P;
     const long nwork = (long)a.ncoltile * a.nslab * a.ncomp;
-    hipError_t e;
-    if (!persistent) {
-        if ((e = allow_lds(k_zmfma1024<OPK>, shm)) != hipSuccess) return e;
-        hipLaunchKernelGGL((k_zmfma1024<OPK>), dim3((unsigned)nwork), dim3(64 * MfmaZ::NW), shm, s, a);
-        return hipGetLastError();
-    }
-    // persistent: one workgroup per compute unit (2 wavefronts per SIMD)
-    if ((e = allow_lds(k_zmfma1024p<OPK>, shm)) != hipSuccess) return e;
-    int dev = 0, ncu = 0;
-    if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
-    if ((e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess) return e;
-    const long nwg = std::min<long>(nwork, (long)std::max(ncu, 1));
-    hipLaunchKernelGGL((k_zmfma1024p<OPK>), dim3((unsigned)nwg), dim3(64 * MfmaZ::NW), shm, s, a);
+    if ((e = allow_lds(k_zmfma1024<OPK>, shm)) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_zmfma1024<OPK>), dim3((unsigned)nwork), dim3(64 * MfmaZ::NW), shm, s, a);
     return hipGetLastError();
 }
 
@@ -160,7 +190,7 @@ template <int OPK> static hipError_t launch_zmfma2048_op(PencilArgs<float> a, in
     return p8 ? launch_zmfma2048_p<OPK, 8>(a, s) : launch_zmfma2048_p<OPK, 4>(a, s);
 }
 
-hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t s)
+hipError_t launch_zmfma(const PencilArgs<float>& a, int variant, hipStream_t s)
 {
     if (a.plan.L == 512) {
         if (a.n_in != 256 || a.n_out != 256 || !a.mfrag16) return hipErrorInvalidValue;
@@ -176,7 +206,7 @@ hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t 
     if (a.plan.L == 2048) {
         // two 1024-point residues per pencil
         if (a.n_in != 1024 || a.n_out != 1024) return hipErrorInvalidValue;
-        const int p8 = persistent ? 1 : 0;
+        const int p8 = variant == 2 ? 1 : 0;
         switch (a.op) {
         case OP_NONE: return launch_zmfma2048_op<OP_NONE>(a, p8, s);
         case OP_PLAIN: return launch_zmfma2048_op<OP_PLAIN>(a, p8, s);
@@ -188,11 +218,11 @@ hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t 
     }
     if (a.plan.L != MfmaZ::L || a.n_in != MfmaZ::L / 2 || a.n_out != MfmaZ::L / 2) return hipErrorInvalidValue;
     switch (a.op) {
-    case OP_NONE: return launch_zmfma_op<OP_NONE>(a, persistent, s);
-    case OP_PLAIN: return launch_zmfma_op<OP_PLAIN>(a, persistent, s);
-    case OP_GRAD: return launch_zmfma_op<OP_GRAD>(a, persistent, s);
-    case OP_RHSF: return launch_zmfma_op<OP_RHSF>(a, persistent, s);
-    case OP_LAPL: return launch_zmfma_op<OP_LAPL>(a, persistent, s);
+    case OP_NONE: return launch_zmfma_op<OP_NONE>(a, variant, s);
+    case OP_PLAIN: return launch_zmfma_op<OP_PLAIN>(a, variant, s);
+    case OP_GRAD: return launch_zmfma_op<OP_GRAD>(a, variant, s);
+    case OP_RHSF: return launch_zmfma_op<OP_RHSF>(a, variant, s);
+    case OP_LAPL: return launch_zmfma_op<OP_LAPL>(a, variant, s);
     default: return hipErrorInvalidValue;
     }
 }

@@ -624,7 +624,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         bool mfma_done = false;
         if constexpr (sizeof(T) == 4) {
             if (c->zmfma && nc == 1 && zmfma_supported(zp.p.L, c->bc[2] == 0, zz.op)) {
-                CK(launch_zmfma(zz, c->tune.mfma == 2, s));
+                CK(launch_zmfma(zz, c->tune.mfma, s));
                 mfma_done = true;
             } else if (c->zmfma && nc == 3 && zz.kmap == KMAP_3D && zmfma3_supported(zp.p.L, c->bc[2] == 0, zz.op)) {
                 CK(launch_zmfma3(zz, s));
@@ -897,11 +897,14 @@ template <typename T> int setup_impl(ps_ctx* c)
         c->nyp = (nz + 15) / 16 * 16;
         c->A2c = (long)c->H * c->nyp;
     }
-    c->g_layout = ((c->zwave || c->zres || c->t2d) && !c->zmfma) ? 1 : 0;
+    // octant layout: [ky][kx][kz] for the kernels that read a pencil's row
+    // along kz (wavefront, two-residue, transposed 2D and the register-pencil
+    // matrix-core kernels), [ky][kz][kx] for those reading adjacent kx
+    c->g_layout = (c->zmfma ? zmfma_reg_layout(c->M[2], tn.mfma) : (c->zwave || c->zres || c->t2d)) ? 1 : 0;
     if (c->zmfma) {
-        std::vector<unsigned short> fr(kZmfmaFragHalves + kZmfma16FragHalves);
+        std::vector<unsigned short> fr(kZmfmaFragHalves + 3 * kZmfma16FragHalves);
         zmfma_fragments(fr.data());
-        zmfma_fragments16(fr.data() + kZmfmaFragHalves);
+        for (int k = 0; k < 3; ++k) zmfma_fragments16k(fr.data() + kZmfmaFragHalves + k * kZmfma16FragHalves, k);
         CK(dev_alloc(c, &c->mfrag, fr.size() * sizeof(unsigned short)));
         CK(hipMemcpy(c->mfrag, fr.data(), fr.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
     }
@@ -1402,6 +1405,13 @@ int ps_dft16_fragments(unsigned short* out, int n)
 {
     if (!out || n < kZmfma16FragHalves) return PS_EINVAL;
     zmfma_fragments16(out);
+    return PS_OK;
+}
+
+int ps_dft16_table(unsigned short* out, int n, int kind)
+{
+    if (!out || n < kZmfma16FragHalves || kind < 0 || kind > 2) return PS_EINVAL;
+    zmfma_fragments16k(out, kind);
     return PS_OK;
 }
 

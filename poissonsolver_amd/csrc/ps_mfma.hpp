@@ -1099,213 +1099,6 @@ __global__ void __launch_bounds__(512, 4) k_zmfma512(PencilArgs<float> a)
     }
 }
 
-// Persistent form (2 wavefronts per SIMD, one workgroup per CU, 256
-// registers): workgroup g handles tiles g, g + G, g + 2G, ... (in
-// XCD-contiguous order).  The next tile's input and k-space multipliers are
-// fetched into the registers that the current tile's forward stage 1 and
-// k-space step have just released, so their latency overlaps the remaining
-// stages.
-template <int OPK>
-__global__ void __launch_bounds__(512, 2) k_zmfma1024p(PencilArgs<float> a)
-{
-    using M = MfmaZ;
-    constexpr int P = M::P, RS = M::RS, PSTR = M::PS, L = M::L;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    float2* sd = reinterpret_cast<float2*>(smem_raw);
-    float2* stw = sd + P * PSTR;
-    unsigned* smax = reinterpret_cast<unsigned*>(stw + L);
-
-    const int tid = threadIdx.x;
-    const int w = tid >> 6, l = tid & 63, q = l >> 4, j = l & 15;
-    const int p = j & 7, bsub = j >> 3;
-    const int bp32 = (l ^ 32) << 2;
-
-    const long ntot = (long)a.ncoltile * a.nslab * a.ncomp;
-    const long G = gridDim.x;
-
-    for (int i = tid; i < L; i += 512) stw[i] = a.plan.tw[i];
-    if (tid < 2 * P) smax[tid] = 0u;
-    const MfDft32 F = mf_dft32_frags(a.mfrag, l);
-
-    // lane geometry shared by every tile: butterfly / column index of block bi
-    // (the same index set serves as b, b', n2), position offsets of the
-    // points z = b + 32 (4q + e) (input) and z = 32 (4q + e) + n2 (output)
-    const long is32 = mf_pos_stride32(a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi);
-    const long os32 = mf_pos_stride32(a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
-    int bb[2];
-    long ipos[2], opos[2];
-    int goff0[2], goff1[2];
-    const int gkz = (int)a.g_kz;
-#pragma unroll
-    for (int bi = 0; bi < 2; ++bi) {
-        bb[bi] = 2 * (w + 8 * bi) + bsub;
-        ipos[bi] = boff(bb[bi], a.ia.pos_sh, a.ia.pos_lo, a.ia.pos_hi) + (long)(4 * q) * is32;
-        opos[bi] = boff(bb[bi], a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi) + (long)(4 * q) * os32;
-        goff0[bi] = (bb[bi] + 128 * q) * gkz * 4;
-        goff1[bi] = (M::L / 2 - bb[bi] - 128 * q) * gkz * 4;
-    }
-    const int gstep = 32 * gkz * 4;
-
-    // first tile: input and multipliers
-    long lin = blockIdx.x;
-    float2 x[2][4];
-    float gk[2][8];
-    MfTile T{};
-    if (lin < ntot) {
-        T = mf_tile(a, a.xcd_remap ? xcd_remap(lin, ntot) : lin, p);
-        const int colr = T.cv ? T.col : a.ncol - 1;
-        const float2* ip = reinterpret_cast<const float2*>(a.in) + T.comp * a.ia.comp + colr * a.ia.col +
-                           boff(T.slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
-        const TileK<float> tk = tile_k(a, T.col, T.slab, T.cv);
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) x[bi][e] = ip[ipos[bi] + e * is32];
-            mf_kfactors<OPK>(gk[bi], a, tk, T.comp, bb[bi], q, goff0[bi], goff1[bi], gstep);
-        }
-    }
-    __syncthreads();                       // twiddle table and max slots initialised
-
-    for (; lin < ntot; lin += G) {
-        const long nlin = lin + G;
-        const bool more = nlin < ntot;
-        const MfTile T1 = more ? mf_tile(a, a.xcd_remap ? xcd_remap(nlin, ntot) : nlin, p) : T;
-        float2* op_ = reinterpret_cast<float2*>(a.out) + T.comp * a.oa.comp + T.col * a.oa.col +
-                      boff(T.slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi);
-        float2* myp = sd + p * PSTR;
-
-        // ---- one scale per pencil: the forward stage-1 output (<= 16 sqrt 2
-        // 2^10) is the stage-2 input without rescaling ----
-        float mx = 0.0f;
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-            for (int e = 0; e < 4; ++e) mx = mf_absmax(x[bi][e], mx);
-        atomicMax(&smax[p], __float_as_uint(mx));
-        __syncthreads();
-        if (tid < P) smax[P + tid] = 0u;   // inverse-stage slots: last read before this barrier
-        float s0, inv0;
-        mf_scale(__uint_as_float(smax[p]), s0, inv0);
-
-        // ---- forward stage 1: DFT over m < 16, 32 outputs k1 per column (p, b) ----
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
-            const int b = bb[bi];
-            MfFrag B[2];
-            B[0] = mf_split4(x[bi], s0, s0, a.fzero);
-            const MfAcc d = mf_dft32_rows32i<1>(F, B);
-            // twiddle W1024^{b k1}; stored transposed as row k1, column b
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int k1 = 4 * q + e + 16 * h;
-                    const float2 y = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
-                    myp[k1 * RS + b] = cmul(y, stw[(b * k1) & (L - 1)]);
-                }
-            }
-        }
-        // the input registers are free: fetch the next tile's
-        if (more) {
-            const int colr = T1.cv ? T1.col : a.ncol - 1;
-            const float2* ip = reinterpret_cast<const float2*>(a.in) + T1.comp * a.ia.comp + colr * a.ia.col +
-                               boff(T1.slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi);
-#pragma unroll
-            for (int bi = 0; bi < 2; ++bi)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) x[bi][e] = ip[ipos[bi] + e * is32];
-        }
-        __syncthreads();
-        if (tid < P) smax[tid] = 0u;       // forward slots: last read before this barrier
-
-        // ---- forward stage 2, k-space operator, inverse stage 1 (registers) ----
-        const TileK<float> tk1 = more ? tile_k(a, T1.col, T1.slab, T1.cv) : TileK<float>{};
-        float m3 = 0.0f;
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
-            const int bp = bb[bi];
-            float2* row = myp + bp * RS;
-            float2 v0[4], v1[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                v0[e] = row[4 * q + e];
-                v1[e] = row[16 + 4 * q + e];
-            }
-            MfFrag B[2];
-            B[0] = mf_split4u(v0, a.fone);
-            B[1] = mf_split4u(v1, a.fone);
-            MfAcc d = mf_dft32_rows32i<2>(F, B);
-            // s0 X[bp + 32 q'], q' = 4q + e (r0/i0) and 16 + 4q + e (r1/i1);
-            // the k-space operator keeps the factor s0
-            float2 z0[4], z1[4];
-            float mz = 0.0f;
-            constexpr bool mimag = mf_imag_multiplier<OPK>();
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const float m0 = gk[bi][e], m1 = gk[bi][4 + e];
-                if constexpr (mimag) {
-                    // i k_d Gk: purely imaginary
-                    z0[e] = float2{-d.i0[e] * m0, d.r0[e] * m0};
-                    z1[e] = float2{-d.i1[e] * m1, d.r1[e] * m1};
-                } else {
-                    z0[e] = float2{d.r0[e] * m0, d.i0[e] * m0};
-                    z1[e] = float2{d.r1[e] * m1, d.i1[e] * m1};
-                }
-                mz = mf_absmax(z0[e], mf_absmax(z1[e], mz));
-            }
-            // this block's multipliers are used: fetch the next tile's
-            if (more) mf_kfactors<OPK>(gk[bi], a, tk1, T1.comp, bp, q, goff0[bi], goff1[bi], gstep);
-            float s2, inv2;
-            mf_scale(mf_colmax(mz, bp32), s2, inv2);
-            // inverse: conj(F) z = conj(F conj(z))
-            B[0] = mf_split4(z0, s2, -s2, a.fzero);
-            B[1] = mf_split4(z1, s2, -s2, a.fzero);
-            d = mf_dft32_rows32i<2>(F, B);
-            // U[n2] = conj of the inverse DFT output (times s0 s2); apply
-            // W1024^{-n2 bp} and undo both scales: conj(U W) inv0 inv2, written
-            // in place as row bp, column n2
-            const float f = inv0 * inv2;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int n2 = 4 * q + e + 16 * h;
-                    const float2 u = h ? float2{d.r1[e], d.i1[e]} : float2{d.r0[e], d.i0[e]};
-                    const float2 t = cmul(u, stw[(n2 * bp) & (L - 1)]);
-                    const float2 r = float2{t.x * f, -t.y * f};
-                    row[n2] = r;
-                    m3 = mf_absmax(r, m3);
-                }
-            }
-        }
-        atomicMax(&smax[P + p], __float_as_uint(m3));
-        __syncthreads();
-
-        // ---- inverse stage 2: column (p, n2), inputs over b', outputs n1 < 16 ----
-        float s3, inv3;
-        mf_scale(__uint_as_float(smax[P + p]), s3, inv3);
-#pragma unroll
-        for (int bi = 0; bi < 2; ++bi) {
-            const int n2 = bb[bi];
-            float2 v0[4], v1[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                v0[e] = myp[(4 * q + e) * RS + n2];
-                v1[e] = myp[(16 + 4 * q + e) * RS + n2];
-            }
-            MfFrag B[2];
-            B[0] = mf_split4(v0, s3, -s3, a.fzero);
-            B[1] = mf_split4(v1, s3, -s3, a.fzero);
-            const MfAcc d = mf_dft32_rows16(F, B);
-            if (T.cv) {
-#pragma unroll
-                for (int e = 0; e < 4; ++e) op_[opos[bi] + e * os32] = float2{d.r0[e] * inv3, -d.i0[e] * inv3};
-            }
-        }
-        T = T1;
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Three-component operators (div, curl) of the fused z pass on the matrix
 // cores, L = 1024.  Workgroup: P3 = 4 pencils, 4 wavefronts; an MFMA column

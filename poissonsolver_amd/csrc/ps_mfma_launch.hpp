@@ -11,11 +11,17 @@ namespace psamd {
 // has a matrix-core kernel
 bool zmfma_supported(int L, int half, int op);
 
-// fused z pass on the matrix cores (FP16 hi/lo split DFT-32 stages, FP32
+// fused z pass on the matrix cores (FP16 hi/lo split DFT stages, FP32
 // accumulation); `a` as for the FP32 fused z pass, one-component operators.
-// persistent = 0: one tile per workgroup, 4 wavefronts per SIMD; 1: one
-// persistent workgroup per CU that prefetches its next tile
-hipError_t launch_zmfma(const PencilArgs<float>& a, int persistent, hipStream_t s);
+// variant (tuning field mfma): transforms of 1024 run the persistent
+// register-pencil kernel for 1 (when the arguments allow it: kx-contiguous
+// rows, the octant in the [ky][kx][kz] layout) and the one-tile-per-workgroup
+// kernel for 2; transforms of 2048 run 4-pencil workgroups for 1 and 8-pencil
+// ones for 2
+hipError_t launch_zmfma(const PencilArgs<float>& a, int variant, hipStream_t s);
+// whether a context's matrix-core z pass of length L with this tuning
+// variant reads the Gk octant in the [ky][kx][kz] layout
+inline bool zmfma_reg_layout(int L, int variant) { return L == 1024 && variant == 1; }
 
 // three-component operators (div, curl) of the fused z pass on the matrix
 // cores: inputs are components 0..2 of `a.in`, outputs components of `a.out`
@@ -31,6 +37,10 @@ void zmfma_fragments(unsigned short* out);
 // entries, [lane][rh, rl, ih, il][8 halves]
 constexpr int kZmfma16FragHalves = 64 * 4 * 8;
 void zmfma_fragments16(unsigned short* out);
+// 16 x 16 tables W32^f(i, k) in that layout: kind 0 the 16-point DFT
+// (f = 2ik), 1 P (f = k (2i + 1)), 2 Q (f = i (2k + 1)); the register-pencil
+// kernel reads kinds 0, 1, 2 back to back (3 kZmfma16FragHalves entries)
+void zmfma_fragments16k(unsigned short* out, int kind);
 // round-to-nearest-even binary16 bit pattern of a finite double below 65504
 // in magnitude, and its value
 unsigned short f16_bits(double v);

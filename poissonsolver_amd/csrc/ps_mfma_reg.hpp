@@ -1,0 +1,484 @@
+// Mixed-precision z pass on the matrix cores, transform length L = 1024
+// (Mz of a 512-cell unbounded z direction), register-pencil form.
+//
+// One wavefront owns one pencil (kx, ky) for the whole fused pass (pad nz ->
+// Mz, FFT, k-space operator, IFFT, keep nz; the reference's
+// solve3d.cpp:243-441).  The 1024-point transform is the four-step
+//   X[k1 + 32 k2] = sum_b W32^(b k2) W1024^(b k1) sum_{m<16} W32^(m k1) x[b + 32 m]
+//   x[32 n1 + n2] = sum_k1 W32^(-n1 k1) W1024^(-n2 k1) sum_k2 W32^(-n2 k2) Z[k1 + 32 k2]
+// whose DFT stages are v_mfma_f32_16x16x32_f16 products with fp16 hi/lo split
+// operands (A_hi B_hi + A_hi B_lo + A_lo B_hi, fp32 accumulation).  The
+// pencil never leaves the wavefront's registers between stages: an MFMA
+// accumulator D[i][j] (column j on lanes j mod 16, rows i in groups of four
+// per lane group) is, element for element, both the B operand with K = i and
+// the A operand of the transpose (rows j, K = i).  Stage 1 multiplies the data
+// (A) by the DFT (B); stage 2 the DFT (A) by the twiddled stage-1 result (B);
+// inverse stage 1 the conjugated spectrum (A) by the DFT (B); inverse stage 2
+// the DFT (A) by the twiddled result (B).  No exchange between stages, so no
+// workgroup barrier inside the transform.
+//
+// Every 32-point stage is split into two 16-point products:
+//   stage 1: even k1 by W16^(m k1'), odd k1 by W32^(m (2 k1' + 1))
+//   stage 2: even b by W16^(k2 b'), odd b by W32^(k2 (2 b' + 1)); rows k2 and
+//            k2 + 16 are E + O and E - O
+//   inverse 1: A = Z[k2] + Z[k2 + 16] gives the even n2 by W16^(n' k2),
+//            B = Z[k2] - Z[k2 + 16] the odd n2 by W32^(k2 (2 n' + 1))
+//   inverse 2: even k1 by W16^(n1 k1'), odd k1 by W32^(n1 (2 k1' + 1))
+// (six MFMAs per 16 x 16 x 16 complex product; 96 per pencil), with three
+// 16 x 16 fp16 hi/lo tables: E16 = W16^(ik), P = W32^(k (2i + 1)),
+// Q = W32^(i (2k + 1)).  The data index layout follows: lane j of a stage-1
+// operand holds b = 2 j + tb, a stage-1 output column k1 = 2 j + tk.
+//
+// Workgroup: 16 wavefronts (one per CU, persistent), tiles of 16 adjacent kx
+// columns of one ky slab, so every HBM row access is a full 128-byte line.
+// A tile (512 z rows x 128 B) is fetched by direct global->LDS loads into
+// one of two LDS buffers while the other tile is transformed; each wavefront
+// reads its pencil (one column) from the buffer, transforms it and writes
+// the result back into the same column; the workgroup then stores the tile.
+//
+// Scales: fp16 spans 2^-24 .. 65504, so every split input is scaled by a
+// power of two mapping its largest magnitude below 2^10 (per pencil: a
+// wavefront-wide maximum); stage-2 inputs (<= 16 sqrt 2 2^10) need none.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ps_mfma.hpp"
+
+namespace psamd {
+
+struct MfmaZR {
+    static constexpr int L = 1024, NZ = 512, P = 16, NW = 16;
+    static constexpr int STG = NZ * P * 8;            // 64 KB per staging buffer
+    static constexpr int TWP = 33;                    // twiddle-matrix row pitch (complex)
+    static constexpr int TWB = 32 * TWP * 8;
+    static constexpr int FTB = 3 * 4 * 64 * 16;       // [E16, P, Q][rh, ih, rl, il][lane]
+    static constexpr size_t lds_bytes() { return 2 * (size_t)STG + TWB + FTB; }
+};
+
+typedef __attribute__((address_space(3))) void* mr_lds_ptr;
+
+// Byte offset of chunk c (pencils 2c, 2c + 1) of row z in a staging buffer:
+// 1 KB blocks of 8 rows (one direct load instruction each); the chunk slots
+// of row r of block blk are XOR-permuted by r and the block parity, so that a
+// pencil column read at consecutive rows spreads over the LDS banks.
+__device__ __forceinline__ unsigned mr_stg_off(int z, int c)
+{
+    const int r = z & 7, blk = z >> 3;
+    return (unsigned)(blk * 1024 + r * 128 + ((c ^ r ^ (blk & 1)) << 4));
+}
+
+// Wavefront-wide maximum of non-negative floats (their bit patterns order as
+// unsigned integers): DPP within rows of 16 lanes, then row broadcasts;
+// returned uniform.
+__device__ __forceinline__ float mr_wave_max(float mf)
+{
+    unsigned m = __float_as_uint(mf);
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0xB1, 0xF, 0xF, false));    // quad [1,0,3,2]
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x4E, 0xF, 0xF, false));    // quad [2,3,0,1]
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x141, 0xF, 0xF, false));   // row half mirror
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp(0, (int)m, 0x140, 0xF, 0xF, false));   // row mirror
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x142, 0xA, 0xF, false));   // row bcast 15
+    m = max(m, (unsigned)__builtin_amdgcn_update_dpp((int)m, (int)m, 0x143, 0xC, 0xF, false));   // row bcast 31
+    return __uint_as_float((unsigned)__builtin_amdgcn_readlane((int)m, 63));
+}
+
+// Power-of-two scale s with s m < 2^10 and its inverse, from the exponent
+// bits of the uniform maximum m >= 0 (clamped to 2^+-100 as mf_scale).
+__device__ __forceinline__ void mr_scale(float m, float& s, float& inv)
+{
+    int e = (int)((__float_as_uint(m) >> 23) & 255) - 126;   // m < 2^e for normal m
+    e = e < -100 ? -100 : (e > 100 ? 100 : e);
+    s = __uint_as_float((unsigned)(127 + 10 - e) << 23);
+    inv = __uint_as_float((unsigned)(127 - 10 + e) << 23);
+}
+
+// largest |re|, |im| of loaded values, on the bit patterns (no NaN
+// canonicalisation of memory operands)
+__device__ __forceinline__ unsigned mr_absmax_bits(float2 v, unsigned m)
+{
+    return max(m, max(__float_as_uint(v.x) & 0x7fffffffu, __float_as_uint(v.y) & 0x7fffffffu));
+}
+
+__device__ __forceinline__ float mr_absmax(float2 v, float m)
+{
+    return __builtin_fmaxf(m, __builtin_fmaxf(__builtin_fabsf(v.x), __builtin_fabsf(v.y)));
+}
+
+// fp16 hi/lo split of four complex values already below 65504: hi rounded
+// toward zero (the remainder x - hi is exact in fp32 and keeps the lost bits)
+__device__ __forceinline__ MfFrag mr_split4u(const float2 (&v)[4], float one)
+{
+    unsigned H[4], Lo[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const mf_h2 h = __builtin_bit_cast(mf_h2, __builtin_amdgcn_cvt_pkrtz(v[e].x, v[e].y));
+        const mf_h2 r = mf_h2{(_Float16)__builtin_fmaf(v[e].x, one, -(float)h.x),
+                              (_Float16)__builtin_fmaf(v[e].y, one, -(float)h.y)};
+        H[e] = __builtin_bit_cast(unsigned, h);
+        Lo[e] = __builtin_bit_cast(unsigned, r);
+    }
+    MfFrag f;
+    f.hi = __builtin_bit_cast(mf_h8, H);
+    f.lo = __builtin_bit_cast(mf_h8, Lo);
+    return f;
+}
+
+// the four fragments of one 16 x 16 table (real-form rows Yr, Yi, hi and lo)
+struct MfT16 {
+    mf_h8 rh, ih, rl, il;
+};
+
+// table tb (0 E16, 1 P, 2 Q) from LDS; the offset is opaque to the compiler
+// so each product loads its table where it is used rather than holding all
+// three in registers across the tile loop
+__device__ __forceinline__ MfT16 mr_table(const mf_h8* ft, int tb, int l)
+{
+    int o = tb * 4 * 64;
+    asm volatile("" : "+s"(o));
+    const mf_h8* q = ft + o + l;
+    MfT16 f;
+    f.rh = q[0];
+    f.ih = q[64];
+    f.rl = q[128];
+    f.il = q[192];
+    return f;
+}
+
+// data (A operand, rows on lanes) times a table (B operand): r, i += ...;
+// FIRST starts the accumulation chain from zero
+template <bool FIRST>
+__device__ __forceinline__ void mr_mfma_a(mf_f4& r, mf_f4& i, const MfFrag& X, const MfT16& F)
+{
+    const mf_f4 zr = FIRST ? mf_f4{0, 0, 0, 0} : r, zi = FIRST ? mf_f4{0, 0, 0, 0} : i;
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(X.hi, F.rh, zr, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(X.hi, F.ih, zi, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(X.lo, F.rh, r, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(X.lo, F.ih, i, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(X.hi, F.rl, r, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(X.hi, F.il, i, 0, 0, 0);
+}
+
+// a table (A operand) times data (B operand, columns on lanes)
+template <bool FIRST>
+__device__ __forceinline__ void mr_mfma_b(mf_f4& r, mf_f4& i, const MfT16& F, const MfFrag& X)
+{
+    const mf_f4 zr = FIRST ? mf_f4{0, 0, 0, 0} : r, zi = FIRST ? mf_f4{0, 0, 0, 0} : i;
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh, X.hi, zr, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih, X.hi, zi, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rh, X.lo, r, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.ih, X.lo, i, 0, 0, 0);
+    r = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.rl, X.hi, r, 0, 0, 0);
+    i = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.il, X.hi, i, 0, 0, 0);
+}
+
+// tile t of the launch: component, ky slab (paired order), first kx column
+struct MrTile {
+    int comp, slab, col0;
+};
+
+__device__ __forceinline__ MrTile mr_tile(const PencilArgs<float>& a, int t, int ntot)
+{
+    int lin = t;
+    if (a.xcd_remap) {
+        // the grid is a multiple of 8 and workgroup g runs on XCD g mod 8:
+        // give each XCD a contiguous range of tiles
+        const int per = ntot >> 3;
+        if (lin < (per << 3)) lin = (lin & 7) * per + (lin >> 3);
+    }
+    const int ntile = a.ncoltile * a.nslab;
+    MrTile z;
+    z.comp = lin / ntile;
+    const int rem = lin - z.comp * ntile;
+    const int slab0 = rem / a.ncoltile;
+    const int ct = rem - slab0 * a.ncoltile;
+    z.slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    z.col0 = ct * 16;
+    return z;
+}
+
+// 32-bit byte offset of position z along a pencil (plain or z-blocked)
+__device__ __forceinline__ unsigned mr_zoff(const Addr& d, int z)
+{
+    return (d.pos_sh >= 30) ? (unsigned)z * (unsigned)d.pos_lo
+                            : (unsigned)(z >> d.pos_sh) * (unsigned)d.pos_hi +
+                                  (unsigned)(z & ((1 << d.pos_sh) - 1)) * (unsigned)d.pos_lo;
+}
+
+// Wavefront w fetches blocks n = w + 16 k (k < 4: rows 8n .. 8n + 7) of tile T
+// into buf: lane l loads 16 bytes of row 8n + (l >> 3) into slot l of the
+// block, i.e. chunk (l & 7) ^ r ^ parity of that row (the staging permutation).
+__device__ __forceinline__ void mr_fetch(const PencilArgs<float>& a, const MrTile& T, unsigned char* buf, int w, int l)
+{
+    const char* ib = reinterpret_cast<const char*>(a.in) +
+                     ((long)T.comp * a.ia.comp + (long)T.col0 * a.ia.col + boff(T.slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi)) * 8;
+    const int r = l >> 3;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int n = w + 16 * k;
+        const int c = (l & 7) ^ r ^ (w & 1);   // block parity n & 1 = w & 1
+        const char* gp = ib + (mr_zoff(a.ia, 8 * n + r) + 2u * c) * 8u;
+        __builtin_amdgcn_global_load_lds(gp, (mr_lds_ptr)(buf + n * 1024), 16, 0, 0);
+    }
+}
+
+// k-space multipliers of lane (j, g)'s 16 spectrum points kz = k1 + 32 k2,
+// k1 = 2 j + tt, k2 = 4 g + e + 16 h, for pencil column col of tile T.  The
+// octant is [ky][kx][kz] (g_kz = 1): kz < 512 for h = 0, and the octant row of
+// h = 1 is Mz - kz = 512 - (kz of h = 0), so all loads are base + immediate.
+template <int OPK>
+__device__ __forceinline__ void mr_kfactors(float (&gm)[2][2][4], const PencilArgs<float>& a, const MrTile& T, int col,
+                                            int j, int g)
+{
+    const TileK<float> tk = tile_k(a, col, T.slab, col < a.ncol);
+    const MfKCoef kc = mf_kcoef<OPK>(a, tk, T.comp);
+    const int kb = 2 * j + 128 * g;
+    const char* gr = reinterpret_cast<const char*>(kc.grow);
+    const unsigned gb0 = (unsigned)kb * 4u, gb1 = (unsigned)(512 - kb) * 4u;
+#pragma unroll
+    for (int tt = 0; tt < 2; ++tt)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int kz0 = kb + tt + 32 * e;
+                const int kz = h ? kz0 + 512 : kz0;
+                if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_LAPL) {
+                    const unsigned go = h ? gb1 - (unsigned)(tt + 32 * e) * 4u : gb0 + (unsigned)(tt + 32 * e) * 4u;
+                    gm[tt][h][e] = mf_kval<OPK>(a, kc, kz, *reinterpret_cast<const float*>(gr + go));
+                } else {
+                    gm[tt][h][e] = (OPK == OP_NONE) ? 1.0f : mf_rhsf(a, tk, kz);
+                }
+            }
+}
+
+// One tile for wavefront w: the multipliers of its pencil, the fetch of the
+// next tile into nxt (issued once the multipliers have arrived: vector memory
+// operations complete in order, and a later wait of this tile must not wait
+// for the fetch), the transform of column w of cur, written back in place.
+// cur, nxt, the twiddle matrix and the tables are disjoint LDS regions; as
+// __restrict__ parameters of one function the compiler can tell that this
+// tile's LDS reads do not depend on the fetch into nxt.
+template <int OPK>
+__device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                        unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
+                                        const mf_h8* __restrict__ ft, int t, int G, int ntot, int w, int l)
+{
+    constexpr int TWP = MfmaZR::TWP;
+    constexpr bool mimag = mf_imag_multiplier<OPK>();
+    const int g = l >> 4, j = l & 15;
+    const int p = w;
+    float gm[2][2][4];
+    {
+        const MrTile T = mr_tile(a, t, ntot);
+        mr_kfactors<OPK>(gm, a, T, T.col0 + p, j, g);
+    }
+    // staging offsets of rows z = 2 j + q + 32 (4 g + e) (q = 0, 1; 128 g + 32 e
+    // is a whole number of blocks): z & 7 = 2 (j & 3) + q, block parity (j >> 2) & 1
+    unsigned sb[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = 2 * (j & 3) + q, par = (j >> 2) & 1;
+        sb[q] = (unsigned)(((j >> 2) + 16 * g) * 1024 + r * 128 + ((((p >> 1) ^ r ^ par) & 7) << 4) + (p & 1) * 8);
+    }
+    // ---- input x[b + 32 m], b = 2 j + tb, m = 4 g + e (m >= 16 is padding) ----
+    float2 x[2][4];
+    unsigned mx = 0u;
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            x[tb][e] = *reinterpret_cast<const float2*>(cur + sb[tb] + e * 4096);
+            mx = mr_absmax_bits(x[tb][e], mx);
+        }
+    float s0, inv0;
+    mr_scale(mr_wave_max(__uint_as_float(mx)), s0, inv0);
+
+    // ---- stage 1 (data x table): Y[b][k1] = sum_m W32^(m k1) x[b + 32 m] ----
+    // yr/yi[tb][tk]: rows b = 2 (4 g + e) + tb, column k1 = 2 j + tk
+    mf_f4 yr[2][2], yi[2][2];
+    {
+        const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) {
+            const MfFrag X = mf_split4(x[tb], s0, s0, a.fzero);
+            mr_mfma_a<true>(yr[tb][0], yi[tb][0], X, Fe);
+            mr_mfma_a<true>(yr[tb][1], yi[tb][1], X, Fo);
+        }
+    }
+    // twiddle W1024^(b k1): matrix row k1 = 2 j + tk, column b = 8 g + 2 e + tb
+    const float2* twl = twm + 2 * j * TWP + 8 * g;
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int tk = 0; tk < 2; ++tk)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float2 v = cmul(float2{yr[tb][tk][e], yi[tb][tk][e]}, twl[tk * TWP + 2 * e + tb]);
+                yr[tb][tk][e] = v.x;
+                yi[tb][tk][e] = v.y;
+            }
+    // ur/ui[tk][tn]: rows k1 = 2 (4 g + e) + tk, column n2 = 2 j + tn; tile tk
+    // carries the factor s0 s2[tk]
+    mf_f4 ur[2][2], ui[2][2];
+    float inv2[2];
+    float m3 = 0.0f;
+#pragma unroll
+    for (int tk = 0; tk < 2; ++tk) {
+        // ---- stage 2 (table x data) for column k1 = 2 j + tk ----
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = float2{yr[0][tk][e], yi[0][tk][e]};
+            v1[e] = float2{yr[1][tk][e], yi[1][tk][e]};
+        }
+        const MfFrag B0 = mr_split4u(v0, a.fone), B1 = mr_split4u(v1, a.fone);
+        mf_f4 er, ei, orr, oi;
+        mr_mfma_b<true>(er, ei, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(orr, oi, mr_table(ft, 2, l), B1);
+        // X[kz] (kz = k1 + 32 k2, k2 = 4 g + e and + 16) and the k-space operator
+        float2 z0[4], z1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float m0 = gm[tk][0][e], m1 = gm[tk][1][e];
+            const float xr0 = er[e] + orr[e], xi0 = ei[e] + oi[e];
+            const float xr1 = er[e] - orr[e], xi1 = ei[e] - oi[e];
+            if constexpr (mimag) {
+                // i k_d Gk: purely imaginary
+                z0[e] = float2{-xi0 * m0, xr0 * m0};
+                z1[e] = float2{-xi1 * m1, xr1 * m1};
+            } else {
+                z0[e] = float2{xr0 * m0, xi0 * m0};
+                z1[e] = float2{xr1 * m1, xi1 * m1};
+            }
+        }
+        if (tk == 0) {
+            // every multiplier has arrived: the next tile's fetch now
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) asm volatile("" ::"v"(gm[1][h][e]));
+            if (t + G < ntot) mr_fetch(a, mr_tile(a, t + G, ntot), nxt, w, l);
+        }
+        // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
+        float2 za[4], zb[4];
+        float mz = 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            za[e] = float2{z0[e].x + z1[e].x, z0[e].y + z1[e].y};
+            zb[e] = float2{z0[e].x - z1[e].x, z0[e].y - z1[e].y};
+            mz = mr_absmax(zb[e], mr_absmax(za[e], mz));
+        }
+        float s2;
+        mr_scale(mr_wave_max(mz), s2, inv2[tk]);
+        {
+            const MfFrag XA = mf_split4(za, s2, -s2, a.fzero), XB = mf_split4(zb, s2, -s2, a.fzero);
+            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
+        }
+        // T = W1024^(n2 k1) V (V = conj of the inverse transform's partial)
+        float mt = 0.0f;
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float2 v = cmul(float2{ur[tk][tn][e], ui[tk][tn][e]}, twl[tn * TWP + 2 * e + tk]);
+                ur[tk][tn][e] = v.x;
+                ui[tk][tn][e] = v.y;
+                mt = mr_absmax(v, mt);
+            }
+        m3 = __builtin_fmaxf(m3, mt * inv2[tk]);
+    }
+    float s3, inv3;
+    mr_scale(mr_wave_max(m3), s3, inv3);
+    const float sa = s3 * inv2[0], sb2 = s3 * inv2[1];   // both k1 tiles at the common scale s0 s3
+    const float fo = inv0 * inv3;
+    // ---- inverse stage 2 (table x data): x[32 n1 + n2] = conj(sum_k1 W32^(n1 k1) T[k1][n2]), n1 < 16 ----
+#pragma unroll
+    for (int tn = 0; tn < 2; ++tn) {
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = float2{ur[0][tn][e], ui[0][tn][e]};
+            v1[e] = float2{ur[1][tn][e], ui[1][tn][e]};
+        }
+        const MfFrag B0 = mf_split4(v0, sa, sa, a.fzero), B1 = mf_split4(v1, sb2, sb2, a.fzero);
+        mf_f4 dr, di;
+        mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
+        mr_mfma_b<false>(dr, di, mr_table(ft, 2, l), B1);
+        // rows z = 32 (4 g + e) + 2 j + tn
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            *reinterpret_cast<float2*>(cur + sb[tn] + e * 4096) = float2{dr[e] * fo, -di[e] * fo};
+    }
+}
+
+// Persistent kernel: workgroup g handles tiles g, g + G, g + 2G, ... (G = the
+// grid); the first tile is fetched before the loop, each later one during
+// the tile before it.  Every wavefront of a workgroup runs the same
+// iterations, so the barriers are uniform.
+template <int OPK>
+__global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
+{
+    using M = MfmaZR;
+    constexpr int L = M::L, TWP = M::TWP;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    unsigned char* stg0 = smem_raw;
+    float2* twm = reinterpret_cast<float2*>(smem_raw + 2 * M::STG);
+    mf_h8* ftw = reinterpret_cast<mf_h8*>(smem_raw + 2 * M::STG + M::TWB);
+
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l0 = tid & 63;
+    const int ntot = a.ncoltile * a.nslab * a.ncomp;
+    const int G = gridDim.x;
+
+    // twiddle matrix W1024^(r s), r, s < 32 (1024 threads, one entry each)
+    {
+        const int r = tid >> 5, s = tid & 31;
+        twm[r * TWP + s] = a.plan.tw[(r * s) & (L - 1)];
+    }
+    if (tid < 3 * 4 * 64) {
+        // tables E16, P, Q: global [table][lane][rh, rl, ih, il] -> LDS [table][rh, ih, rl, il][lane]
+        const int ln = tid & 63, f = (tid >> 6) & 3, tb = tid >> 8;
+        const int fo = (f == 0) ? 0 : (f == 1) ? 2 : (f == 2) ? 1 : 3;
+        ftw[(tb * 4 + fo) * 64 + ln] = reinterpret_cast<const mf_h8*>(a.mfrag16)[(tb * 64 + ln) * 4 + f];
+    }
+    int t = blockIdx.x;
+    if (t < ntot) mr_fetch(a, mr_tile(a, t, ntot), stg0, w, l0);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    for (int it = 0; t < ntot; t += G, ++it) {
+        // lane indices recomputed per tile (an opaque copy): lane-dependent
+        // offsets are not held in registers across the loop
+        int l = l0;
+        asm volatile("" : "+v"(l));
+        unsigned char* cur = stg0 + (it & 1) * M::STG;
+        unsigned char* nxt = stg0 + ((it & 1) ^ 1) * M::STG;
+        const MrTile T = mr_tile(a, t, ntot);
+        mr_step<OPK>(a, cur, nxt, twm, ftw, t, G, ntot, w, l);
+        __syncthreads();
+        // this wavefront's fetch of the next tile (and its earlier stores) done
+        __builtin_amdgcn_s_waitcnt(0);
+        // ---- store of the tile: 16 B per lane, full 128-byte rows ----
+        {
+            char* ob = reinterpret_cast<char*>(a.out) +
+                       ((long)T.comp * a.oa.comp + (long)T.col0 * a.oa.col + boff(T.slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi)) * 8;
+            int tq = tid;
+            asm volatile("" : "+v"(tq));
+            const int c = tq & 7;
+            const int nv = a.ncol - T.col0 - 2 * c;   // valid columns of this lane's pair
+            float4 v[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) v[k] = *reinterpret_cast<const float4*>(cur + mr_stg_off((tq >> 3) + 128 * k, c));
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                char* o = ob + (mr_zoff(a.oa, (tq >> 3) + 128 * k) + 2u * c) * 8u;
+                if (nv >= 2) *reinterpret_cast<float4*>(o) = v[k];
+                else if (nv == 1) *reinterpret_cast<float2*>(o) = float2{v[k].x, v[k].y};
+            }
+        }
+        __syncthreads();   // every wavefront has read cur: the fetch after next may overwrite it
+    }
+}
+
+}  // namespace psamd
