@@ -105,7 +105,8 @@ template <int OPK> static hipError_t launch_zmfma512_op(PencilArgs<float> a, hip
 // and the E16/P/Q tables after the DFT-16 one
 static bool zmfma_reg_ok(const PencilArgs<float>& a)
 {
-    return a.ia.col == 1 && a.oa.col == 1 && a.g_kz == 1 && a.mfrag16 && a.ncol > 0;
+    auto zok = [](const Addr& d) { return d.pos_sh >= 30 || (d.pos_sh >= 3 && d.pos_sh <= 7); };
+    return a.ia.col == 1 && a.oa.col == 1 && a.g_kz == 1 && a.mfrag16 && a.ncol > 0 && zok(a.ia) && zok(a.oa);
 }
 
 static int device_cus()
@@ -126,10 +127,10 @@ template <int OPK> static hipError_t launch_zmfma_op(PencilArgs<float> a, int va
         const size_t shm = MfmaZR::lds_bytes();
         if ((e = allow_lds(k_zmfma1024r<OPK>, shm)) != hipSuccess) return e;
         a.ncoltile = (a.ncol + MfmaZR::P - 1) / MfmaZR::P;
-        const long ntile = (long)a.ncoltile * a.nslab * a.ncomp;
-        if (ntile >= (1L << 31)) return hipErrorInvalidValue;
+        const long nunit = (long)a.ncoltile * ((a.nslab + 1) / 2) * a.ncomp;   // slab pairs x column tiles
+        if ((long)a.ncoltile * a.nslab * a.ncomp >= (1L << 31)) return hipErrorInvalidValue;
         const long ncu = std::max(device_cus(), 8);
-        const long nwg = std::min<long>(ntile, 4 * ncu);
+        const long nwg = std::min<long>(nunit, 4 * ncu);
         hipLaunchKernelGGL((k_zmfma1024r<OPK>), dim3((unsigned)nwg), dim3(64 * MfmaZR::NW), shm, s, a);
         return hipGetLastError();
     }

@@ -171,37 +171,74 @@ __device__ __forceinline__ void mr_mfma_b(mf_f4& r, mf_f4& i, const MfT16& F, co
     i = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.il, X.hi, i, 0, 0, 0);
 }
 
-// tile t of the launch: component, ky slab (paired order), first kx column
+// Work units: unit u is (component, slab pair, column tile) and holds the
+// tiles of slabs 2 sp and 2 sp + 1 of the paired slab order (ky and My - ky,
+// which share a row of the kernel octant), transformed back to back by one
+// workgroup so that the second reuses the first's multipliers.
 struct MrTile {
     int comp, slab, col0;
+    int kyo;     // octant row min(ky, My - ky)
 };
 
-__device__ __forceinline__ MrTile mr_tile(const PencilArgs<float>& a, int t, int ntot)
+__device__ __forceinline__ int mr_nunit(const PencilArgs<float>& a)
 {
-    int lin = t;
-    if (a.xcd_remap) {
-        // the grid is a multiple of 8 and workgroup g runs on XCD g mod 8:
-        // give each XCD a contiguous range of tiles
-        const int per = ntot >> 3;
-        if (lin < (per << 3)) lin = (lin & 7) * per + (lin >> 3);
-    }
-    const int ntile = a.ncoltile * a.nslab;
-    MrTile z;
-    z.comp = lin / ntile;
-    const int rem = lin - z.comp * ntile;
-    const int slab0 = rem / a.ncoltile;
-    const int ct = rem - slab0 * a.ncoltile;
-    z.slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
-    z.col0 = ct * 16;
-    return z;
+    return a.ncomp * ((a.nslab + 1) >> 1) * a.ncoltile;
 }
 
-// 32-bit byte offset of position z along a pencil (plain or z-blocked)
-__device__ __forceinline__ unsigned mr_zoff(const Addr& d, int z)
+// tile m (0, 1) of unit u; false if the unit has no such tile
+__device__ __forceinline__ bool mr_tile(const PencilArgs<float>& a, int u, int m, int nunit, MrTile& z)
 {
-    return (d.pos_sh >= 30) ? (unsigned)z * (unsigned)d.pos_lo
-                            : (unsigned)(z >> d.pos_sh) * (unsigned)d.pos_hi +
-                                  (unsigned)(z & ((1 << d.pos_sh) - 1)) * (unsigned)d.pos_lo;
+    int lin = u;
+    if (a.xcd_remap) {
+        // the grid is a multiple of 8 and workgroup g runs on XCD g mod 8:
+        // give each XCD a contiguous range of units
+        const int per = nunit >> 3;
+        if (lin < (per << 3)) lin = (lin & 7) * per + (lin >> 3);
+    }
+    const int nunit1 = ((a.nslab + 1) >> 1) * a.ncoltile;
+    z.comp = lin / nunit1;
+    const int rem = lin - z.comp * nunit1;
+    const int sp = rem / a.ncoltile;
+    const int ct = rem - sp * a.ncoltile;
+    const int slab0 = 2 * sp + m;
+    z.slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
+    z.col0 = ct * 16;
+    z.kyo = (z.slab <= a.nslab - z.slab) ? z.slab : a.nslab - z.slab;
+    return slab0 < a.nslab;
+}
+
+// whether the multipliers of component comp are even in ky, so that slabs ky
+// and My - ky (one octant row) share them: every operator except a derivative
+// along y (grad of component y)
+template <int OPK>
+__device__ __forceinline__ bool mr_even_in_ky(const PencilArgs<float>& a, int comp)
+{
+    if constexpr (OPK == OP_GRAD) return kcomp<float>(a.kmap, comp, 0.0f, 1.0f, 0.0f) == 0.0f;
+    return true;
+}
+
+// byte offset of tile T's column 0 in a component array (addressing d)
+__device__ __forceinline__ const char* mr_tile_base(const void* base, const Addr& d, const MrTile& T)
+{
+    return reinterpret_cast<const char*>(base) +
+           ((long)T.comp * d.comp + (long)T.col0 * d.col + boff(T.slab, d.slab_sh, d.slab_lo, d.slab_hi)) * 8;
+}
+
+// Position z = z0 + zr of a pencil, z0 a multiple of 8 (uniform), zr < 8 or
+// z0 a multiple of 128 and zr < 128: plain positions, or blocks of 2^sh >= 8
+// planes (2^sh <= 128 for the second form), split the byte offset into a
+// uniform part of z0 and a per-lane part of zr.
+__device__ __forceinline__ unsigned mr_zoff_u(const Addr& d, int z0)
+{
+    return (d.pos_sh >= 30) ? (unsigned)z0 * (unsigned)d.pos_lo * 8u
+                            : ((unsigned)(z0 >> d.pos_sh) * (unsigned)d.pos_hi +
+                               (unsigned)(z0 & ((1 << d.pos_sh) - 1)) * (unsigned)d.pos_lo) * 8u;
+}
+__device__ __forceinline__ unsigned mr_zoff_l(const Addr& d, int zr)
+{
+    return (d.pos_sh >= 30) ? (unsigned)zr * (unsigned)d.pos_lo * 8u
+                            : ((unsigned)(zr >> d.pos_sh) * (unsigned)d.pos_hi +
+                               (unsigned)(zr & ((1 << d.pos_sh) - 1)) * (unsigned)d.pos_lo) * 8u;
 }
 
 // Wavefront w fetches blocks n = w + 16 k (k < 4: rows 8n .. 8n + 7) of tile T
@@ -209,15 +246,14 @@ __device__ __forceinline__ unsigned mr_zoff(const Addr& d, int z)
 // block, i.e. chunk (l & 7) ^ r ^ parity of that row (the staging permutation).
 __device__ __forceinline__ void mr_fetch(const PencilArgs<float>& a, const MrTile& T, unsigned char* buf, int w, int l)
 {
-    const char* ib = reinterpret_cast<const char*>(a.in) +
-                     ((long)T.comp * a.ia.comp + (long)T.col0 * a.ia.col + boff(T.slab, a.ia.slab_sh, a.ia.slab_lo, a.ia.slab_hi)) * 8;
+    const char* ib = mr_tile_base(a.in, a.ia, T);
     const int r = l >> 3;
+    const int c = (l & 7) ^ r ^ (w & 1);   // block parity n & 1 = w & 1
+    const unsigned lo = mr_zoff_l(a.ia, r) + 16u * (unsigned)c;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int n = w + 16 * k;
-        const int c = (l & 7) ^ r ^ (w & 1);   // block parity n & 1 = w & 1
-        const char* gp = ib + (mr_zoff(a.ia, 8 * n + r) + 2u * c) * 8u;
-        __builtin_amdgcn_global_load_lds(gp, (mr_lds_ptr)(buf + n * 1024), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds(ib + mr_zoff_u(a.ia, 8 * n) + lo, (mr_lds_ptr)(buf + n * 1024), 16, 0, 0);
     }
 }
 
@@ -251,27 +287,50 @@ __device__ __forceinline__ void mr_kfactors(float (&gm)[2][2][4], const PencilAr
             }
 }
 
-// One tile for wavefront w: the multipliers of its pencil, the fetch of the
-// next tile into nxt (issued once the multipliers have arrived: vector memory
-// operations complete in order, and a later wait of this tile must not wait
-// for the fetch), the transform of column w of cur, written back in place.
-// cur, nxt, the twiddle matrix and the tables are disjoint LDS regions; as
-// __restrict__ parameters of one function the compiler can tell that this
-// tile's LDS reads do not depend on the fetch into nxt.
+// Slice k of wavefront w's share of the other buffer (block n = w + 16 k,
+// rows 8n .. 8n + 7, read and then overwritten by this wavefront only):
+// store the previous tile's output rows of the block, then fetch the next
+// tile's rows into it.  Four slices spread over a tile's transform keep the
+// stores and the fetch from arriving at the memory system as one burst.
+__device__ __forceinline__ void mr_slice(const PencilArgs<float>& a, unsigned char* __restrict__ buf, const MrTile& Tp,
+                                         bool hasprev, const MrTile& Tn, bool more, int w, int l, int k)
+{
+    const int n = w + 16 * k, r = l >> 3, c = l & 7;
+    if (hasprev) {
+        const float4 v = *reinterpret_cast<const float4*>(buf + mr_stg_off(8 * n + r, c));
+        const int nv = a.ncol - Tp.col0 - 2 * c;   // valid columns of this lane's pair
+        char* o = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp)) + mr_zoff_u(a.oa, 8 * n) + mr_zoff_l(a.oa, r) + 16u * c;
+        if (nv >= 2) *reinterpret_cast<float4*>(o) = v;
+        else if (nv == 1) *reinterpret_cast<float2*>(o) = float2{v.x, v.y};
+    }
+    if (more) {
+        const int cf = c ^ r ^ (w & 1);   // the staging permutation of row r (block parity w & 1)
+        const char* gp = mr_tile_base(a.in, a.ia, Tn) + mr_zoff_u(a.ia, 8 * n) + mr_zoff_l(a.ia, r) + 16u * cf;
+        __builtin_amdgcn_global_load_lds(gp, (mr_lds_ptr)(buf + n * 1024), 16, 0, 0);
+    }
+}
+
+// One tile for wavefront w: the transform of column w of cur with the
+// multipliers gm (already in registers), written back in place, and spread
+// over it the four slices that store the previous tile from nxt and fetch the
+// next one into it.  Nothing here waits for vector memory.  cur, nxt, the
+// twiddle matrix and the tables are disjoint LDS regions; as __restrict__
+// parameters of one function the compiler can tell that this tile's LDS
+// reads of cur do not depend on the fetch into nxt.
 template <int OPK>
 __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
                                         unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
-                                        const mf_h8* __restrict__ ft, int t, int G, int ntot, int w, int l)
+                                        const mf_h8* __restrict__ ft, const MrTile& T, const MrTile& Tp, bool hasprev,
+                                        const MrTile& Tn, bool more, bool reuse_g, float (&gm)[2][2][4], int w, int l)
 {
     constexpr int TWP = MfmaZR::TWP;
     constexpr bool mimag = mf_imag_multiplier<OPK>();
     const int g = l >> 4, j = l & 15;
     const int p = w;
-    float gm[2][2][4];
-    {
-        const MrTile T = mr_tile(a, t, ntot);
-        mr_kfactors<OPK>(gm, a, T, T.col0 + p, j, g);
-    }
+    // this tile's multipliers (the second tile of a slab pair keeps the
+    // first's); they are used before the first slice, so that their wait
+    // cannot wait for a store (loads and stores share one counter)
+    if (!reuse_g) mr_kfactors<OPK>(gm, a, T, T.col0 + p, j, g);
     // staging offsets of rows z = 2 j + q + 32 (4 g + e) (q = 0, 1; 128 g + 32 e
     // is a whole number of blocks): z & 7 = 2 (j & 3) + q, block parity (j >> 2) & 1
     unsigned sb[2];
@@ -320,8 +379,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
     // ur/ui[tk][tn]: rows k1 = 2 (4 g + e) + tk, column n2 = 2 j + tn; tile tk
     // carries the factor s0 s2[tk]
     mf_f4 ur[2][2], ui[2][2];
-    float inv2[2];
-    float m3 = 0.0f;
+    float s2[2], inv2[2];
 #pragma unroll
     for (int tk = 0; tk < 2; ++tk) {
         // ---- stage 2 (table x data) for column k1 = 2 j + tk ----
@@ -352,13 +410,13 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
             }
         }
         if (tk == 0) {
-            // every multiplier has arrived: the next tile's fetch now
+            // every multiplier has arrived before the first slice is issued
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) asm volatile("" ::"v"(gm[1][h][e]));
-            if (t + G < ntot) mr_fetch(a, mr_tile(a, t + G, ntot), nxt, w, l);
         }
+        mr_slice(a, nxt, Tp, hasprev, Tn, more, w, l, 2 * tk);
         // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
         float2 za[4], zb[4];
         float mz = 0.0f;
@@ -368,15 +426,14 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
             zb[e] = float2{z0[e].x - z1[e].x, z0[e].y - z1[e].y};
             mz = mr_absmax(zb[e], mr_absmax(za[e], mz));
         }
-        float s2;
-        mr_scale(mr_wave_max(mz), s2, inv2[tk]);
+        mr_scale(mr_wave_max(mz), s2[tk], inv2[tk]);
         {
-            const MfFrag XA = mf_split4(za, s2, -s2, a.fzero), XB = mf_split4(zb, s2, -s2, a.fzero);
+            const MfFrag XA = mf_split4(za, s2[tk], -s2[tk], a.fzero), XB = mf_split4(zb, s2[tk], -s2[tk], a.fzero);
             mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
+        mr_slice(a, nxt, Tp, hasprev, Tn, more, w, l, 2 * tk + 1);
         // T = W1024^(n2 k1) V (V = conj of the inverse transform's partial)
-        float mt = 0.0f;
 #pragma unroll
         for (int tn = 0; tn < 2; ++tn)
 #pragma unroll
@@ -384,14 +441,16 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
                 const float2 v = cmul(float2{ur[tk][tn][e], ui[tk][tn][e]}, twl[tn * TWP + 2 * e + tk]);
                 ur[tk][tn][e] = v.x;
                 ui[tk][tn][e] = v.y;
-                mt = mr_absmax(v, mt);
             }
-        m3 = __builtin_fmaxf(m3, mt * inv2[tk]);
     }
-    float s3, inv3;
-    mr_scale(mr_wave_max(m3), s3, inv3);
-    const float sa = s3 * inv2[0], sb2 = s3 * inv2[1];   // both k1 tiles at the common scale s0 s3
-    const float fo = inv0 * inv3;
+    // Inverse stage 2 input scale from the bound, not a maximum: the split
+    // inverse stage-1 inputs are below 2^10, a 16-point DFT and the unit
+    // twiddles keep |T| below 16 sqrt 2 2^10 < 2^15, so tile tk, which carries
+    // s0 s2[tk], is split with c[tk] = smin / (2 s2[tk]) (smin the smaller
+    // s2): both tiles then carry s0 smin / 2 and stay below 2^14.5
+    const float smin = __builtin_fminf(s2[0], s2[1]);
+    const float sa = 0.5f * smin * inv2[0], sb2 = 0.5f * smin * inv2[1];
+    const float fo = 2.0f * inv0 * __builtin_fmaxf(inv2[0], inv2[1]);
     // ---- inverse stage 2 (table x data): x[32 n1 + n2] = conj(sum_k1 W32^(n1 k1) T[k1][n2]), n1 < 16 ----
 #pragma unroll
     for (int tn = 0; tn < 2; ++tn) {
@@ -428,7 +487,7 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
 
     const int tid = threadIdx.x;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l0 = tid & 63;
-    const int ntot = a.ncoltile * a.nslab * a.ncomp;
+    const int nunit = mr_nunit(a);
     const int G = gridDim.x;
 
     // twiddle matrix W1024^(r s), r, s < 32 (1024 threads, one entry each)
@@ -442,42 +501,52 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
         const int fo = (f == 0) ? 0 : (f == 1) ? 2 : (f == 2) ? 1 : 3;
         ftw[(tb * 4 + fo) * 64 + ln] = reinterpret_cast<const mf_h8*>(a.mfrag16)[(tb * 64 + ln) * 4 + f];
     }
-    int t = blockIdx.x;
-    if (t < ntot) mr_fetch(a, mr_tile(a, t, ntot), stg0, w, l0);
+    // iterations: tiles (u, 0), (u, 1) of units u = g, g + G, ...
+    int u = blockIdx.x, m = 0;
+    MrTile T;
+    const bool any = u < nunit && mr_tile(a, u, 0, nunit, T);
+    if (any) mr_fetch(a, T, stg0, w, l0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    float gm[2][2][4];   // multipliers of the tile (kept for the second tile of a pair)
+    int kyo_g = -1;      // octant row they belong to
 
-    for (int it = 0; t < ntot; t += G, ++it) {
+    MrTile Tp = T;
+    int it = 0;
+    for (; any && u < nunit; ++it) {
         // lane indices recomputed per tile (an opaque copy): lane-dependent
         // offsets are not held in registers across the loop
         int l = l0;
         asm volatile("" : "+v"(l));
         unsigned char* cur = stg0 + (it & 1) * M::STG;
         unsigned char* nxt = stg0 + ((it & 1) ^ 1) * M::STG;
-        const MrTile T = mr_tile(a, t, ntot);
-        mr_step<OPK>(a, cur, nxt, twm, ftw, t, G, ntot, w, l);
-        __syncthreads();
-        // this wavefront's fetch of the next tile (and its earlier stores) done
-        __builtin_amdgcn_s_waitcnt(0);
-        // ---- store of the tile: 16 B per lane, full 128-byte rows ----
-        {
-            char* ob = reinterpret_cast<char*>(a.out) +
-                       ((long)T.comp * a.oa.comp + (long)T.col0 * a.oa.col + boff(T.slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi)) * 8;
-            int tq = tid;
-            asm volatile("" : "+v"(tq));
-            const int c = tq & 7;
-            const int nv = a.ncol - T.col0 - 2 * c;   // valid columns of this lane's pair
-            float4 v[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) v[k] = *reinterpret_cast<const float4*>(cur + mr_stg_off((tq >> 3) + 128 * k, c));
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                char* o = ob + (mr_zoff(a.oa, (tq >> 3) + 128 * k) + 2u * c) * 8u;
-                if (nv >= 2) *reinterpret_cast<float4*>(o) = v[k];
-                else if (nv == 1) *reinterpret_cast<float2*>(o) = float2{v[k].x, v[k].y};
-            }
+        // the next tile: the pair's second, else the next unit's first
+        MrTile Tn;
+        int un = u, mn = 1;
+        if (m == 1 || !mr_tile(a, u, 1, nunit, Tn)) {
+            un = u + G;
+            mn = 0;
         }
-        __syncthreads();   // every wavefront has read cur: the fetch after next may overwrite it
+        const bool more = un < nunit && (mn == 1 || mr_tile(a, un, 0, nunit, Tn));
+        // the second tile of a slab pair keeps the first one's multipliers
+        const bool reuse_g = m == 1 && T.kyo == kyo_g && mr_even_in_ky<OPK>(a, T.comp);
+        mr_step<OPK>(a, cur, nxt, twm, ftw, T, Tp, it > 0, Tn, more, reuse_g, gm, w, l);
+        kyo_g = T.kyo;
+        // this wavefront's fetch of the next tile and the previous tile's
+        // stores are done; every wavefront's output is in cur
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        Tp = T;
+        T = Tn;
+        u = un;
+        m = mn;
+        if (!more) break;
+    }
+    // the last tile's output (in the buffer of the last iteration)
+    if (any) {
+        unsigned char* last = stg0 + (it & 1) * M::STG;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mr_slice(a, last, Tp, true, Tp, false, w, l0, k);
     }
 }
 
