@@ -185,8 +185,12 @@ __device__ __forceinline__ int mr_nunit(const PencilArgs<float>& a)
     return a.ncomp * ((a.nslab + 1) >> 1) * a.ncoltile;
 }
 
-// tile m (0, 1) of unit u; false if the unit has no such tile
-__device__ __forceinline__ bool mr_tile(const PencilArgs<float>& a, int u, int m, int nunit, MrTile& z)
+// unit u's component, slab pair and column tile
+struct MrUnit {
+    int comp, sp, col0;
+};
+
+__device__ __forceinline__ MrUnit mr_unit(const PencilArgs<float>& a, int u, int nunit)
 {
     int lin = u;
     if (a.xcd_remap) {
@@ -196,13 +200,21 @@ __device__ __forceinline__ bool mr_tile(const PencilArgs<float>& a, int u, int m
         if (lin < (per << 3)) lin = (lin & 7) * per + (lin >> 3);
     }
     const int nunit1 = ((a.nslab + 1) >> 1) * a.ncoltile;
-    z.comp = lin / nunit1;
-    const int rem = lin - z.comp * nunit1;
-    const int sp = rem / a.ncoltile;
-    const int ct = rem - sp * a.ncoltile;
-    const int slab0 = 2 * sp + m;
+    MrUnit v;
+    v.comp = lin / nunit1;
+    const int rem = lin - v.comp * nunit1;
+    v.sp = rem / a.ncoltile;
+    v.col0 = (rem - v.sp * a.ncoltile) * 16;
+    return v;
+}
+
+// tile m (0, 1) of a unit; false if the unit has no such tile (odd slab count)
+__device__ __forceinline__ bool mr_tile(const PencilArgs<float>& a, const MrUnit& v, int m, MrTile& z)
+{
+    const int slab0 = 2 * v.sp + m;
+    z.comp = v.comp;
+    z.col0 = v.col0;
     z.slab = a.slab_pair ? paired_slab(slab0, a.nslab) : slab0;
-    z.col0 = ct * 16;
     z.kyo = (z.slab <= a.nslab - z.slab) ? z.slab : a.nslab - z.slab;
     return slab0 < a.nslab;
 }
@@ -241,22 +253,6 @@ __device__ __forceinline__ unsigned mr_zoff_l(const Addr& d, int zr)
                                (unsigned)(zr & ((1 << d.pos_sh) - 1)) * (unsigned)d.pos_lo) * 8u;
 }
 
-// Wavefront w fetches blocks n = w + 16 k (k < 4: rows 8n .. 8n + 7) of tile T
-// into buf: lane l loads 16 bytes of row 8n + (l >> 3) into slot l of the
-// block, i.e. chunk (l & 7) ^ r ^ parity of that row (the staging permutation).
-__device__ __forceinline__ void mr_fetch(const PencilArgs<float>& a, const MrTile& T, unsigned char* buf, int w, int l)
-{
-    const char* ib = mr_tile_base(a.in, a.ia, T);
-    const int r = l >> 3;
-    const int c = (l & 7) ^ r ^ (w & 1);   // block parity n & 1 = w & 1
-    const unsigned lo = mr_zoff_l(a.ia, r) + 16u * (unsigned)c;
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const int n = w + 16 * k;
-        __builtin_amdgcn_global_load_lds(ib + mr_zoff_u(a.ia, 8 * n) + lo, (mr_lds_ptr)(buf + n * 1024), 16, 0, 0);
-    }
-}
-
 // k-space multipliers of lane (j, g)'s 16 spectrum points kz = k1 + 32 k2,
 // k1 = 2 j + tt, k2 = 4 g + e + 16 h, for pencil column col of tile T.  The
 // octant is [ky][kx][kz] (g_kz = 1): kz < 512 for h = 0, and the octant row of
@@ -287,27 +283,51 @@ __device__ __forceinline__ void mr_kfactors(float (&gm)[2][2][4], const PencilAr
             }
 }
 
-// Slice k of wavefront w's share of the other buffer (block n = w + 16 k,
-// rows 8n .. 8n + 7, read and then overwritten by this wavefront only):
-// store the previous tile's output rows of the block, then fetch the next
-// tile's rows into it.  Four slices spread over a tile's transform keep the
-// stores and the fetch from arriving at the memory system as one burst.
-__device__ __forceinline__ void mr_slice(const PencilArgs<float>& a, unsigned char* __restrict__ buf, const MrTile& Tp,
-                                         bool hasprev, const MrTile& Tn, bool more, int w, int l, int k)
+// Per-lane and per-wavefront offsets of the slices, fixed for the launch:
+// slice k of wavefront w is block n = w + 16 k of a staging buffer (rows 8n ..
+// 8n + 7).  Lane l (row r = l >> 3, c = l & 7) reads chunk c of row r for the
+// store (its LDS slot is c ^ r ^ parity) and fetches the chunk that belongs in
+// slot c (chunk c ^ r ^ parity).
+struct MrSliceOffs {
+    unsigned lds;       // LDS offset of lane l's chunk in block 0 of its wavefront's slices (+ n 1024)
+    unsigned ilane;     // fetch: byte offset of the chunk the lane loads (the staging permutation)
+    unsigned olane;     // store: byte offset of the chunk the lane stores
+    unsigned iu[4], ou[4];   // uniform byte offsets of rows 8n, fetch and store addressing
+};
+
+__device__ __forceinline__ MrSliceOffs mr_slice_offs(const PencilArgs<float>& a, int w, int l)
 {
-    const int n = w + 16 * k, r = l >> 3, c = l & 7;
+    MrSliceOffs o;
+    const int r = l >> 3, c = l & 7, par = w & 1;   // block parity n & 1 = w & 1
+    o.lds = (unsigned)(r * 128 + ((c ^ r ^ par) << 4));
+    o.ilane = mr_zoff_l(a.ia, r) + 16u * (unsigned)(c ^ r ^ par);
+    o.olane = mr_zoff_l(a.oa, r) + 16u * (unsigned)c;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        o.iu[k] = mr_zoff_u(a.ia, 8 * (w + 16 * k));
+        o.ou[k] = mr_zoff_u(a.oa, 8 * (w + 16 * k));
+    }
+    return o;
+}
+
+// Slice k: store the previous tile's output rows of the wavefront's block
+// (to obase: that tile's column 0; nv valid columns from the lane's chunk),
+// then fetch the next tile's rows (from ibase) into the same block; the
+// wavefront alone reads and overwrites its blocks.  Four slices spread over a
+// tile's transform keep the stores and the fetch from reaching the memory
+// system as one burst.  Each lane handles the chunk whose LDS slot it reads,
+// so the fetch and the store use the same (permuted) chunk index.
+__device__ __forceinline__ void mr_slice(unsigned char* __restrict__ buf, const MrSliceOffs& so, char* obase, int nv,
+                                         bool hasprev, const char* ibase, bool more, int w, int k)
+{
+    const int n = w + 16 * k;
     if (hasprev) {
-        const float4 v = *reinterpret_cast<const float4*>(buf + mr_stg_off(8 * n + r, c));
-        const int nv = a.ncol - Tp.col0 - 2 * c;   // valid columns of this lane's pair
-        char* o = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp)) + mr_zoff_u(a.oa, 8 * n) + mr_zoff_l(a.oa, r) + 16u * c;
+        const float4 v = *reinterpret_cast<const float4*>(buf + n * 1024 + so.lds);
+        char* o = obase + so.ou[k] + so.olane;
         if (nv >= 2) *reinterpret_cast<float4*>(o) = v;
         else if (nv == 1) *reinterpret_cast<float2*>(o) = float2{v.x, v.y};
     }
-    if (more) {
-        const int cf = c ^ r ^ (w & 1);   // the staging permutation of row r (block parity w & 1)
-        const char* gp = mr_tile_base(a.in, a.ia, Tn) + mr_zoff_u(a.ia, 8 * n) + mr_zoff_l(a.ia, r) + 16u * cf;
-        __builtin_amdgcn_global_load_lds(gp, (mr_lds_ptr)(buf + n * 1024), 16, 0, 0);
-    }
+    if (more) __builtin_amdgcn_global_load_lds(ibase + so.iu[k] + so.ilane, (mr_lds_ptr)(buf + n * 1024), 16, 0, 0);
 }
 
 // One tile for wavefront w: the transform of column w of cur with the
@@ -320,8 +340,9 @@ __device__ __forceinline__ void mr_slice(const PencilArgs<float>& a, unsigned ch
 template <int OPK>
 __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
                                         unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
-                                        const mf_h8* __restrict__ ft, const MrTile& T, const MrTile& Tp, bool hasprev,
-                                        const MrTile& Tn, bool more, bool reuse_g, float (&gm)[2][2][4], int w, int l)
+                                        const mf_h8* __restrict__ ft, const MrTile& T, const MrSliceOffs& so,
+                                        char* obase, int nv, bool hasprev, const char* ibase, bool more, bool reuse_g,
+                                        float (&gm)[2][2][4], int w, int l)
 {
     constexpr int TWP = MfmaZR::TWP;
     constexpr bool mimag = mf_imag_multiplier<OPK>();
@@ -416,7 +437,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
 #pragma unroll
                 for (int e = 0; e < 4; ++e) asm volatile("" ::"v"(gm[1][h][e]));
         }
-        mr_slice(a, nxt, Tp, hasprev, Tn, more, w, l, 2 * tk);
+        mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 2 * tk);
         // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
         float2 za[4], zb[4];
         float mz = 0.0f;
@@ -432,7 +453,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
             mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
-        mr_slice(a, nxt, Tp, hasprev, Tn, more, w, l, 2 * tk + 1);
+        mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 2 * tk + 1);
         // T = W1024^(n2 k1) V (V = conj of the inverse transform's partial)
 #pragma unroll
         for (int tn = 0; tn < 2; ++tn)
@@ -502,10 +523,17 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
         ftw[(tb * 4 + fo) * 64 + ln] = reinterpret_cast<const mf_h8*>(a.mfrag16)[(tb * 64 + ln) * 4 + f];
     }
     // iterations: tiles (u, 0), (u, 1) of units u = g, g + G, ...
+    const MrSliceOffs so = mr_slice_offs(a, w, l0);
     int u = blockIdx.x, m = 0;
-    MrTile T;
-    const bool any = u < nunit && mr_tile(a, u, 0, nunit, T);
-    if (any) mr_fetch(a, T, stg0, w, l0);
+    MrUnit U{};
+    MrTile T{};
+    const bool any = u < nunit && (U = mr_unit(a, u, nunit), mr_tile(a, U, 0, T));
+    if (any) {
+        const char* ib = mr_tile_base(a.in, a.ia, T);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            __builtin_amdgcn_global_load_lds(ib + so.iu[k] + so.ilane, (mr_lds_ptr)(stg0 + (w + 16 * k) * 1024), 16, 0, 0);
+    }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     float gm[2][2][4];   // multipliers of the tile (kept for the second tile of a pair)
@@ -513,7 +541,7 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
 
     MrTile Tp = T;
     int it = 0;
-    for (; any && u < nunit; ++it) {
+    for (; any; ++it) {
         // lane indices recomputed per tile (an opaque copy): lane-dependent
         // offsets are not held in registers across the loop
         int l = l0;
@@ -521,16 +549,20 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
         unsigned char* cur = stg0 + (it & 1) * M::STG;
         unsigned char* nxt = stg0 + ((it & 1) ^ 1) * M::STG;
         // the next tile: the pair's second, else the next unit's first
-        MrTile Tn;
+        MrTile Tn{};
+        MrUnit Un = U;
         int un = u, mn = 1;
-        if (m == 1 || !mr_tile(a, u, 1, nunit, Tn)) {
+        if (m == 1 || !mr_tile(a, U, 1, Tn)) {
             un = u + G;
             mn = 0;
         }
-        const bool more = un < nunit && (mn == 1 || mr_tile(a, un, 0, nunit, Tn));
+        const bool more = un < nunit && (mn == 1 || (Un = mr_unit(a, un, nunit), mr_tile(a, Un, 0, Tn)));
         // the second tile of a slab pair keeps the first one's multipliers
         const bool reuse_g = m == 1 && T.kyo == kyo_g && mr_even_in_ky<OPK>(a, T.comp);
-        mr_step<OPK>(a, cur, nxt, twm, ftw, T, Tp, it > 0, Tn, more, reuse_g, gm, w, l);
+        char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp));
+        const char* ibase = mr_tile_base(a.in, a.ia, Tn);
+        const int nv = a.ncol - Tp.col0 - 2 * (l & 7);   // valid columns of this lane's chunk of Tp
+        mr_step<OPK>(a, cur, nxt, twm, ftw, T, so, obase, nv, it > 0, ibase, more, reuse_g, gm, w, l);
         kyo_g = T.kyo;
         // this wavefront's fetch of the next tile and the previous tile's
         // stores are done; every wavefront's output is in cur
@@ -538,6 +570,7 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
         __syncthreads();
         Tp = T;
         T = Tn;
+        U = Un;
         u = un;
         m = mn;
         if (!more) break;
@@ -545,8 +578,10 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
     // the last tile's output (in the buffer of the last iteration)
     if (any) {
         unsigned char* last = stg0 + (it & 1) * M::STG;
+        char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp));
+        const int nv = a.ncol - Tp.col0 - 2 * (l0 & 7);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) mr_slice(a, last, Tp, true, Tp, false, w, l0, k);
+        for (int k = 0; k < 4; ++k) mr_slice(last, so, obase, nv, true, nullptr, false, w, k);
     }
 }
 
