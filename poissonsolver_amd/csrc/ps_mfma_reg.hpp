@@ -92,16 +92,88 @@ __device__ __forceinline__ void mr_scale(float m, float& s, float& inv)
     inv = __uint_as_float((unsigned)(127 - 10 + e) << 23);
 }
 
-// largest |re|, |im| of loaded values, on the bit patterns (no NaN
-// canonicalisation of memory operands)
-__device__ __forceinline__ unsigned mr_absmax_bits(float2 v, unsigned m)
+// largest |re|, |im| of eight complex values (max3 with absolute-value
+// modifiers: one instruction per value, no NaN canonicalisation of memory
+// operands; a NaN input is skipped, and propagates through the transform
+// regardless).  The trailing nop gives the result the two wait states a DPP
+// read of a VGPR written by a VALU instruction needs.
+__device__ __forceinline__ float mr_absmax8(const float2 (&u)[4], const float2 (&v)[4])
 {
-    return max(m, max(__float_as_uint(v.x) & 0x7fffffffu, __float_as_uint(v.y) & 0x7fffffffu));
+    float m;
+    asm("v_max3_f32 %0, |%1|, |%2|, 0\n\t"
+        "v_max3_f32 %0, |%3|, |%4|, %0\n\t"
+        "v_max3_f32 %0, |%5|, |%6|, %0\n\t"
+        "v_max3_f32 %0, |%7|, |%8|, %0\n\t"
+        "v_max3_f32 %0, |%9|, |%10|, %0\n\t"
+        "v_max3_f32 %0, |%11|, |%12|, %0\n\t"
+        "v_max3_f32 %0, |%13|, |%14|, %0\n\t"
+        "v_max3_f32 %0, |%15|, |%16|, %0\n\t"
+        "s_nop 1"
+        : "=&v"(m)
+        : "v"(u[0].x), "v"(u[0].y), "v"(u[1].x), "v"(u[1].y), "v"(u[2].x), "v"(u[2].y), "v"(u[3].x), "v"(u[3].y),
+          "v"(v[0].x), "v"(v[0].y), "v"(v[1].x), "v"(v[1].y), "v"(v[2].x), "v"(v[2].y), "v"(v[3].x), "v"(v[3].y));
+    return m;
 }
 
-__device__ __forceinline__ float mr_absmax(float2 v, float m)
+// fp16 hi/lo split of four complex values scaled by s (the imaginary parts
+// by -s for NEGY): hi = f16(x s) rounded to nearest, lo = f16(x s - hi), four
+// mixed-precision FMAs per complex value (v_fma_mixlo/mixhi write one half
+// of a register; op_sel picks the half of hi that the remainder subtracts).
+// The nop gives the two wait states an MFMA read of a VGPR written by a VALU
+// instruction needs, which the compiler does not count for inline assembly.
+template <bool NEGY>
+__device__ __forceinline__ MfFrag mr_split4s(const float2 (&v)[4], float s)
 {
-    return __builtin_fmaxf(m, __builtin_fmaxf(__builtin_fabsf(v.x), __builtin_fabsf(v.y)));
+    unsigned H[4], Lo[4];
+    if constexpr (NEGY) {
+        asm("v_fma_mixlo_f16 %0, %8, %16, 0\n\t"
+            "v_fma_mixhi_f16 %0, %9, -%16, 0\n\t"
+            "v_fma_mixlo_f16 %4, %8, %16, -%0 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %4, %9, -%16, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %1, %10, %16, 0\n\t"
+            "v_fma_mixhi_f16 %1, %11, -%16, 0\n\t"
+            "v_fma_mixlo_f16 %5, %10, %16, -%1 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %5, %11, -%16, -%1 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %2, %12, %16, 0\n\t"
+            "v_fma_mixhi_f16 %2, %13, -%16, 0\n\t"
+            "v_fma_mixlo_f16 %6, %12, %16, -%2 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %6, %13, -%16, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %3, %14, %16, 0\n\t"
+            "v_fma_mixhi_f16 %3, %15, -%16, 0\n\t"
+            "v_fma_mixlo_f16 %7, %14, %16, -%3 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %7, %15, -%16, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "s_nop 1"
+            : "=&v"(H[0]), "=&v"(H[1]), "=&v"(H[2]), "=&v"(H[3]), "=&v"(Lo[0]), "=&v"(Lo[1]), "=&v"(Lo[2]),
+              "=&v"(Lo[3])
+            : "v"(v[0].x), "v"(v[0].y), "v"(v[1].x), "v"(v[1].y), "v"(v[2].x), "v"(v[2].y), "v"(v[3].x),
+              "v"(v[3].y), "v"(s));
+    } else {
+        asm("v_fma_mixlo_f16 %0, %8, %16, 0\n\t"
+            "v_fma_mixhi_f16 %0, %9, %16, 0\n\t"
+            "v_fma_mixlo_f16 %4, %8, %16, -%0 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %4, %9, %16, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %1, %10, %16, 0\n\t"
+            "v_fma_mixhi_f16 %1, %11, %16, 0\n\t"
+            "v_fma_mixlo_f16 %5, %10, %16, -%1 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %5, %11, %16, -%1 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %2, %12, %16, 0\n\t"
+            "v_fma_mixhi_f16 %2, %13, %16, 0\n\t"
+            "v_fma_mixlo_f16 %6, %12, %16, -%2 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %6, %13, %16, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixlo_f16 %3, %14, %16, 0\n\t"
+            "v_fma_mixhi_f16 %3, %15, %16, 0\n\t"
+            "v_fma_mixlo_f16 %7, %14, %16, -%3 op_sel_hi:[0,0,1]\n\t"
+            "v_fma_mixhi_f16 %7, %15, %16, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+            "s_nop 1"
+            : "=&v"(H[0]), "=&v"(H[1]), "=&v"(H[2]), "=&v"(H[3]), "=&v"(Lo[0]), "=&v"(Lo[1]), "=&v"(Lo[2]),
+              "=&v"(Lo[3])
+            : "v"(v[0].x), "v"(v[0].y), "v"(v[1].x), "v"(v[1].y), "v"(v[2].x), "v"(v[2].y), "v"(v[3].x),
+              "v"(v[3].y), "v"(s));
+    }
+    MfFrag f;
+    f.hi = __builtin_bit_cast(mf_h8, H);
+    f.lo = __builtin_bit_cast(mf_h8, Lo);
+    return f;
 }
 
 // fp16 hi/lo split of four complex values already below 65504: hi rounded
@@ -219,16 +291,6 @@ __device__ __forceinline__ bool mr_tile(const PencilArgs<float>& a, const MrUnit
     return slab0 < a.nslab;
 }
 
-// whether the multipliers of component comp are even in ky, so that slabs ky
-// and My - ky (one octant row) share them: every operator except a derivative
-// along y (grad of component y)
-template <int OPK>
-__device__ __forceinline__ bool mr_even_in_ky(const PencilArgs<float>& a, int comp)
-{
-    if constexpr (OPK == OP_GRAD) return kcomp<float>(a.kmap, comp, 0.0f, 1.0f, 0.0f) == 0.0f;
-    return true;
-}
-
 // byte offset of tile T's column 0 in a component array (addressing d)
 __device__ __forceinline__ const char* mr_tile_base(const void* base, const Addr& d, const MrTile& T)
 {
@@ -253,34 +315,55 @@ __device__ __forceinline__ unsigned mr_zoff_l(const Addr& d, int zr)
                                (unsigned)(zr & ((1 << d.pos_sh) - 1)) * (unsigned)d.pos_lo) * 8u;
 }
 
-// k-space multipliers of lane (j, g)'s 16 spectrum points kz = k1 + 32 k2,
-// k1 = 2 j + tt, k2 = 4 g + e + 16 h, for pencil column col of tile T.  The
-// octant is [ky][kx][kz] (g_kz = 1): kz < 512 for h = 0, and the octant row of
-// h = 1 is Mz - kz = 512 - (kz of h = 0), so all loads are base + immediate.
+// Multiplier inputs of lane (j, g)'s 16 spectrum points kz = k1 + 32 k2,
+// k1 = 2 j + tt, k2 = 4 g + e + 16 h, for pencil column col of tile T, kept as
+// loaded: gr[h][e] holds the pair tt = 0, 1 (h = 0) or tt = 1, 0 (h = 1).  The
+// octant is [ky][kx][kz] (g_kz = 1): kz < 512 for h = 0, and the octant index
+// of h = 1 is Mz - kz = 512 - (kz of h = 0), so each pair is one 8-byte load
+// at base + immediate.  The operators that read the octant keep the raw Gk
+// values (the same for slabs ky and My - ky, one octant row) and apply their
+// k-dependent factor where the multiplier is used; the others compute theirs
+// here.
+typedef float mr_f2a __attribute__((ext_vector_type(2), aligned(4)));
+
+template <int OPK> constexpr bool mr_loads_g() { return OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_LAPL; }
+
 template <int OPK>
-__device__ __forceinline__ void mr_kfactors(float (&gm)[2][2][4], const PencilArgs<float>& a, const MrTile& T, int col,
-                                            int j, int g)
+__device__ __forceinline__ void mr_kload(mr_f2a (&gr)[2][4], const PencilArgs<float>& a, const MrTile& T, int col, int j,
+                                         int g)
 {
     const TileK<float> tk = tile_k(a, col, T.slab, col < a.ncol);
-    const MfKCoef kc = mf_kcoef<OPK>(a, tk, T.comp);
     const int kb = 2 * j + 128 * g;
-    const char* gr = reinterpret_cast<const char*>(kc.grow);
-    const unsigned gb0 = (unsigned)kb * 4u, gb1 = (unsigned)(512 - kb) * 4u;
+    if constexpr (mr_loads_g<OPK>()) {
+        const char* row = reinterpret_cast<const char*>(tk.grow ? tk.grow : a.G);
+        const char* r0 = row + kb * 4;
+        const char* r1 = row + (512 - kb) * 4 - 4 - 3 * 128;
 #pragma unroll
-    for (int tt = 0; tt < 2; ++tt)
+        for (int e = 0; e < 4; ++e) {
+            gr[0][e] = *reinterpret_cast<const mr_f2a*>(r0 + 128 * e);
+            gr[1][e] = *reinterpret_cast<const mr_f2a*>(r1 + 128 * (3 - e));
+        }
+    } else {
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int kz0 = kb + tt + 32 * e;
-                const int kz = h ? kz0 + 512 : kz0;
-                if constexpr (OPK == OP_PLAIN || OPK == OP_GRAD || OPK == OP_LAPL) {
-                    const unsigned go = h ? gb1 - (unsigned)(tt + 32 * e) * 4u : gb0 + (unsigned)(tt + 32 * e) * 4u;
-                    gm[tt][h][e] = mf_kval<OPK>(a, kc, kz, *reinterpret_cast<const float*>(gr + go));
-                } else {
-                    gm[tt][h][e] = (OPK == OP_NONE) ? 1.0f : mf_rhsf(a, tk, kz);
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int tt = 0; tt < 2; ++tt) {
+                    const int kz = kb + tt + 32 * e + 512 * h;
+                    gr[h][e][h ? 1 - tt : tt] = (OPK == OP_NONE) ? 1.0f : mf_rhsf(a, tk, kz);
                 }
-            }
+    }
+}
+
+// multiplier of point (tt, h, e) from gr (kc: the tile's coefficients)
+template <int OPK>
+__device__ __forceinline__ float mr_kmul(const PencilArgs<float>& a, const MfKCoef& kc, const mr_f2a (&gr)[2][4], int tt,
+                                         int h, int e, int kb)
+{
+    const float v = gr[h][e][h ? 1 - tt : tt];
+    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL) return mf_kval<OPK>(a, kc, kb + tt + 32 * e + 512 * h, v);
+    return v;
 }
 
 // Per-lane and per-wavefront offsets of the slices, fixed for the launch:
@@ -330,28 +413,29 @@ __device__ __forceinline__ void mr_slice(unsigned char* __restrict__ buf, const 
     if (more) __builtin_amdgcn_global_load_lds(ibase + so.iu[k] + so.ilane, (mr_lds_ptr)(buf + n * 1024), 16, 0, 0);
 }
 
-// One tile for wavefront w: the transform of column w of cur with the
-// multipliers gm (already in registers), written back in place, and spread
-// over it the four slices that store the previous tile from nxt and fetch the
-// next one into it.  Nothing here waits for vector memory.  cur, nxt, the
-// twiddle matrix and the tables are disjoint LDS regions; as __restrict__
-// parameters of one function the compiler can tell that this tile's LDS
-// reads of cur do not depend on the fetch into nxt.
+// One tile T for wavefront w: the transform of column w of cur with the
+// multiplier inputs gm (already in registers), written back in place, and
+// spread over its first half the four slices that store the previous tile
+// from nxt and fetch the next one into it; after the last use of gm, the next
+// tile Tn's multiplier inputs are loaded into it (load_g; the second tile of
+// a slab pair keeps the first's).  Nothing here waits for vector memory: the loads are
+// waited for at the end of the tile, so the multiplier loads never wait for a
+// store (loads and stores share one counter).  cur, nxt, the twiddle matrix
+// and the tables are disjoint LDS regions; as __restrict__ parameters of one
+// function the compiler can tell that this tile's LDS reads of cur do not
+// depend on the fetch into nxt.
 template <int OPK>
 __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
                                         unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
                                         const mf_h8* __restrict__ ft, const MrTile& T, const MrSliceOffs& so,
-                                        char* obase, int nv, bool hasprev, const char* ibase, bool more, bool reuse_g,
-                                        float (&gm)[2][2][4], int w, int l)
+                                        char* obase, int nv, bool hasprev, const char* ibase, bool more,
+                                        const MrTile& Tn, bool load_g, mr_f2a (&gm)[2][4], int w, int l)
 {
     constexpr int TWP = MfmaZR::TWP;
     constexpr bool mimag = mf_imag_multiplier<OPK>();
     const int g = l >> 4, j = l & 15;
-    const int p = w;
-    // this tile's multipliers (the second tile of a slab pair keeps the
-    // first's); they are used before the first slice, so that their wait
-    // cannot wait for a store (loads and stores share one counter)
-    if (!reuse_g) mr_kfactors<OPK>(gm, a, T, T.col0 + p, j, g);
+    const int p = w, kb = 2 * j + 128 * g;
+    mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 0);
     // staging offsets of rows z = 2 j + q + 32 (4 g + e) (q = 0, 1; 128 g + 32 e
     // is a whole number of blocks): z & 7 = 2 (j & 3) + q, block parity (j >> 2) & 1
     unsigned sb[2];
@@ -362,16 +446,12 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
     }
     // ---- input x[b + 32 m], b = 2 j + tb, m = 4 g + e (m >= 16 is padding) ----
     float2 x[2][4];
-    unsigned mx = 0u;
 #pragma unroll
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            x[tb][e] = *reinterpret_cast<const float2*>(cur + sb[tb] + e * 4096);
-            mx = mr_absmax_bits(x[tb][e], mx);
-        }
+        for (int e = 0; e < 4; ++e) x[tb][e] = *reinterpret_cast<const float2*>(cur + sb[tb] + e * 4096);
     float s0, inv0;
-    mr_scale(mr_wave_max(__uint_as_float(mx)), s0, inv0);
+    mr_scale(mr_wave_max(mr_absmax8(x[0], x[1])), s0, inv0);
 
     // ---- stage 1 (data x table): Y[b][k1] = sum_m W32^(m k1) x[b + 32 m] ----
     // yr/yi[tb][tk]: rows b = 2 (4 g + e) + tb, column k1 = 2 j + tk
@@ -380,11 +460,12 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) {
-            const MfFrag X = mf_split4(x[tb], s0, s0, a.fzero);
+            const MfFrag X = mr_split4s<false>(x[tb], s0);
             mr_mfma_a<true>(yr[tb][0], yi[tb][0], X, Fe);
             mr_mfma_a<true>(yr[tb][1], yi[tb][1], X, Fo);
         }
     }
+    mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 1);
     // twiddle W1024^(b k1): matrix row k1 = 2 j + tk, column b = 8 g + 2 e + tb
     const float2* twl = twm + 2 * j * TWP + 8 * g;
 #pragma unroll
@@ -401,6 +482,9 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
     // carries the factor s0 s2[tk]
     mf_f4 ur[2][2], ui[2][2];
     float s2[2], inv2[2];
+    MfKCoef kc{};
+    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL)
+        kc = mf_kcoef<OPK>(a, tile_k(a, T.col0 + p, T.slab, T.col0 + p < a.ncol), T.comp);
 #pragma unroll
     for (int tk = 0; tk < 2; ++tk) {
         // ---- stage 2 (table x data) for column k1 = 2 j + tk ----
@@ -418,7 +502,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         float2 z0[4], z1[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const float m0 = gm[tk][0][e], m1 = gm[tk][1][e];
+            const float m0 = mr_kmul<OPK>(a, kc, gm, tk, 0, e, kb), m1 = mr_kmul<OPK>(a, kc, gm, tk, 1, e, kb);
             const float xr0 = er[e] + orr[e], xi0 = ei[e] + oi[e];
             const float xr1 = er[e] - orr[e], xi1 = ei[e] - oi[e];
             if constexpr (mimag) {
@@ -430,30 +514,23 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
                 z1[e] = float2{xr1 * m1, xi1 * m1};
             }
         }
-        if (tk == 0) {
-            // every multiplier has arrived before the first slice is issued
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) asm volatile("" ::"v"(gm[1][h][e]));
-        }
-        mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 2 * tk);
+        if (tk == 0) mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 2);
+        // the next tile's multipliers, once this tile's are used
+        if (tk == 1 && load_g) mr_kload<OPK>(gm, a, Tn, Tn.col0 + p, j, g);
         // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
         float2 za[4], zb[4];
-        float mz = 0.0f;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             za[e] = float2{z0[e].x + z1[e].x, z0[e].y + z1[e].y};
             zb[e] = float2{z0[e].x - z1[e].x, z0[e].y - z1[e].y};
-            mz = mr_absmax(zb[e], mr_absmax(za[e], mz));
         }
-        mr_scale(mr_wave_max(mz), s2[tk], inv2[tk]);
+        mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
-            const MfFrag XA = mf_split4(za, s2[tk], -s2[tk], a.fzero), XB = mf_split4(zb, s2[tk], -s2[tk], a.fzero);
+            const MfFrag XA = mr_split4s<true>(za, s2[tk]), XB = mr_split4s<true>(zb, s2[tk]);
             mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
-        mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 2 * tk + 1);
+        if (tk == 0) mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 3);
         // T = W1024^(n2 k1) V (V = conj of the inverse transform's partial)
 #pragma unroll
         for (int tn = 0; tn < 2; ++tn)
@@ -481,7 +558,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
             v0[e] = float2{ur[0][tn][e], ui[0][tn][e]};
             v1[e] = float2{ur[1][tn][e], ui[1][tn][e]};
         }
-        const MfFrag B0 = mf_split4(v0, sa, sa, a.fzero), B1 = mf_split4(v1, sb2, sb2, a.fzero);
+        const MfFrag B0 = mr_split4s<false>(v0, sa), B1 = mr_split4s<false>(v1, sb2);
         mf_f4 dr, di;
         mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
         mr_mfma_b<false>(dr, di, mr_table(ft, 2, l), B1);
@@ -528,16 +605,16 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
     MrUnit U{};
     MrTile T{};
     const bool any = u < nunit && (U = mr_unit(a, u, nunit), mr_tile(a, U, 0, T));
+    mr_f2a gm[2][4];   // multiplier inputs of the tile (kept for the second tile of a pair)
     if (any) {
         const char* ib = mr_tile_base(a.in, a.ia, T);
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             __builtin_amdgcn_global_load_lds(ib + so.iu[k] + so.ilane, (mr_lds_ptr)(stg0 + (w + 16 * k) * 1024), 16, 0, 0);
+        mr_kload<OPK>(gm, a, T, T.col0 + w, l0 & 15, l0 >> 4);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    float gm[2][2][4];   // multipliers of the tile (kept for the second tile of a pair)
-    int kyo_g = -1;      // octant row they belong to
 
     MrTile Tp = T;
     int it = 0;
@@ -558,12 +635,11 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
         }
         const bool more = un < nunit && (mn == 1 || (Un = mr_unit(a, un, nunit), mr_tile(a, Un, 0, Tn)));
         // the second tile of a slab pair keeps the first one's multipliers
-        const bool reuse_g = m == 1 && T.kyo == kyo_g && mr_even_in_ky<OPK>(a, T.comp);
+        const bool load_g = more && !(mn == 1 && Tn.kyo == T.kyo);
         char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp));
         const char* ibase = mr_tile_base(a.in, a.ia, Tn);
         const int nv = a.ncol - Tp.col0 - 2 * (l & 7);   // valid columns of this lane's chunk of Tp
-        mr_step<OPK>(a, cur, nxt, twm, ftw, T, so, obase, nv, it > 0, ibase, more, reuse_g, gm, w, l);
-        kyo_g = T.kyo;
+        mr_step<OPK>(a, cur, nxt, twm, ftw, T, so, obase, nv, it > 0, ibase, more, Tn, load_g, gm, w, l);
         // this wavefront's fetch of the next tile and the previous tile's
         // stores are done; every wavefront's output is in cur
         __builtin_amdgcn_s_waitcnt(0);
