@@ -92,87 +92,65 @@ __device__ __forceinline__ void mr_scale(float m, float& s, float& inv)
     inv = __uint_as_float((unsigned)(127 - 10 + e) << 23);
 }
 
-// largest |re|, |im| of eight complex values (max3 with absolute-value
-// modifiers: one instruction per value, no NaN canonicalisation of memory
-// operands; a NaN input is skipped, and propagates through the transform
-// regardless).  The trailing nop gives the result the two wait states a DPP
-// read of a VGPR written by a VALU instruction needs.
+// Inline assembly here never writes a register the compiler has not just
+// written itself: every output is tied to an input produced by compiler-visible
+// code, so the wait states that protect an in-flight MFMA's operands from a
+// later write (which the compiler does not count for inline assembly) are
+// already in place.  A trailing nop gives the results the two wait states a
+// DPP or MFMA read of a VGPR written by a VALU instruction needs.
+
+// largest |re|, |im| of eight complex values: max3 with absolute-value
+// modifiers, one instruction per two values, on no NaN-canonicalised copies
+// of memory operands (a NaN is skipped by the maximum and propagates through
+// the transform regardless)
 __device__ __forceinline__ float mr_absmax8(const float2 (&u)[4], const float2 (&v)[4])
 {
-    float m;
-    asm("v_max3_f32 %0, |%1|, |%2|, 0\n\t"
+    float m = __builtin_fabsf(u[0].x);
+    asm("v_max3_f32 %0, |%1|, |%2|, %0\n\t"
         "v_max3_f32 %0, |%3|, |%4|, %0\n\t"
         "v_max3_f32 %0, |%5|, |%6|, %0\n\t"
         "v_max3_f32 %0, |%7|, |%8|, %0\n\t"
         "v_max3_f32 %0, |%9|, |%10|, %0\n\t"
         "v_max3_f32 %0, |%11|, |%12|, %0\n\t"
         "v_max3_f32 %0, |%13|, |%14|, %0\n\t"
-        "v_max3_f32 %0, |%15|, |%16|, %0\n\t"
+        "v_max3_f32 %0, |%15|, |%15|, %0\n\t"
         "s_nop 1"
-        : "=&v"(m)
-        : "v"(u[0].x), "v"(u[0].y), "v"(u[1].x), "v"(u[1].y), "v"(u[2].x), "v"(u[2].y), "v"(u[3].x), "v"(u[3].y),
-          "v"(v[0].x), "v"(v[0].y), "v"(v[1].x), "v"(v[1].y), "v"(v[2].x), "v"(v[2].y), "v"(v[3].x), "v"(v[3].y));
+        : "+v"(m)
+        : "v"(u[0].y), "v"(u[1].x), "v"(u[1].y), "v"(u[2].x), "v"(u[2].y), "v"(u[3].x), "v"(u[3].y), "v"(v[0].x),
+          "v"(v[0].y), "v"(v[1].x), "v"(v[1].y), "v"(v[2].x), "v"(v[2].y), "v"(v[3].x), "v"(v[3].y));
     return m;
 }
 
 // fp16 hi/lo split of four complex values scaled by s (the imaginary parts
-// by -s for NEGY): hi = f16(x s) rounded to nearest, lo = f16(x s - hi), four
-// mixed-precision FMAs per complex value (v_fma_mixlo/mixhi write one half
-// of a register; op_sel picks the half of hi that the remainder subtracts).
-// The nop gives the two wait states an MFMA read of a VGPR written by a VALU
-// instruction needs, which the compiler does not count for inline assembly.
+// by -s for NEGY; z a run-time zero): hi = f16(x s) rounded to nearest, by
+// compiler-visible mixed FMAs, and lo = f16(x s - hi) by two v_fma_mix per
+// value written into the register of the real part (op_sel picks the half of
+// hi that the remainder subtracts).  The inputs are consumed.
 template <bool NEGY>
-__device__ __forceinline__ MfFrag mr_split4s(const float2 (&v)[4], float s)
+__device__ __forceinline__ MfFrag mr_split4s(float2 (&v)[4], float s, float z)
 {
-    unsigned H[4], Lo[4];
-    if constexpr (NEGY) {
-        asm("v_fma_mixlo_f16 %0, %8, %16, 0\n\t"
-            "v_fma_mixhi_f16 %0, %9, -%16, 0\n\t"
-            "v_fma_mixlo_f16 %4, %8, %16, -%0 op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixhi_f16 %4, %9, -%16, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixlo_f16 %1, %10, %16, 0\n\t"
-            "v_fma_mixhi_f16 %1, %11, -%16, 0\n\t"
-            "v_fma_mixlo_f16 %5, %10, %16, -%1 op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixhi_f16 %5, %11, -%16, -%1 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixlo_f16 %2, %12, %16, 0\n\t"
-            "v_fma_mixhi_f16 %2, %13, -%16, 0\n\t"
-            "v_fma_mixlo_f16 %6, %12, %16, -%2 op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixhi_f16 %6, %13, -%16, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixlo_f16 %3, %14, %16, 0\n\t"
-            "v_fma_mixhi_f16 %3, %15, -%16, 0\n\t"
-            "v_fma_mixlo_f16 %7, %14, %16, -%3 op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixhi_f16 %7, %15, -%16, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-            "s_nop 1"
-            : "=&v"(H[0]), "=&v"(H[1]), "=&v"(H[2]), "=&v"(H[3]), "=&v"(Lo[0]), "=&v"(Lo[1]), "=&v"(Lo[2]),
-              "=&v"(Lo[3])
-            : "v"(v[0].x), "v"(v[0].y), "v"(v[1].x), "v"(v[1].y), "v"(v[2].x), "v"(v[2].y), "v"(v[3].x),
-              "v"(v[3].y), "v"(s));
-    } else {
-        asm("v_fma_mixlo_f16 %0, %8, %16, 0\n\t"
-            "v_fma_mixhi_f16 %0, %9, %16, 0\n\t"
-            "v_fma_mixlo_f16 %4, %8, %16, -%0 op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixhi_f16 %4, %9, %16, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixlo_f16 %1, %10, %16, 0\n\t"
-            "v_fma_mixhi_f16 %1, %11, %16, 0\n\t"
-            "v_fma_mixlo_f16 %5, %10, %16, -%1 op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixhi_f16 %5, %11, %16, -%1 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixlo_f16 %2, %12, %16, 0\n\t"
-            "v_fma_mixhi_f16 %2, %13, %16, 0\n\t"
-            "v_fma_mixlo_f16 %6, %12, %16, -%2 op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixhi_f16 %6, %13, %16, -%2 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixlo_f16 %3, %14, %16, 0\n\t"
-            "v_fma_mixhi_f16 %3, %15, %16, 0\n\t"
-            "v_fma_mixlo_f16 %7, %14, %16, -%3 op_sel_hi:[0,0,1]\n\t"
-            "v_fma_mixhi_f16 %7, %15, %16, -%3 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
-            "s_nop 1"
-            : "=&v"(H[0]), "=&v"(H[1]), "=&v"(H[2]), "=&v"(H[3]), "=&v"(Lo[0]), "=&v"(Lo[1]), "=&v"(Lo[2]),
-              "=&v"(Lo[3])
-            : "v"(v[0].x), "v"(v[0].y), "v"(v[1].x), "v"(v[1].y), "v"(v[2].x), "v"(v[2].y), "v"(v[3].x),
-              "v"(v[3].y), "v"(s));
+    const float sy = NEGY ? -s : s;
+    unsigned H[4], X[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        H[e] = __builtin_bit_cast(unsigned, mf_h2{(_Float16)__builtin_fmaf(v[e].x, s, z), (_Float16)__builtin_fmaf(v[e].y, sy, z)});
+        X[e] = __float_as_uint(v[e].x);
     }
+    asm("v_fma_mixlo_f16 %0, %0, %8, -%4 op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixhi_f16 %0, %9, %13, -%4 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixlo_f16 %1, %1, %8, -%5 op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixhi_f16 %1, %10, %13, -%5 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixlo_f16 %2, %2, %8, -%6 op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixhi_f16 %2, %11, %13, -%6 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixlo_f16 %3, %3, %8, -%7 op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixhi_f16 %3, %12, %13, -%7 op_sel:[0,0,1] op_sel_hi:[0,0,1]\n\t"
+        "s_nop 1"
+        : "+v"(X[0]), "+v"(X[1]), "+v"(X[2]), "+v"(X[3])
+        : "v"(H[0]), "v"(H[1]), "v"(H[2]), "v"(H[3]), "v"(s), "v"(v[0].y), "v"(v[1].y), "v"(v[2].y), "v"(v[3].y),
+          "v"(sy));
     MfFrag f;
     f.hi = __builtin_bit_cast(mf_h8, H);
-    f.lo = __builtin_bit_cast(mf_h8, Lo);
+    f.lo = __builtin_bit_cast(mf_h8, X);
     return f;
 }
 
@@ -460,7 +438,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) {
-            const MfFrag X = mr_split4s<false>(x[tb], s0);
+            const MfFrag X = mr_split4s<false>(x[tb], s0, a.fzero);
             mr_mfma_a<true>(yr[tb][0], yi[tb][0], X, Fe);
             mr_mfma_a<true>(yr[tb][1], yi[tb][1], X, Fo);
         }
@@ -526,7 +504,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         }
         mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
-            const MfFrag XA = mr_split4s<true>(za, s2[tk]), XB = mr_split4s<true>(zb, s2[tk]);
+            const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
             mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
@@ -558,7 +536,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
             v0[e] = float2{ur[0][tn][e], ui[0][tn][e]};
             v1[e] = float2{ur[1][tn][e], ui[1][tn][e]};
         }
-        const MfFrag B0 = mr_split4s<false>(v0, sa), B1 = mr_split4s<false>(v1, sb2);
+        const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
         mf_f4 dr, di;
         mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
         mr_mfma_b<false>(dr, di, mr_table(ft, 2, l), B1);
