@@ -49,8 +49,7 @@ namespace psamd {
 struct MfmaZR {
     static constexpr int L = 1024, NZ = 512, P = 16, NW = 16;
     static constexpr int STG = NZ * P * 8;            // 64 KB per staging buffer
-    static constexpr int TWP = 33;                    // twiddle-matrix row pitch (complex)
-    static constexpr int TWB = 32 * TWP * 8;
+    static constexpr int TWB = 16 * 64 * 8;           // twiddles [slot][lane] (complex)
     static constexpr int FTB = 3 * 4 * 64 * 16;       // [E16, P, Q][rh, ih, rl, il][lane]
     static constexpr size_t lds_bytes() { return 2 * (size_t)STG + TWB + FTB; }
 };
@@ -409,7 +408,6 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
                                         char* obase, int nv, bool hasprev, const char* ibase, bool more,
                                         const MrTile& Tn, bool load_g, mr_f2a (&gm)[2][4], int w, int l)
 {
-    constexpr int TWP = MfmaZR::TWP;
     constexpr bool mimag = mf_imag_multiplier<OPK>();
     const int g = l >> 4, j = l & 15;
     const int p = w, kb = 2 * j + 128 * g;
@@ -445,14 +443,15 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
     }
     mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 1);
     // twiddle W1024^(b k1): matrix row k1 = 2 j + tk, column b = 8 g + 2 e + tb
-    const float2* twl = twm + 2 * j * TWP + 8 * g;
+    // twiddles lane-major: slot 8 t + 2 e + u of lane (j, g) holds W1024^((2 j + t)(8 g + 2 e + u))
+    const float2* twl = twm + l;
 #pragma unroll
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
         for (int tk = 0; tk < 2; ++tk)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const float2 v = cmul(float2{yr[tb][tk][e], yi[tb][tk][e]}, twl[tk * TWP + 2 * e + tb]);
+                const float2 v = cmul(float2{yr[tb][tk][e], yi[tb][tk][e]}, twl[(8 * tk + 2 * e + tb) * 64]);
                 yr[tb][tk][e] = v.x;
                 yi[tb][tk][e] = v.y;
             }
@@ -514,7 +513,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         for (int tn = 0; tn < 2; ++tn)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const float2 v = cmul(float2{ur[tk][tn][e], ui[tk][tn][e]}, twl[tn * TWP + 2 * e + tk]);
+                const float2 v = cmul(float2{ur[tk][tn][e], ui[tk][tn][e]}, twl[(8 * tn + 2 * e + tk) * 64]);
                 ur[tk][tn][e] = v.x;
                 ui[tk][tn][e] = v.y;
             }
@@ -555,7 +554,7 @@ template <int OPK>
 __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
 {
     using M = MfmaZR;
-    constexpr int L = M::L, TWP = M::TWP;
+    constexpr int L = M::L;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     unsigned char* stg0 = smem_raw;
     float2* twm = reinterpret_cast<float2*>(smem_raw + 2 * M::STG);
@@ -566,10 +565,13 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
     const int nunit = mr_nunit(a);
     const int G = gridDim.x;
 
-    // twiddle matrix W1024^(r s), r, s < 32 (1024 threads, one entry each)
+    // twiddles W1024^(r s) lane-major (1024 threads, one entry each): entry
+    // (slot 8 t + 2 e + u, lane 16 g + j) is r = 2 j + t, s = 8 g + 2 e + u, so
+    // every twiddle read of a wavefront is one linear 512-byte access
     {
-        const int r = tid >> 5, s = tid & 31;
-        twm[r * TWP + s] = a.plan.tw[(r * s) & (L - 1)];
+        const int slot = tid >> 6, j = tid & 15, g = (tid >> 4) & 3;
+        const int r = 2 * j + (slot >> 3), s = 8 * g + (slot & 7);
+        twm[tid] = a.plan.tw[(r * s) & (L - 1)];
     }
     if (tid < 3 * 4 * 64) {
         // tables E16, P, Q: global [table][lane][rh, rl, ih, il] -> LDS [table][rh, ih, rl, il][lane]
