@@ -2,6 +2,7 @@
 #include "ps_mfma.hpp"
 #include "ps_mfma_launch.hpp"
 #include "ps_mfma_reg.hpp"
+#include "ps_mfma_reg2048.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -184,11 +185,26 @@ template <int OPK, int P> static hipError_t launch_zmfma2048_p(PencilArgs<float>
     return hipGetLastError();
 }
 
-// persistent (tuning mfma 2) selects the 8-pencil workgroups that hold the
-// pencil for both residues; otherwise 4-pencil workgroups that reload it
-template <int OPK> static hipError_t launch_zmfma2048_op(PencilArgs<float> a, int p8, hipStream_t s)
+// tuning mfma 1: the register-pencil kernel when the arguments allow it (as
+// for transforms of 1024), else 4-pencil workgroups that reload the pencil for
+// residue 1; 2: 8-pencil workgroups that hold it; 3: the 4-pencil ones
+template <int OPK> static hipError_t launch_zmfma2048_op(PencilArgs<float> a, int variant, hipStream_t s)
 {
-    return p8 ? launch_zmfma2048_p<OPK, 8>(a, s) : launch_zmfma2048_p<OPK, 4>(a, s);
+    if (variant == 1 && zmfma_reg_ok(a)) {
+        // one workgroup of 16 wavefronts (two per pencil) per CU at a time,
+        // four per CU over the launch, as for transforms of 1024
+        const size_t shm = MfmaZR2::lds_bytes();
+        hipError_t e = allow_lds(k_zmfma2048r<OPK>, shm);
+        if (e != hipSuccess) return e;
+        a.ncoltile = (a.ncol + MfmaZR2::P - 1) / MfmaZR2::P;
+        if ((long)a.ncoltile * a.nslab * a.ncomp >= (1L << 31)) return hipErrorInvalidValue;
+        const long nunit = (long)a.ncoltile * ((a.nslab + 1) / 2) * a.ncomp;
+        const long ncu = std::max(device_cus(), 8);
+        const long nwg = std::min<long>(nunit, 4 * ncu);
+        hipLaunchKernelGGL((k_zmfma2048r<OPK>), dim3((unsigned)nwg), dim3(64 * MfmaZR2::NW), shm, s, a);
+        return hipGetLastError();
+    }
+    return variant == 2 ? launch_zmfma2048_p<OPK, 8>(a, s) : launch_zmfma2048_p<OPK, 4>(a, s);
 }
 
 hipError_t launch_zmfma(const PencilArgs<float>& a, int variant, hipStream_t s)
@@ -207,13 +223,12 @@ hipError_t launch_zmfma(const PencilArgs<float>& a, int variant, hipStream_t s)
     if (a.plan.L == 2048) {
         // two 1024-point residues per pencil
         if (a.n_in != 1024 || a.n_out != 1024) return hipErrorInvalidValue;
-        const int p8 = variant == 2 ? 1 : 0;
         switch (a.op) {
-        case OP_NONE: return launch_zmfma2048_op<OP_NONE>(a, p8, s);
-        case OP_PLAIN: return launch_zmfma2048_op<OP_PLAIN>(a, p8, s);
-        case OP_GRAD: return launch_zmfma2048_op<OP_GRAD>(a, p8, s);
-        case OP_RHSF: return launch_zmfma2048_op<OP_RHSF>(a, p8, s);
-        case OP_LAPL: return launch_zmfma2048_op<OP_LAPL>(a, p8, s);
+        case OP_NONE: return launch_zmfma2048_op<OP_NONE>(a, variant, s);
+        case OP_PLAIN: return launch_zmfma2048_op<OP_PLAIN>(a, variant, s);
+        case OP_GRAD: return launch_zmfma2048_op<OP_GRAD>(a, variant, s);
+        case OP_RHSF: return launch_zmfma2048_op<OP_RHSF>(a, variant, s);
+        case OP_LAPL: return launch_zmfma2048_op<OP_LAPL>(a, variant, s);
         default: return hipErrorInvalidValue;
         }
     }

@@ -234,11 +234,12 @@ __device__ __forceinline__ int mr_nunit(const PencilArgs<float>& a)
     return a.ncomp * ((a.nslab + 1) >> 1) * a.ncoltile;
 }
 
-// unit u's component, slab pair and column tile
+// unit u's component, slab pair and column tile (tiles of P columns)
 struct MrUnit {
     int comp, sp, col0;
 };
 
+template <int P = 16>
 __device__ __forceinline__ MrUnit mr_unit(const PencilArgs<float>& a, int u, int nunit)
 {
     int lin = u;
@@ -253,7 +254,7 @@ __device__ __forceinline__ MrUnit mr_unit(const PencilArgs<float>& a, int u, int
     v.comp = lin / nunit1;
     const int rem = lin - v.comp * nunit1;
     v.sp = rem / a.ncoltile;
-    v.col0 = (rem - v.sp * a.ncoltile) * 16;
+    v.col0 = (rem - v.sp * a.ncoltile) * P;
     return v;
 }
 

@@ -1,0 +1,414 @@
+// Mixed-precision z pass on the matrix cores, transform length L = 2048
+// (Mz of a 1024-cell unbounded z direction), register-pencil form.
+//
+// The zero-padded 2048-point transform of a pencil x (1024 points) is two
+// 1024-point transforms, one per residue of the frequency mod 2:
+//   X[2 k'] = DFT1024(x)[k'],   X[2 k' + 1] = DFT1024(W2048^n x)[k'],
+// and the truncated inverse is y[n] = IDFT1024(Z0)[n] + W2048^-n IDFT1024(Z1)[n],
+// n < 1024 (the reference's solve3d.cpp:243-441 at Mz = 2048).  Each residue
+// is the four-stage transform of the register-pencil kernel of 1024 (same
+// tables, twiddles and accumulator-as-operand chaining), run by a wavefront of
+// its own: wavefronts w and w + 8 of a workgroup own residues 0 and 1 of
+// pencil w, column w of an 8-column tile.  With 32 inputs, stage 1 splits the
+// outputs by parity through the inputs (a = x[m] + x[m + 16] by E16 gives the
+// even k1, d = x[m] - x[m + 16] by P the odd ones); with 32 outputs, inverse
+// stage 2 keeps its even and odd products apart and forms rows n1 and n1 + 16
+// as E + O and E - O.  The residue-1 wavefront multiplies its input by
+// W2048^n and leaves its result in the tile buffer, where the residue-0
+// wavefront multiplies it by W2048^-n and adds its own.
+//
+// Tiles: 8 adjacent kx columns of one ky slab (64-byte rows) x 1024 z rows,
+// 64 KB in LDS; two buffers, stored and fetched in four slices per wavefront
+// over the first half of each transform, as in the kernel of 1024.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ps_mfma_reg.hpp"
+
+namespace psamd {
+
+struct MfmaZR2 {
+    static constexpr int L = 2048, NZ = 1024, P = 8, NW = 16;
+    static constexpr int STG = NZ * P * 8;            // 64 KB per staging buffer
+    static constexpr int TWB = 16 * 64 * 8;           // W1024 twiddles [slot][lane]
+    static constexpr int TSB = 16 * 64 * 8;           // W2048^n of the lane's 16 positions [slot][lane]
+    static constexpr int FTB = 3 * 4 * 64 * 16;       // [E16, P, Q][rh, ih, rl, il][lane]
+    static constexpr size_t lds_bytes() { return 2 * (size_t)STG + TWB + TSB + FTB; }
+};
+
+// Staging layout: block n (1 KB, one direct-load instruction) holds rows
+// 16 n .. 16 n + 15, 64 bytes each; chunk c (columns 2 c, 2 c + 1) of row r
+// sits in slot c ^ ((r >> 2) & 3), which spreads a column read over the banks.
+__device__ __forceinline__ int mr2_swz(int r) { return (r >> 2) & 3; }
+
+// 64-bit byte offset of position z0 (a multiple of 16) and the 32-bit one of
+// zr < 16 (plain positions or blocks of 2^sh >= 8 planes)
+__device__ __forceinline__ long mr2_zoff_u(const Addr& d, int z0)
+{
+    return (d.pos_sh >= 30) ? (long)z0 * d.pos_lo * 8
+                            : ((long)(z0 >> d.pos_sh) * d.pos_hi + (long)(z0 & ((1 << d.pos_sh) - 1)) * d.pos_lo) * 8;
+}
+
+// Per-lane offsets of the slices (lane l: row r = l >> 2 of a block, LDS
+// slot l & 3): the fetch loads the chunk that belongs in the lane's slot, the
+// store writes chunk l & 3 from its slot.
+struct Mr2SliceOffs {
+    unsigned lds;      // LDS offset of the stored chunk within a block
+    unsigned ilane;    // fetch: byte offset of the chunk the lane loads, within 16 rows
+    unsigned olane;    // store: byte offset of chunk l & 3, within 16 rows
+    int c;             // store: chunk index (columns 2 c, 2 c + 1)
+};
+
+__device__ __forceinline__ Mr2SliceOffs mr2_slice_offs(const PencilArgs<float>& a, int l)
+{
+    Mr2SliceOffs o;
+    const int r = l >> 2, s = l & 3;
+    o.c = s;
+    o.lds = (unsigned)(r * 64 + ((s ^ mr2_swz(r)) << 4));
+    o.ilane = mr_zoff_l(a.ia, r) + 16u * (unsigned)(s ^ mr2_swz(r));
+    o.olane = mr_zoff_l(a.oa, r) + 16u * (unsigned)s;
+    return o;
+}
+
+// Slice k of wavefront w: store block n = w + 16 k of the previous tile (from
+// buf, to obase: that tile's column 0, ncv valid columns) and fetch the next
+// tile's block n (from ibase) into the same place.
+__device__ __forceinline__ void mr2_slice(unsigned char* __restrict__ buf, const PencilArgs<float>& a,
+                                          const Mr2SliceOffs& so, char* obase, int ncv, bool hasprev,
+                                          const char* ibase, bool more, int w, int k)
+{
+    const int n = w + 16 * k;
+    if (hasprev) {
+        const float4 v = *reinterpret_cast<const float4*>(buf + n * 1024 + so.lds);
+        char* o = obase + mr2_zoff_u(a.oa, 16 * n) + so.olane;
+        const int nv = ncv - 2 * so.c;
+        if (nv >= 2) *reinterpret_cast<float4*>(o) = v;
+        else if (nv == 1) *reinterpret_cast<float2*>(o) = float2{v.x, v.y};
+    }
+    if (more)
+        __builtin_amdgcn_global_load_lds(ibase + mr2_zoff_u(a.ia, 16 * n) + so.ilane, (mr_lds_ptr)(buf + n * 1024),
+                                         16, 0, 0);
+}
+
+// Multiplier inputs of residue rs: lane (j, g)'s 16 spectrum points
+// kz = 2 k' + rs, k' = k1 + 32 k2, k1 = 2 j + tt, k2 = 4 g + e + 16 h, i.e.
+// kz = kb + 2 tt + 64 e + 1024 h with kb = 4 j + 256 g + rs; octant index kz
+// (h = 0) or 2048 - kz = 1024 - (kz of h = 0) (h = 1), [ky][kx][kz] layout.
+// Kept raw as for the kernel of 1024 (the operator's k-factor is applied
+// where the multiplier is used).
+template <int OPK>
+__device__ __forceinline__ void mr2_kload(float (&gr)[2][4][2], const PencilArgs<float>& a, const MrTile& T, int col,
+                                          int kb)
+{
+    const TileK<float> tk = tile_k(a, col, T.slab, col < a.ncol);
+    if constexpr (mr_loads_g<OPK>()) {
+        const float* row = tk.grow ? tk.grow : a.G;
+        const float* r0 = row + kb;
+        const float* r1 = row + 1024 - kb;
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int tt = 0; tt < 2; ++tt) {
+                gr[0][e][tt] = r0[2 * tt + 64 * e];
+                gr[1][e][tt] = r1[-(2 * tt + 64 * e)];
+            }
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int tt = 0; tt < 2; ++tt)
+                    gr[h][e][tt] = (OPK == OP_NONE) ? 1.0f : mf_rhsf(a, tk, kb + 2 * tt + 64 * e + 1024 * h);
+    }
+}
+
+template <int OPK>
+__device__ __forceinline__ float mr2_kmul(const PencilArgs<float>& a, const MfKCoef& kc, const float (&gr)[2][4][2],
+                                          int tt, int h, int e, int kb)
+{
+    const float v = gr[h][e][tt];
+    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL) return mf_kval<OPK>(a, kc, kb + 2 * tt + 64 * e + 1024 * h, v);
+    return v;
+}
+
+// LDS-only workgroup barrier: this wavefront's LDS accesses complete, vector
+// memory (the slices' stores and fetches) still in flight
+__device__ __forceinline__ void mr2_lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// One tile T for wavefront w (pencil p = w & 7, residue rs = w >> 3), with the
+// slices of the previous and next tiles over its first half and the next
+// tile's multiplier inputs loaded after the last use of gm; the result of
+// pencil p is left in column p of cur.  Nothing here waits for vector memory.
+template <int OPK>
+__device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                         unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
+                                         const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
+                                         const MrTile& T, const Mr2SliceOffs& so, char* obase, int ncv, bool hasprev,
+                                         const char* ibase, bool more, const MrTile& Tn, bool load_g,
+                                         float (&gm)[2][4][2], int w, int l)
+{
+    constexpr bool mimag = mf_imag_multiplier<OPK>();
+    const int g = l >> 4, j = l & 15;
+    const int p = w & 7, rs = w >> 3, kb = 4 * j + 256 * g + rs;
+    // staging offsets of rows z = 2 j + q + 32 (4 g + e + 16 h) (q = 0, 1):
+    // block (j >> 3) + 8 g + 2 e + 32 h, row 2 (j & 7) + q
+    unsigned sb[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = 2 * (j & 7) + q;
+        sb[q] = (unsigned)(((j >> 3) + 8 * g) * 1024 + r * 64 + (((p >> 1) ^ mr2_swz(r)) << 4) + (p & 1) * 8);
+    }
+    // ---- input x[b + 32 m], b = 2 j + tb, m = 4 g + e + 16 h ----
+    float2 x[2][2][4];
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                x[tb][h][e] = *reinterpret_cast<const float2*>(cur + sb[tb] + 2048 * e + 32768 * h);
+    // every wavefront has read its pencil before any residue result is written
+    mr2_lds_barrier();
+    mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 0);
+    const float2* twl = twm + l;
+    const float2* tsl = tws + l;   // slot 8 tb + 4 h + e: W2048^(2 j + tb + 32 (4 g + e + 16 h))
+    if (rs) {
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) x[tb][h][e] = cmul(x[tb][h][e], tsl[(8 * tb + 4 * h + e) * 64]);
+    }
+    // one scale for the pencil: |x[m] +- x[m + 16]| <= 2 max
+    float s0, inv0;
+    {
+        const float m0 = mr_absmax8(x[0][0], x[0][1]), m1 = mr_absmax8(x[1][0], x[1][1]);
+        mr_scale(2.0f * mr_wave_max(__builtin_fmaxf(m0, m1)), s0, inv0);
+    }
+    // ---- stage 1 (data x table): Y[b][k1] = sum_m W32^(m k1) x[b + 32 m] ----
+    mf_f4 yr[2][2], yi[2][2];
+    {
+        const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) {
+            float2 xa[4], xd[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                xa[e] = float2{x[tb][0][e].x + x[tb][1][e].x, x[tb][0][e].y + x[tb][1][e].y};
+                xd[e] = float2{x[tb][0][e].x - x[tb][1][e].x, x[tb][0][e].y - x[tb][1][e].y};
+            }
+            const MfFrag XA = mr_split4s<false>(xa, s0, a.fzero), XD = mr_split4s<false>(xd, s0, a.fzero);
+            mr_mfma_a<true>(yr[tb][0], yi[tb][0], XA, Fe);
+            mr_mfma_a<true>(yr[tb][1], yi[tb][1], XD, Fo);
+        }
+    }
+    mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 1);
+    // twiddle W1024^(b k1), lane-major as in the kernel of 1024
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int tk = 0; tk < 2; ++tk)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float2 v = cmul(float2{yr[tb][tk][e], yi[tb][tk][e]}, twl[(8 * tk + 2 * e + tb) * 64]);
+                yr[tb][tk][e] = v.x;
+                yi[tb][tk][e] = v.y;
+            }
+    mf_f4 ur[2][2], ui[2][2];
+    float s2[2], inv2[2];
+    MfKCoef kc{};
+    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL)
+        kc = mf_kcoef<OPK>(a, tile_k(a, T.col0 + p, T.slab, T.col0 + p < a.ncol), T.comp);
+#pragma unroll
+    for (int tk = 0; tk < 2; ++tk) {
+        // ---- stage 2 (table x data) for column k1 = 2 j + tk ----
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = float2{yr[0][tk][e], yi[0][tk][e]};
+            v1[e] = float2{yr[1][tk][e], yi[1][tk][e]};
+        }
+        const MfFrag B0 = mr_split4u(v0, a.fone), B1 = mr_split4u(v1, a.fone);
+        mf_f4 er, ei, orr, oi;
+        mr_mfma_b<true>(er, ei, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(orr, oi, mr_table(ft, 2, l), B1);
+        float2 z0[4], z1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float m0 = mr2_kmul<OPK>(a, kc, gm, tk, 0, e, kb), m1 = mr2_kmul<OPK>(a, kc, gm, tk, 1, e, kb);
+            const float xr0 = er[e] + orr[e], xi0 = ei[e] + oi[e];
+            const float xr1 = er[e] - orr[e], xi1 = ei[e] - oi[e];
+            if constexpr (mimag) {
+                z0[e] = float2{-xi0 * m0, xr0 * m0};
+                z1[e] = float2{-xi1 * m1, xr1 * m1};
+            } else {
+                z0[e] = float2{xr0 * m0, xi0 * m0};
+                z1[e] = float2{xr1 * m1, xi1 * m1};
+            }
+        }
+        if (tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 2);
+        if (tk == 1 && load_g) mr2_kload<OPK>(gm, a, Tn, Tn.col0 + p, kb);
+        // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
+        float2 za[4], zb[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            za[e] = float2{z0[e].x + z1[e].x, z0[e].y + z1[e].y};
+            zb[e] = float2{z0[e].x - z1[e].x, z0[e].y - z1[e].y};
+        }
+        mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
+        {
+            const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
+            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
+        }
+        if (tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 3);
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float2 v = cmul(float2{ur[tk][tn][e], ui[tk][tn][e]}, twl[(8 * tn + 2 * e + tk) * 64]);
+                ur[tk][tn][e] = v.x;
+                ui[tk][tn][e] = v.y;
+            }
+    }
+    // inverse stage 2 input scales from the bound, as in the kernel of 1024
+    const float smin = __builtin_fminf(s2[0], s2[1]);
+    const float sa = 0.5f * smin * inv2[0], sb2 = 0.5f * smin * inv2[1];
+    const float fo = 2.0f * inv0 * __builtin_fmaxf(inv2[0], inv2[1]);
+    // ---- inverse stage 2: y[32 n1 + n2] = conj(sum_k1 W32^(n1 k1) T[k1][n2]) for n1 < 32:
+    // rows n1 = 4 g + e (E + O) and n1 + 16 (E - O) ----
+    float2 o[2][2][4];   // [tn][h][e]: rows z = 2 j + tn + 32 (4 g + e + 16 h)
+#pragma unroll
+    for (int tn = 0; tn < 2; ++tn) {
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = float2{ur[0][tn][e], ui[0][tn][e]};
+            v1[e] = float2{ur[1][tn][e], ui[1][tn][e]};
+        }
+        const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
+        mf_f4 er, ei, orr, oi;
+        mr_mfma_b<true>(er, ei, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(orr, oi, mr_table(ft, 2, l), B1);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            o[tn][0][e] = float2{(er[e] + orr[e]) * fo, -(ei[e] + oi[e]) * fo};
+            o[tn][1][e] = float2{(er[e] - orr[e]) * fo, -(ei[e] - oi[e]) * fo};
+        }
+        if (rs) {
+            // residue 1 leaves its result in the pencil's column
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    *reinterpret_cast<float2*>(cur + sb[tn] + 2048 * e + 32768 * h) = o[tn][h][e];
+        }
+    }
+    mr2_lds_barrier();
+    if (!rs) {
+        // y = y0 + W2048^-n y1
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float2* q = reinterpret_cast<float2*>(cur + sb[tn] + 2048 * e + 32768 * h);
+                    const float2 t = tsl[(8 * tn + 4 * h + e) * 64];
+                    const float2 y1 = *q;
+                    *q = float2{o[tn][h][e].x + y1.x * t.x + y1.y * t.y, o[tn][h][e].y + y1.y * t.x - y1.x * t.y};
+                }
+    }
+}
+
+// Persistent kernel over units of two tiles (slabs ky, My - ky) of 8 columns,
+// as the kernel of 1024.
+template <int OPK>
+__global__ void __launch_bounds__(1024, 4) k_zmfma2048r(PencilArgs<float> a)
+{
+    using M = MfmaZR2;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    unsigned char* stg0 = smem_raw;
+    float2* twm = reinterpret_cast<float2*>(smem_raw + 2 * M::STG);
+    float2* tws = reinterpret_cast<float2*>(smem_raw + 2 * M::STG + M::TWB);
+    mf_h8* ftw = reinterpret_cast<mf_h8*>(smem_raw + 2 * M::STG + M::TWB + M::TSB);
+
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l0 = tid & 63;
+    const int nunit = mr_nunit(a);
+    const int G = gridDim.x;
+
+    {
+        const int slot = tid >> 6, j = tid & 15, g = (tid >> 4) & 3;
+        // W1024^(r s) = W2048^(2 r s), r = 2 j + t, s = 8 g + 2 e + u (slot 8 t + 2 e + u)
+        const int r = 2 * j + (slot >> 3), s = 8 * g + (slot & 7);
+        twm[tid] = a.plan.tw[(2 * r * s) & (M::L - 1)];
+        // W2048^n, n = 2 j + tb + 32 (4 g + e + 16 h) (slot 8 tb + 4 h + e)
+        const int n = 2 * j + (slot >> 3) + 32 * (4 * g + (slot & 3) + 16 * ((slot >> 2) & 1));
+        tws[tid] = a.plan.tw[n];
+    }
+    if (tid < 3 * 4 * 64) {
+        const int ln = tid & 63, f = (tid >> 6) & 3, tb = tid >> 8;
+        const int fo = (f == 0) ? 0 : (f == 1) ? 2 : (f == 2) ? 1 : 3;
+        ftw[(tb * 4 + fo) * 64 + ln] = reinterpret_cast<const mf_h8*>(a.mfrag16)[(tb * 64 + ln) * 4 + f];
+    }
+    const Mr2SliceOffs so = mr2_slice_offs(a, l0);
+    int u = blockIdx.x, m = 0;
+    MrUnit U{};
+    MrTile T{};
+    const bool any = u < nunit && (U = mr_unit<M::P>(a, u, nunit), mr_tile(a, U, 0, T));
+    float gm[2][4][2];
+    if (any) {
+        const char* ib = mr_tile_base(a.in, a.ia, T);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int n = w + 16 * k;
+            __builtin_amdgcn_global_load_lds(ib + mr2_zoff_u(a.ia, 16 * n) + so.ilane, (mr_lds_ptr)(stg0 + n * 1024),
+                                             16, 0, 0);
+        }
+        mr2_kload<OPK>(gm, a, T, T.col0 + (w & 7), 4 * (l0 & 15) + 256 * (l0 >> 4) + (w >> 3));
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    MrTile Tp = T;
+    int it = 0;
+    for (; any; ++it) {
+        int l = l0;
+        asm volatile("" : "+v"(l));
+        unsigned char* cur = stg0 + (it & 1) * M::STG;
+        unsigned char* nxt = stg0 + ((it & 1) ^ 1) * M::STG;
+        MrTile Tn{};
+        MrUnit Un = U;
+        int un = u, mn = 1;
+        if (m == 1 || !mr_tile(a, U, 1, Tn)) {
+            un = u + G;
+            mn = 0;
+        }
+        const bool more = un < nunit && (mn == 1 || (Un = mr_unit<M::P>(a, un, nunit), mr_tile(a, Un, 0, Tn)));
+        const bool load_g = more && !(mn == 1 && Tn.kyo == T.kyo);
+        char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp));
+        const char* ibase = mr_tile_base(a.in, a.ia, Tn);
+        const int ncv = a.ncol - Tp.col0;   // valid columns of the previous tile (may exceed 8)
+        mr2_step<OPK>(a, cur, nxt, twm, tws, ftw, T, so, obase, ncv, it > 0, ibase, more, Tn, load_g, gm, w, l);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        Tp = T;
+        T = Tn;
+        U = Un;
+        u = un;
+        m = mn;
+        if (!more) break;
+    }
+    if (any) {
+        unsigned char* last = stg0 + (it & 1) * M::STG;
+        char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp));
+        const int ncv = a.ncol - Tp.col0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mr2_slice(last, a, so, obase, ncv, true, nullptr, false, w, k);
+    }
+}
+
+}  // namespace psamd

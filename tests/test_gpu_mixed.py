@@ -48,8 +48,12 @@ def _compare(n, bc, dx, reg, op, b, want_rhs=False, tol=MIXED_TOL):
     return err
 
 
-# matrix-core kernel forms: 1 one tile per workgroup, 2 persistent workgroups
+# matrix-core kernel forms (PS_TUNE_MFMA) of transforms of 1024: 1 the
+# register-pencil kernel, 2 one tile per workgroup
 FORMS = ["1", "2"]
+# of transforms of 2048: 1 the register-pencil kernel, 2 eight-pencil and
+# 3 four-pencil workgroups
+FORMS2048 = ["1", "2", "3"]
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
@@ -141,21 +145,28 @@ CASES2048 = [
 
 @pytest.mark.parametrize("case", CASES2048, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
 @pytest.mark.parametrize("reg", [0, 2, 10])
-def test_mixed_2048_plain_grad_filtered_rhs(case, reg):
+@pytest.mark.parametrize("form", FORMS2048)
+def test_mixed_2048_plain_grad_filtered_rhs(case, reg, form, monkeypatch):
     """nz = 1024 unbounded: z transforms of 2048 as two 1024-point residues
-    on the matrix cores (plain, component-wise grad, filtered rhs)."""
+    on the matrix cores (plain, component-wise grad, filtered rhs), each
+    kernel form; rel-L2 <= 1e-6 against the oracle."""
+    monkeypatch.setenv("PS_TUNE_MFMA", form)
     n, bc, dx = case
-    announce("mixed 2048", n, bc, reg)
+    announce("mixed 2048", n, bc, reg, "form", form)
     assert _variant(n, bc, dx, reg) == "matrix-core"
     b = random_rhs(n, 61 + reg)
     _compare(n, bc, dx, reg, "plain", [b[0]], want_rhs=True)
     _compare(n, bc, dx, reg, "grad", b, want_rhs=True)
 
 
-def test_mixed_2048_potential_and_range():
-    """The potential extension's five quantities on the 2048-point
-    matrix-core kernel, and a source spanning 2^-40 .. 2^40 in magnitude."""
+@pytest.mark.parametrize("form", FORMS2048)
+def test_mixed_2048_potential_and_range(form, monkeypatch):
+    """The potential extension's five quantities on each 2048-point
+    matrix-core kernel form, and a source spanning 2^-40 .. 2^40 in
+    magnitude; rel-L2 <= 1e-6 against the oracle."""
+    monkeypatch.setenv("PS_TUNE_MFMA", form)
     n, bc, dx = CASES2048[0]
+    announce("mixed 2048 potential", n, "form", form)
     rng = np.random.default_rng(67)
     b = rng.standard_normal((n[2], n[1], n[0])) * np.exp2(rng.uniform(-40, 40, (n[2], 1, 1)))
     s = P.poisson_solver(P.PS_MIXED)
