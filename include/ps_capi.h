@@ -247,8 +247,9 @@ typedef struct ps_tuning {
     int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT) kernels where the length
                          has them: z transforms of 1024: 1 the persistent register-pencil
                          kernel (one pencil per wavefront, 16-column tiles), 2 one 8-pencil
-                         tile per workgroup; of 2048: 1 four pencils per workgroup, 2 eight;
-                         0 the FP32 kernels (PS_TUNE_MFMA, 1) */
+                         tile per workgroup; of 2048: 1 the register-pencil kernel (two
+                         wavefronts per pencil, 8-column tiles), 2 eight pencils per
+                         workgroup, 3 four; 0 the FP32 kernels (PS_TUNE_MFMA, 1) */
     int t2d;          /* 2D contexts whose fused y pass holds one or two columns per workgroup
                          (long pencils): the x pass output is transposed so that y pencils are
                          contiguous, and grad / curl transform their scalar once (PS_TUNE_T2D, 1) */

@@ -1916,7 +1916,7 @@ int ps_set_tuning(ps_ctx* c, const ps_tuning* t)
     }
     if (t->zblk < -2 || t->zblk > 10 || t->tile_x < 0 || t->tile_x > 64 || t->tile_y < 0 || t->tile_y > 64 ||
         t->tile_z < 0 || t->tile_z > 64 || t->radix_e < 0 || t->radix_e > 70 || t->lds_kb < 16 || t->lds_kb > 160 ||
-        t->halfb_z < -1 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1 || t->mfma < 0 || t->mfma > 2)
+        t->halfb_z < -1 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1 || t->mfma < 0 || t->mfma > 3)
         return fail(c, PS_EINVAL, "set_tuning: field out of range");
     c->tune_req = *t;
     c->tune_set = true;
@@ -1991,7 +1991,7 @@ int ps_autotune(ps_ctx* c, int iters, ps_tuning* best_out, double* best_ms_out)
         {&Tuning::tile_z, {0, 4, 8, 16, 32}, true},
         {&Tuning::tile_x, {0, 2, 4, 8, 16}, true},
         {&Tuning::xcd, {1, 0}, false},
-        {&Tuning::mfma, {1, 2, 0}, false},
+        {&Tuning::mfma, {1, 2, 3, 0}, false},
         {&Tuning::t2d, {1, 0}, false},
     };
     bool best_all_stat = all_stat;
