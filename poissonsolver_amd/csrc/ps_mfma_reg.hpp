@@ -293,6 +293,21 @@ __device__ __forceinline__ unsigned mr_zoff_l(const Addr& d, int zr)
                                (unsigned)(zr & ((1 << d.pos_sh) - 1)) * (unsigned)d.pos_lo) * 8u;
 }
 
+// An opaque copy of the lane offset v that depends on the eight complex
+// values z0, z1: loads addressed through it cannot be scheduled before those
+// values exist, so loads of the next tile's multiplier inputs into the
+// registers of this tile's (once z0, z1, which use them, are formed) need no
+// register copies.
+__device__ __forceinline__ int mr_after(int v, const float2 (&z0)[4], const float2 (&z1)[4])
+{
+    asm volatile(""
+                 : "+v"(v)
+                 : "v"(z0[0].x), "v"(z0[0].y), "v"(z0[1].x), "v"(z0[1].y), "v"(z0[2].x), "v"(z0[2].y), "v"(z0[3].x),
+                   "v"(z0[3].y), "v"(z1[0].x), "v"(z1[0].y), "v"(z1[1].x), "v"(z1[1].y), "v"(z1[2].x), "v"(z1[2].y),
+                   "v"(z1[3].x), "v"(z1[3].y));
+    return v;
+}
+
 // Multiplier inputs of lane (j, g)'s 16 spectrum points kz = k1 + 32 k2,
 // k1 = 2 j + tt, k2 = 4 g + e + 16 h, for pencil column col of tile T, kept as
 // loaded: gr[h][e] holds the pair tt = 0, 1 (h = 0) or tt = 1, 0 (h = 1).  The
@@ -308,14 +323,15 @@ template <int OPK> constexpr bool mr_loads_g() { return OPK == OP_PLAIN || OPK =
 
 template <int OPK>
 __device__ __forceinline__ void mr_kload(mr_f2a (&gr)[2][4], const PencilArgs<float>& a, const MrTile& T, int col, int j,
-                                         int g)
+                                         int g, int kbl)
 {
     const TileK<float> tk = tile_k(a, col, T.slab, col < a.ncol);
     const int kb = 2 * j + 128 * g;
     if constexpr (mr_loads_g<OPK>()) {
+        // kbl: kb, possibly an opaque copy that orders the loads
         const char* row = reinterpret_cast<const char*>(tk.grow ? tk.grow : a.G);
-        const char* r0 = row + kb * 4;
-        const char* r1 = row + (512 - kb) * 4 - 4 - 3 * 128;
+        const char* r0 = row + kbl * 4;
+        const char* r1 = row + (512 - kbl) * 4 - 4 - 3 * 128;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             gr[0][e] = *reinterpret_cast<const mr_f2a*>(r0 + 128 * e);
@@ -494,7 +510,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         }
         if (tk == 0) mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 2);
         // the next tile's multipliers, once this tile's are used
-        if (tk == 1 && load_g) mr_kload<OPK>(gm, a, Tn, Tn.col0 + p, j, g);
+        if (tk == 1 && load_g) mr_kload<OPK>(gm, a, Tn, Tn.col0 + p, j, g, mr_after(kb, z0, z1));
         // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
         float2 za[4], zb[4];
 #pragma unroll
@@ -592,7 +608,7 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma1024r(PencilArgs<float> a)
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             __builtin_amdgcn_global_load_lds(ib + so.iu[k] + so.ilane, (mr_lds_ptr)(stg0 + (w + 16 * k) * 1024), 16, 0, 0);
-        mr_kload<OPK>(gm, a, T, T.col0 + w, l0 & 15, l0 >> 4);
+        mr_kload<OPK>(gm, a, T, T.col0 + w, l0 & 15, l0 >> 4, 2 * (l0 & 15) + 128 * (l0 >> 4));
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();

@@ -98,13 +98,14 @@ __device__ __forceinline__ void mr2_slice(unsigned char* __restrict__ buf, const
 // where the multiplier is used).
 template <int OPK>
 __device__ __forceinline__ void mr2_kload(float (&gr)[2][4][2], const PencilArgs<float>& a, const MrTile& T, int col,
-                                          int kb)
+                                          int kb, int kbl)
 {
     const TileK<float> tk = tile_k(a, col, T.slab, col < a.ncol);
     if constexpr (mr_loads_g<OPK>()) {
+        // kbl: kb, possibly an opaque copy that orders the loads
         const float* row = tk.grow ? tk.grow : a.G;
-        const float* r0 = row + kb;
-        const float* r1 = row + 1024 - kb;
+        const float* r0 = row + kbl;
+        const float* r1 = row + 1024 - kbl;
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
@@ -249,7 +250,7 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
             }
         }
         if (tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 2);
-        if (tk == 1 && load_g) mr2_kload<OPK>(gm, a, Tn, Tn.col0 + p, kb);
+        if (tk == 1 && load_g) mr2_kload<OPK>(gm, a, Tn, Tn.col0 + p, kb, mr_after(kb, z0, z1));
         // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
         float2 za[4], zb[4];
 #pragma unroll
@@ -368,7 +369,8 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma2048r(PencilArgs<float> a)
             __builtin_amdgcn_global_load_lds(ib + mr2_zoff_u(a.ia, 16 * n) + so.ilane, (mr_lds_ptr)(stg0 + n * 1024),
                                              16, 0, 0);
         }
-        mr2_kload<OPK>(gm, a, T, T.col0 + (w & 7), 4 * (l0 & 15) + 256 * (l0 >> 4) + (w >> 3));
+        const int kb0 = 4 * (l0 & 15) + 256 * (l0 >> 4) + (w >> 3);
+        mr2_kload<OPK>(gm, a, T, T.col0 + (w & 7), kb0, kb0);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
