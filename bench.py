@@ -93,16 +93,17 @@ def alg_bytes_potential(ncell, bc, outputs, eb):
 
 def cpu_baseline(n_target, cin, op, bc=(0, 0, 0)):
     """The oracle (numpy FP64 restatement of the reference, one core) timed on
-    a bounded sample: push+solve+pull at 128^3, scaled to n_target by
+    a bounded sample: push+solve+pull at 256^3 (15-20 s a solve on one core;
+    repeated while the total stays under 10 s), scaled to n_target by
     n^3 log n.  The reference itself cannot be built here (it needs MPI)."""
     from oracle import oracle as O
-    ns = 128
+    ns = min(256, n_target)
     b, h = gaussian_source(ns)
     s = O.OracleSolver()
     s.setup3d((ns,) * 3, tuple(bc), (h,) * 3, 2)
     s.set_return_curl(op == "curl")
     reps, t = 0, 0.0
-    while reps < 3 and t < 20.0:
+    while reps < 3 and t < 10.0:
         t0 = time.perf_counter()
         s.push(*([b] * cin))
         s.solve3d()
