@@ -37,9 +37,13 @@ struct MfmaZR2 {
 };
 
 // Staging layout: block n (1 KB, one direct-load instruction) holds rows
-// 16 n .. 16 n + 15, 64 bytes each; chunk c (columns 2 c, 2 c + 1) of row r
-// sits in slot c ^ ((r >> 2) & 3), which spreads a column read over the banks.
+// 16 n .. 16 n + 15 in 16-byte slots; chunk c (columns 2 c, 2 c + 1) of row r
+// sits in slot 4 ((r + rot) & 15) + (c ^ ((r >> 2) & 3)), rot = (n >> 3) & 1.
+// A wavefront's column access (its 64 lanes read rows 2 j + q of blocks
+// (j >> 3) + 8 g + ...) then covers 16 of the 16-byte positions of a 256-byte
+// bank period, 4 lanes each: the fewest conflicts an 8-byte column allows.
 __device__ __forceinline__ int mr2_swz(int r) { return (r >> 2) & 3; }
+__device__ __forceinline__ unsigned mr2_slot(int r, int c, int rot) { return 4u * ((r + rot) & 15) + (c ^ mr2_swz(r)); }
 
 // 64-bit byte offset of position z0 (a multiple of 16) and the 32-bit one of
 // zr < 16 (plain positions or blocks of 2^sh >= 8 planes)
@@ -49,9 +53,10 @@ __device__ __forceinline__ long mr2_zoff_u(const Addr& d, int z0)
                             : ((long)(z0 >> d.pos_sh) * d.pos_hi + (long)(z0 & ((1 << d.pos_sh) - 1)) * d.pos_lo) * 8;
 }
 
-// Per-lane offsets of the slices (lane l: row r = l >> 2 of a block, LDS
-// slot l & 3): the fetch loads the chunk that belongs in the lane's slot, the
-// store writes chunk l & 3 from its slot.
+// Per-lane offsets of the slices of wavefront w (its blocks n = w + 16 k all
+// have rot = (w >> 3) & 1): the fetch lands lane l in slot l, so the lane
+// loads the row and chunk that slot holds; the store writes chunk l & 3 of
+// row l >> 2 from that chunk's slot.
 struct Mr2SliceOffs {
     unsigned lds;      // LDS offset of the stored chunk within a block
     unsigned ilane;    // fetch: byte offset of the chunk the lane loads, within 16 rows
@@ -59,14 +64,17 @@ struct Mr2SliceOffs {
     int c;             // store: chunk index (columns 2 c, 2 c + 1)
 };
 
-__device__ __forceinline__ Mr2SliceOffs mr2_slice_offs(const PencilArgs<float>& a, int l)
+__device__ __forceinline__ Mr2SliceOffs mr2_slice_offs(const PencilArgs<float>& a, int w, int l)
 {
     Mr2SliceOffs o;
-    const int r = l >> 2, s = l & 3;
-    o.c = s;
-    o.lds = (unsigned)(r * 64 + ((s ^ mr2_swz(r)) << 4));
-    o.ilane = mr_zoff_l(a.ia, r) + 16u * (unsigned)(s ^ mr2_swz(r));
-    o.olane = mr_zoff_l(a.oa, r) + 16u * (unsigned)s;
+    const int rot = (w >> 3) & 1;
+    const int r = l >> 2, c = l & 3;                   // store: row, chunk
+    const int fr = ((l >> 2) - rot) & 15;              // fetch: the row slot l holds
+    const int fc = (l & 3) ^ mr2_swz(fr);              // and its chunk
+    o.c = c;
+    o.lds = 16u * mr2_slot(r, c, rot);
+    o.ilane = mr_zoff_l(a.ia, fr) + 16u * (unsigned)fc;
+    o.olane = mr_zoff_l(a.oa, r) + 16u * (unsigned)c;
     return o;
 }
 
@@ -137,6 +145,25 @@ __device__ __forceinline__ float mr2_kmul(const PencilArgs<float>& a, const MfKC
 // memory (the slices' stores and fetches) still in flight
 __device__ __forceinline__ void mr2_lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
+// rows 2 j + TN + 32 (4 g + e + 16 h) of a pencil: y = y0 + W2048^-n y1 with
+// this wavefront's own residue TN in mine and the other one's in the column
+template <int TN>
+__device__ __forceinline__ void mr2_combine(unsigned char* __restrict__ cur, unsigned sbt, const float2* __restrict__ tsl,
+                                            const float2 (&mine)[2][4])
+{
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float2* q = reinterpret_cast<float2*>(cur + sbt + 2048 * e + 32768 * h);
+            const float2 t = tsl[(8 * TN + 4 * h + e) * 64];
+            const float2 other = *q;
+            const float2 y0 = TN ? other : mine[h][e];
+            const float2 y1 = TN ? mine[h][e] : other;
+            *q = float2{y0.x + y1.x * t.x + y1.y * t.y, y0.y + y1.y * t.x - y1.x * t.y};
+        }
+}
+
 // One tile T for wavefront w (pencil p = w & 7, residue rs = w >> 3), with the
 // slices of the previous and next tiles over its first half and the next
 // tile's multiplier inputs loaded after the last use of gm; the result of
@@ -153,12 +180,12 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
     const int g = l >> 4, j = l & 15;
     const int p = w & 7, rs = w >> 3, kb = 4 * j + 256 * g + rs;
     // staging offsets of rows z = 2 j + q + 32 (4 g + e + 16 h) (q = 0, 1):
-    // block (j >> 3) + 8 g + 2 e + 32 h, row 2 (j & 7) + q
+    // block (j >> 3) + 8 g + 2 e + 32 h (rot = g & 1), row 2 (j & 7) + q
     unsigned sb[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int r = 2 * (j & 7) + q;
-        sb[q] = (unsigned)(((j >> 3) + 8 * g) * 1024 + r * 64 + (((p >> 1) ^ mr2_swz(r)) << 4) + (p & 1) * 8);
+        sb[q] = (unsigned)(((j >> 3) + 8 * g) * 1024) + 16u * mr2_slot(r, p >> 1, g & 1) + (p & 1) * 8;
     }
     // ---- input x[b + 32 m], b = 2 j + tb, m = 4 g + e + 16 h ----
     float2 x[2][2][4];
@@ -298,8 +325,9 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
             o[tn][0][e] = float2{(er[e] + orr[e]) * fo, -(ei[e] + oi[e]) * fo};
             o[tn][1][e] = float2{(er[e] - orr[e]) * fo, -(ei[e] - oi[e]) * fo};
         }
-        if (rs) {
-            // residue 1 leaves its result in the pencil's column
+        if (tn != rs) {
+            // the half of the rows the other residue's wavefront combines
+            // (residue 1 hands over rows of even z, residue 0 those of odd z)
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
@@ -308,20 +336,10 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
         }
     }
     mr2_lds_barrier();
-    if (!rs) {
-        // y = y0 + W2048^-n y1
-#pragma unroll
-        for (int tn = 0; tn < 2; ++tn)
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    float2* q = reinterpret_cast<float2*>(cur + sb[tn] + 2048 * e + 32768 * h);
-                    const float2 t = tsl[(8 * tn + 4 * h + e) * 64];
-                    const float2 y1 = *q;
-                    *q = float2{o[tn][h][e].x + y1.x * t.x + y1.y * t.y, o[tn][h][e].y + y1.y * t.x - y1.x * t.y};
-                }
-    }
+    // y = y0 + W2048^-n y1 on the rows of z parity rs, the other residue's
+    // half from the column
+    if (rs) mr2_combine<1>(cur, sb[1], tsl, o[1]);
+    else mr2_combine<0>(cur, sb[0], tsl, o[0]);
 }
 
 // Persistent kernel over units of two tiles (slabs ky, My - ky) of 8 columns,
@@ -355,7 +373,7 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma2048r(PencilArgs<float> a)
         const int fo = (f == 0) ? 0 : (f == 1) ? 2 : (f == 2) ? 1 : 3;
         ftw[(tb * 4 + fo) * 64 + ln] = reinterpret_cast<const mf_h8*>(a.mfrag16)[(tb * 64 + ln) * 4 + f];
     }
-    const Mr2SliceOffs so = mr2_slice_offs(a, l0);
+    const Mr2SliceOffs so = mr2_slice_offs(a, w, l0);
     int u = blockIdx.x, m = 0;
     MrUnit U{};
     MrTile T{};
