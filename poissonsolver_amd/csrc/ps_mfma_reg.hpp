@@ -56,14 +56,16 @@ struct MfmaZR {
 
 typedef __attribute__((address_space(3))) void* mr_lds_ptr;
 
-// Byte offset of chunk c (pencils 2c, 2c + 1) of row z in a staging buffer:
-// 1 KB blocks of 8 rows (one direct load instruction each); the chunk slots
-// of row r of block blk are XOR-permuted by r and the block parity, so that a
-// pencil column read at consecutive rows spreads over the LDS banks.
-__device__ __forceinline__ unsigned mr_stg_off(int z, int c)
+// Staging layout: 1 KB blocks n of 8 rows of 128 bytes (one direct load
+// instruction each); row r of block n sits in row slot (r + rot) & 7 with
+// rot = (n >> 1) & 1, and chunk c (pencils 2 c, 2 c + 1) in chunk slot
+// (c ^ r ^ (n & 1)) & 7.  A wavefront's pencil column (rows 2 j + q of blocks
+// (j >> 2) + 16 g + ...) then covers all 16 of the 16-byte positions of a
+// 256-byte bank period with its p & 1 half, 4 lanes each: the fewest
+// conflicts an 8-byte column allows.
+__device__ __forceinline__ unsigned mr_stg_pos(int r, int c, int rot, int par)
 {
-    const int r = z & 7, blk = z >> 3;
-    return (unsigned)(blk * 1024 + r * 128 + ((c ^ r ^ (blk & 1)) << 4));
+    return (unsigned)(128 * ((r + rot) & 7) + 16 * ((c ^ r ^ par) & 7));
 }
 
 // Wavefront-wide maximum of non-negative floats (their bit patterns order as
@@ -296,8 +298,8 @@ __device__ __forceinline__ unsigned mr_zoff_l(const Addr& d, int zr)
 // An opaque copy of the lane offset v that depends on the eight complex
 // values z0, z1: loads addressed through it cannot be scheduled before those
 // values exist, so loads of the next tile's multiplier inputs into the
-// registers of this tile's (once z0, z1, which use them, are formed) need no
-// register copies.
+// registers of this tile's (once z0, z1, the last values formed from them,
+// exist) need no register copies.
 __device__ __forceinline__ int mr_after(int v, const float2 (&z0)[4], const float2 (&z1)[4])
 {
     asm volatile(""
@@ -362,9 +364,9 @@ __device__ __forceinline__ float mr_kmul(const PencilArgs<float>& a, const MfKCo
 
 // Per-lane and per-wavefront offsets of the slices, fixed for the launch:
 // slice k of wavefront w is block n = w + 16 k of a staging buffer (rows 8n ..
-// 8n + 7).  Lane l (row r = l >> 3, c = l & 7) reads chunk c of row r for the
-// store (its LDS slot is c ^ r ^ parity) and fetches the chunk that belongs in
-// slot c (chunk c ^ r ^ parity).
+// 8n + 7).  For the store, lane l reads chunk l & 7 of row l >> 3 from its
+// slot; the fetch lands lane l in slot l, so the lane loads the row and chunk
+// that slot holds.
 struct MrSliceOffs {
     unsigned lds;       // LDS offset of lane l's chunk in block 0 of its wavefront's slices (+ n 1024)
     unsigned ilane;     // fetch: byte offset of the chunk the lane loads (the staging permutation)
@@ -375,9 +377,13 @@ struct MrSliceOffs {
 __device__ __forceinline__ MrSliceOffs mr_slice_offs(const PencilArgs<float>& a, int w, int l)
 {
     MrSliceOffs o;
-    const int r = l >> 3, c = l & 7, par = w & 1;   // block parity n & 1 = w & 1
-    o.lds = (unsigned)(r * 128 + ((c ^ r ^ par) << 4));
-    o.ilane = mr_zoff_l(a.ia, r) + 16u * (unsigned)(c ^ r ^ par);
+    // blocks n = w + 16 k: n & 1 = w & 1, (n >> 1) & 1 = (w >> 1) & 1
+    const int par = w & 1, rot = (w >> 1) & 1;
+    const int r = l >> 3, c = l & 7;                        // store: row, chunk
+    const int fr = ((l >> 3) - rot) & 7;                    // fetch: the row slot l holds
+    const int fc = ((l & 7) ^ fr ^ par) & 7;                // and its chunk
+    o.lds = mr_stg_pos(r, c, rot, par);
+    o.ilane = mr_zoff_l(a.ia, fr) + 16u * (unsigned)fc;
     o.olane = mr_zoff_l(a.oa, r) + 16u * (unsigned)c;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -430,12 +436,13 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
     const int p = w, kb = 2 * j + 128 * g;
     mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 0);
     // staging offsets of rows z = 2 j + q + 32 (4 g + e) (q = 0, 1; 128 g + 32 e
-    // is a whole number of blocks): z & 7 = 2 (j & 3) + q, block parity (j >> 2) & 1
+    // is a whole number of blocks): block (j >> 2) + 16 g + 4 e, row
+    // 2 (j & 3) + q, n & 1 = (j >> 2) & 1, (n >> 1) & 1 = (j >> 3) & 1
     unsigned sb[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
-        const int r = 2 * (j & 3) + q, par = (j >> 2) & 1;
-        sb[q] = (unsigned)(((j >> 2) + 16 * g) * 1024 + r * 128 + ((((p >> 1) ^ r ^ par) & 7) << 4) + (p & 1) * 8);
+        const int r = 2 * (j & 3) + q;
+        sb[q] = (unsigned)(((j >> 2) + 16 * g) * 1024) + mr_stg_pos(r, p >> 1, (j >> 3) & 1, (j >> 2) & 1) + (p & 1) * 8;
     }
     // ---- input x[b + 32 m], b = 2 j + tb, m = 4 g + e (m >= 16 is padding) ----
     float2 x[2][4];
@@ -493,31 +500,30 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         mr_mfma_b<true>(er, ei, mr_table(ft, 0, l), B0);
         mr_mfma_b<true>(orr, oi, mr_table(ft, 2, l), B1);
         // X[kz] (kz = k1 + 32 k2, k2 = 4 g + e and + 16) and the k-space operator
-        float2 z0[4], z1[4];
+        // the k-space operator and inverse stage 1's butterflies in one step:
+        // with X = E +- O at rows k2 and k2 + 16 and multipliers m0, m1 there,
+        // A = X0 m0 + X1 m1 = E (m0 + m1) + O (m0 - m1) and
+        // B = X0 m0 - X1 m1 = E (m0 - m1) + O (m0 + m1) (times i for an
+        // imaginary multiplier)
+        float2 za[4], zb[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const float m0 = mr_kmul<OPK>(a, kc, gm, tk, 0, e, kb), m1 = mr_kmul<OPK>(a, kc, gm, tk, 1, e, kb);
-            const float xr0 = er[e] + orr[e], xi0 = ei[e] + oi[e];
-            const float xr1 = er[e] - orr[e], xi1 = ei[e] - oi[e];
+            const float mp = m0 + m1, mm = m0 - m1;
+            const float ar = er[e] * mp + orr[e] * mm, ai = ei[e] * mp + oi[e] * mm;
+            const float br = er[e] * mm + orr[e] * mp, bi = ei[e] * mm + oi[e] * mp;
             if constexpr (mimag) {
-                // i k_d Gk: purely imaginary
-                z0[e] = float2{-xi0 * m0, xr0 * m0};
-                z1[e] = float2{-xi1 * m1, xr1 * m1};
+                za[e] = float2{-ai, ar};
+                zb[e] = float2{-bi, br};
             } else {
-                z0[e] = float2{xr0 * m0, xi0 * m0};
-                z1[e] = float2{xr1 * m1, xi1 * m1};
+                za[e] = float2{ar, ai};
+                zb[e] = float2{br, bi};
             }
         }
         if (tk == 0) mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 2);
         // the next tile's multipliers, once this tile's are used
-        if (tk == 1 && load_g) mr_kload<OPK>(gm, a, Tn, Tn.col0 + p, j, g, mr_after(kb, z0, z1));
+        if (tk == 1 && load_g) mr_kload<OPK>(gm, a, Tn, Tn.col0 + p, j, g, mr_after(kb, za, zb));
         // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
-        float2 za[4], zb[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            za[e] = float2{z0[e].x + z1[e].x, z0[e].y + z1[e].y};
-            zb[e] = float2{z0[e].x - z1[e].x, z0[e].y - z1[e].y};
-        }
         mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
             const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);

@@ -262,29 +262,29 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
         mf_f4 er, ei, orr, oi;
         mr_mfma_b<true>(er, ei, mr_table(ft, 0, l), B0);
         mr_mfma_b<true>(orr, oi, mr_table(ft, 2, l), B1);
-        float2 z0[4], z1[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const float m0 = mr2_kmul<OPK>(a, kc, gm, tk, 0, e, kb), m1 = mr2_kmul<OPK>(a, kc, gm, tk, 1, e, kb);
-            const float xr0 = er[e] + orr[e], xi0 = ei[e] + oi[e];
-            const float xr1 = er[e] - orr[e], xi1 = ei[e] - oi[e];
-            if constexpr (mimag) {
-                z0[e] = float2{-xi0 * m0, xr0 * m0};
-                z1[e] = float2{-xi1 * m1, xr1 * m1};
-            } else {
-                z0[e] = float2{xr0 * m0, xi0 * m0};
-                z1[e] = float2{xr1 * m1, xi1 * m1};
-            }
-        }
-        if (tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 2);
-        if (tk == 1 && load_g) mr2_kload<OPK>(gm, a, Tn, Tn.col0 + p, kb, mr_after(kb, z0, z1));
-        // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
+        // the k-space operator and inverse stage 1's butterflies in one step:
+        // with X = E +- O at rows k2 and k2 + 16 and multipliers m0, m1 there,
+        // A = X0 m0 + X1 m1 = E (m0 + m1) + O (m0 - m1) and
+        // B = X0 m0 - X1 m1 = E (m0 - m1) + O (m0 + m1) (times i for an
+        // imaginary multiplier)
         float2 za[4], zb[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            za[e] = float2{z0[e].x + z1[e].x, z0[e].y + z1[e].y};
-            zb[e] = float2{z0[e].x - z1[e].x, z0[e].y - z1[e].y};
+            const float m0 = mr2_kmul<OPK>(a, kc, gm, tk, 0, e, kb), m1 = mr2_kmul<OPK>(a, kc, gm, tk, 1, e, kb);
+            const float mp = m0 + m1, mm = m0 - m1;
+            const float ar = er[e] * mp + orr[e] * mm, ai = ei[e] * mp + oi[e] * mm;
+            const float br = er[e] * mm + orr[e] * mp, bi = ei[e] * mm + oi[e] * mp;
+            if constexpr (mimag) {
+                za[e] = float2{-ai, ar};
+                zb[e] = float2{-bi, br};
+            } else {
+                za[e] = float2{ar, ai};
+                zb[e] = float2{br, bi};
+            }
         }
+        if (tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 2);
+        if (tk == 1 && load_g) mr2_kload<OPK>(gm, a, Tn, Tn.col0 + p, kb, mr_after(kb, za, zb));
+        // ---- inverse stage 1 (conj Z x table), row k1 = 2 j + tk ----
         mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
             const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
