@@ -352,13 +352,27 @@ __device__ __forceinline__ void mr_kload(mr_f2a (&gr)[2][4], const PencilArgs<fl
     }
 }
 
-// multiplier of point (tt, h, e) from gr (kc: the tile's coefficients)
+// multiplier of point (tt, h, e) from gr (kc: the tile's coefficients).  The
+// point's wavenumber index is kz = kb + tt + 32 e + 512 h, taken as kz - Mz
+// from Mz / 2 on (Mz = 1024: exactly the points of h = 1), so with kbf = kb
+// as a float it is kbf plus a compile-time integer: (float)(kz or kz - Mz)
+// in one addition, exact, as the gradient and Laplacian forms of the other
+// kernels compute it.  The gradient has no derivative at the Nyquist point
+// kz = Mz / 2 (h = 1, tt = e = 0 on the lane with kb = 0).
 template <int OPK>
-__device__ __forceinline__ float mr_kmul(const PencilArgs<float>& a, const MfKCoef& kc, const mr_f2a (&gr)[2][4], int tt,
-                                         int h, int e, int kb)
+__device__ __forceinline__ float mr_kmul(const MfKCoef& kc, const mr_f2a (&gr)[2][4], int tt, int h, int e, int kb,
+                                         float kbf)
 {
     const float v = gr[h][e][h ? 1 - tt : tt];
-    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL) return mf_kval<OPK>(a, kc, kb + tt + 32 * e + 512 * h, v);
+    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL) {
+        const float kzw = (kbf + (float)(tt + 32 * e - 512 * h)) * kc.kst;
+        if constexpr (OPK == OP_GRAD) {
+            const float kzh = (h == 1 && tt == 0 && e == 0 && kb == 0) ? 0.0f : kzw;
+            return (kc.c0 + kc.c1 * kzh) * v;
+        } else {
+            return (kc.c0 - kzw * kzw) * v;
+        }
+    }
     return v;
 }
 
@@ -486,6 +500,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
     MfKCoef kc{};
     if constexpr (OPK == OP_GRAD || OPK == OP_LAPL)
         kc = mf_kcoef<OPK>(a, tile_k(a, T.col0 + p, T.slab, T.col0 + p < a.ncol), T.comp);
+    const float kbf = (float)kb;
 #pragma unroll
     for (int tk = 0; tk < 2; ++tk) {
         // ---- stage 2 (table x data) for column k1 = 2 j + tk ----
@@ -508,7 +523,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         float2 za[4], zb[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const float m0 = mr_kmul<OPK>(a, kc, gm, tk, 0, e, kb), m1 = mr_kmul<OPK>(a, kc, gm, tk, 1, e, kb);
+            const float m0 = mr_kmul<OPK>(kc, gm, tk, 0, e, kb, kbf), m1 = mr_kmul<OPK>(kc, gm, tk, 1, e, kb, kbf);
             const float mp = m0 + m1, mm = m0 - m1;
             const float ar = er[e] * mp + orr[e] * mm, ai = ei[e] * mp + oi[e] * mm;
             const float br = er[e] * mm + orr[e] * mp, bi = ei[e] * mm + oi[e] * mp;

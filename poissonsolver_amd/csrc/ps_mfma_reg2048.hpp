@@ -132,12 +132,24 @@ __device__ __forceinline__ void mr2_kload(float (&gr)[2][4][2], const PencilArgs
     }
 }
 
+// multiplier of point (tt, h, e): kz = kb + 2 tt + 64 e + 1024 h, taken as
+// kz - Mz from Mz / 2 = 1024 on (the points of h = 1), formed from kbf = kb
+// as for the kernel of 1024; the gradient's Nyquist point is h = 1,
+// tt = e = 0 on the lane with kb = 0
 template <int OPK>
-__device__ __forceinline__ float mr2_kmul(const PencilArgs<float>& a, const MfKCoef& kc, const float (&gr)[2][4][2],
-                                          int tt, int h, int e, int kb)
+__device__ __forceinline__ float mr2_kmul(const MfKCoef& kc, const float (&gr)[2][4][2], int tt, int h, int e, int kb,
+                                          float kbf)
 {
     const float v = gr[h][e][tt];
-    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL) return mf_kval<OPK>(a, kc, kb + 2 * tt + 64 * e + 1024 * h, v);
+    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL) {
+        const float kzw = (kbf + (float)(2 * tt + 64 * e - 1024 * h)) * kc.kst;
+        if constexpr (OPK == OP_GRAD) {
+            const float kzh = (h == 1 && tt == 0 && e == 0 && kb == 0) ? 0.0f : kzw;
+            return (kc.c0 + kc.c1 * kzh) * v;
+        } else {
+            return (kc.c0 - kzw * kzw) * v;
+        }
+    }
     return v;
 }
 
@@ -249,6 +261,7 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
     MfKCoef kc{};
     if constexpr (OPK == OP_GRAD || OPK == OP_LAPL)
         kc = mf_kcoef<OPK>(a, tile_k(a, T.col0 + p, T.slab, T.col0 + p < a.ncol), T.comp);
+    const float kbf = (float)kb;
 #pragma unroll
     for (int tk = 0; tk < 2; ++tk) {
         // ---- stage 2 (table x data) for column k1 = 2 j + tk ----
@@ -270,7 +283,7 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
         float2 za[4], zb[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const float m0 = mr2_kmul<OPK>(a, kc, gm, tk, 0, e, kb), m1 = mr2_kmul<OPK>(a, kc, gm, tk, 1, e, kb);
+            const float m0 = mr2_kmul<OPK>(kc, gm, tk, 0, e, kb, kbf), m1 = mr2_kmul<OPK>(kc, gm, tk, 1, e, kb, kbf);
             const float mp = m0 + m1, mm = m0 - m1;
             const float ar = er[e] * mp + orr[e] * mm, ai = ei[e] * mp + oi[e] * mm;
             const float br = er[e] * mm + orr[e] * mp, bi = ei[e] * mm + oi[e] * mp;
