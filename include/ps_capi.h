@@ -245,9 +245,9 @@ typedef struct ps_tuning {
                          transforms of 512 or shorter, else B (PS_TUNE_HALFB_Z, -1) */
     int halfb_y;      /* y tile-width divisor 0/1 -> B, B/2 (PS_TUNE_HALFB_Y, 0) */
     int mfma;         /* PS_MIXED: matrix-core (FP16 hi/lo DFT) kernels where the length
-                         has them: z transforms of 1024: 1 the persistent register-pencil
-                         kernel (one pencil per wavefront, 16-column tiles), 2 one 8-pencil
-                         tile per workgroup; of 2048: 1 the register-pencil kernel (two
+                         has them: z transforms of 512 and 1024: 1 the persistent
+                         register-pencil kernel (one pencil per wavefront, 16-column tiles),
+                         2 one tile per workgroup; of 2048: 1 the register-pencil kernel (two
                          wavefronts per pencil, 8-column tiles), 2 eight pencils per
                          workgroup, 3 four; 0 the FP32 kernels (PS_TUNE_MFMA, 1) */
     int t2d;          /* 2D contexts whose fused y pass holds one or two columns per workgroup

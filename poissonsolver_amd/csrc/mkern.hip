@@ -3,6 +3,7 @@
 #include "ps_mfma_launch.hpp"
 #include "ps_mfma_reg.hpp"
 #include "ps_mfma_reg2048.hpp"
+#include "ps_mfma_reg512.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -89,8 +90,25 @@ void zmfma_fragments16(unsigned short* out)
     zmfma_fragments16k(out, 0);
 }
 
-template <int OPK> static hipError_t launch_zmfma512_op(PencilArgs<float> a, hipStream_t s)
+static bool zmfma_reg_ok(const PencilArgs<float>& a);
+static int device_cus();
+
+template <int OPK> static hipError_t launch_zmfma512_op(PencilArgs<float> a, int variant, hipStream_t s)
 {
+    if (variant == 1 && zmfma_reg_ok(a)) {
+        // register-pencil kernel: 16-column tiles of 256 rows, one workgroup
+        // of 16 wavefronts per CU at a time, four per CU over the launch
+        const size_t shm = MfmaZR5::lds_bytes();
+        hipError_t e = allow_lds(k_zmfma512r<OPK>, shm);
+        if (e != hipSuccess) return e;
+        a.ncoltile = (a.ncol + MfmaZR5::P - 1) / MfmaZR5::P;
+        if ((long)a.ncoltile * a.nslab * a.ncomp >= (1L << 31)) return hipErrorInvalidValue;
+        const long nunit = (long)a.ncoltile * ((a.nslab + 1) / 2) * a.ncomp;
+        const long ncu = std::max(device_cus(), 8);
+        const long nwg = std::min<long>(nunit, 4 * ncu);
+        hipLaunchKernelGGL((k_zmfma512r<OPK>), dim3((unsigned)nwg), dim3(64 * MfmaZR5::NW), shm, s, a);
+        return hipGetLastError();
+    }
     using M = MfmaZ512;
     const size_t shm = M::lds_bytes();
     hipError_t e = allow_lds(k_zmfma512<OPK>, shm);
@@ -212,11 +230,11 @@ hipError_t launch_zmfma(const PencilArgs<float>& a, int variant, hipStream_t s)
     if (a.plan.L == 512) {
         if (a.n_in != 256 || a.n_out != 256 || !a.mfrag16) return hipErrorInvalidValue;
         switch (a.op) {
-        case OP_NONE: return launch_zmfma512_op<OP_NONE>(a, s);
-        case OP_PLAIN: return launch_zmfma512_op<OP_PLAIN>(a, s);
-        case OP_GRAD: return launch_zmfma512_op<OP_GRAD>(a, s);
-        case OP_RHSF: return launch_zmfma512_op<OP_RHSF>(a, s);
-        case OP_LAPL: return launch_zmfma512_op<OP_LAPL>(a, s);
+        case OP_NONE: return launch_zmfma512_op<OP_NONE>(a, variant, s);
+        case OP_PLAIN: return launch_zmfma512_op<OP_PLAIN>(a, variant, s);
+        case OP_GRAD: return launch_zmfma512_op<OP_GRAD>(a, variant, s);
+        case OP_RHSF: return launch_zmfma512_op<OP_RHSF>(a, variant, s);
+        case OP_LAPL: return launch_zmfma512_op<OP_LAPL>(a, variant, s);
         default: return hipErrorInvalidValue;
         }
     }

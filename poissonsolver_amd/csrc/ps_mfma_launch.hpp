@@ -13,15 +13,16 @@ bool zmfma_supported(int L, int half, int op);
 
 // fused z pass on the matrix cores (FP16 hi/lo split DFT stages, FP32
 // accumulation); `a` as for the FP32 fused z pass, one-component operators.
-// variant (tuning field mfma): transforms of 1024 and 2048 run the persistent
-// register-pencil kernels for 1 (when the arguments allow it: kx-contiguous
-// rows, the octant in the [ky][kx][kz] layout); for 2, transforms of 1024 run
-// the one-tile-per-workgroup kernel and transforms of 2048 8-pencil
-// workgroups; for 3, transforms of 2048 run 4-pencil workgroups
+// variant (tuning field mfma): transforms of 512, 1024 and 2048 run the
+// persistent register-pencil kernels for 1 (when the arguments allow it:
+// kx-contiguous rows, the octant in the [ky][kx][kz] layout); for 2,
+// transforms of 512 and 1024 run the one-tile-per-workgroup kernels and
+// transforms of 2048 8-pencil workgroups; for 3, transforms of 512 and 1024
+// the one-tile kernels and of 2048 4-pencil workgroups
 hipError_t launch_zmfma(const PencilArgs<float>& a, int variant, hipStream_t s);
 // whether a context's matrix-core z pass of length L with this tuning
 // variant reads the Gk octant in the [ky][kx][kz] layout
-inline bool zmfma_reg_layout(int L, int variant) { return (L == 1024 || L == 2048) && variant == 1; }
+inline bool zmfma_reg_layout(int L, int variant) { return (L == 512 || L == 1024 || L == 2048) && variant == 1; }
 
 // three-component operators (div, curl) of the fused z pass on the matrix
 // cores: inputs are components 0..2 of `a.in`, outputs components of `a.out`

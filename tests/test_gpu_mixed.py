@@ -48,8 +48,8 @@ def _compare(n, bc, dx, reg, op, b, want_rhs=False, tol=MIXED_TOL):
     return err
 
 
-# matrix-core kernel forms (PS_TUNE_MFMA) of transforms of 1024: 1 the
-# register-pencil kernel, 2 one tile per workgroup
+# matrix-core kernel forms (PS_TUNE_MFMA) of transforms of 512 and 1024: 1
+# the register-pencil kernel, 2 one tile per workgroup
 FORMS = ["1", "2"]
 # of transforms of 2048: 1 the register-pencil kernel, 2 eight-pencil and
 # 3 four-pencil workgroups
@@ -195,11 +195,14 @@ CASES512 = [
 
 @pytest.mark.parametrize("case", CASES512, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
 @pytest.mark.parametrize("reg", [0, 2, 10])
-def test_mixed_512_plain_grad_filtered_rhs(case, reg):
+@pytest.mark.parametrize("form", FORMS)
+def test_mixed_512_plain_grad_filtered_rhs(case, reg, form, monkeypatch):
     """nz = 256 unbounded: z transforms of 512 = 32 x 16 on the matrix cores
-    (plain, component-wise grad, filtered rhs)."""
+    (plain, component-wise grad, filtered rhs), on the register-pencil (1) and
+    the one-tile (2) kernel; rel-L2 <= 1e-6 against the oracle."""
+    monkeypatch.setenv("PS_TUNE_MFMA", form)
     n, bc, dx = case
-    announce("mixed 512", n, bc, reg)
+    announce("mixed 512", n, bc, reg, "form", form)
     assert _variant(n, bc, dx, reg) == "matrix-core"
     b = random_rhs(n, 71 + reg)
     _compare(n, bc, dx, reg, "plain", [b[0]], want_rhs=True)
