@@ -253,6 +253,10 @@ typedef struct ps_tuning {
     int t2d;          /* 2D contexts whose fused y pass holds one or two columns per workgroup
                          (long pencils): the x pass output is transposed so that y pencils are
                          contiguous, and grad / curl transform their scalar once (PS_TUNE_T2D, 1) */
+    int zfan;         /* PS_MIXED potential form (ps_solve3d_potential): 1 runs its z outputs (A,
+                         d/dz A, Laplacian) from one forward transform of the scalar where the
+                         matrix-core kernel has that form (z transforms of 512, mfma 1); 0 one
+                         z launch per output (PS_TUNE_ZFAN, 1) */
 } ps_tuning;
 /* the defaults (with PS_TUNE_* overrides applied) */
 int ps_tuning_default(ps_tuning* t);

@@ -21,7 +21,7 @@ _c_long_p = ctypes.POINTER(ctypes.c_long)
 
 TUNING_FIELDS = ("zblk", "tile_x", "tile_y", "tile_z", "radix_e", "lds_kb", "xcd", "specialised",
                  "specialised3", "twreg_z", "twreg_y", "occ4_z", "zwave", "halfb_z", "halfb_y", "mfma",
-                 "t2d")
+                 "t2d", "zfan")
 
 
 class ps_tuning(ctypes.Structure):

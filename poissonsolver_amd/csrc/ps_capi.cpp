@@ -74,6 +74,7 @@ Tuning tuning_default()
     t.halfb_y = env_int("PS_TUNE_HALFB_Y", 0);
     t.mfma = env_int("PS_TUNE_MFMA", 1);
     t.t2d = env_int("PS_TUNE_T2D", 1);
+    t.zfan = env_int("PS_TUNE_ZFAN", 1);
     return t;
 }
 
@@ -644,7 +645,23 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         return PS_OK;
     };
     char* A2o = static_cast<char*>(A2) + (size_t)c->A2c * cbytes;
-    if (potfan) {
+    bool zfan_done = false;
+    if constexpr (sizeof(T) == 4) {
+        // the z outputs from one forward transform where the kernel has it
+        // (zslot 0 .. nzq - 1 in mask order)
+        const int zmask = (zslot[0] >= 0 ? 1 : 0) | (zslot[3] >= 0 ? 2 : 0) | (zslot[4] >= 0 ? 4 : 0);
+        if (potfan && c->zmfma && nzq > 1 && c->tune.zfan) {
+            PencilArgs<T> zq = za;
+            zq.ncomp = 1;
+            zq.out = A2o;
+            if (zmfma_fan_ok(zq, zmask, c->tune.mfma)) {
+                CK(launch_zmfma_fan(zq, zmask, s));
+                zfan_done = true;
+            }
+        }
+    }
+    if (zfan_done) {
+    } else if (potfan) {
         for (int qn : {0, 3, 4}) {
             if (zslot[qn] < 0) continue;
             PencilArgs<T> zq = za;
@@ -1916,7 +1933,8 @@ int ps_set_tuning(ps_ctx* c, const ps_tuning* t)
     }
     if (t->zblk < -2 || t->zblk > 10 || t->tile_x < 0 || t->tile_x > 64 || t->tile_y < 0 || t->tile_y > 64 ||
         t->tile_z < 0 || t->tile_z > 64 || t->radix_e < 0 || t->radix_e > 70 || t->lds_kb < 16 || t->lds_kb > 160 ||
-        t->halfb_z < -1 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1 || t->mfma < 0 || t->mfma > 3)
+        t->halfb_z < -1 || t->halfb_z > 2 || t->halfb_y < 0 || t->halfb_y > 1 || t->mfma < 0 || t->mfma > 3 ||
+        t->zfan < 0 || t->zfan > 1)
         return fail(c, PS_EINVAL, "set_tuning: field out of range");
     c->tune_req = *t;
     c->tune_set = true;

@@ -24,6 +24,14 @@ hipError_t launch_zmfma(const PencilArgs<float>& a, int variant, hipStream_t s);
 // variant reads the Gk octant in the [ky][kx][kz] layout
 inline bool zmfma_reg_layout(int L, int variant) { return (L == 512 || L == 1024 || L == 2048) && variant == 1; }
 
+// several z-pass outputs of one scalar from one forward transform (the
+// potential form): mask bit 0 A (Gk), bit 1 d/dz A (i kzh Gk), bit 2 the
+// Laplacian (-|k|^2 Gk); output o (the o-th set bit) is written to component
+// o of a.out (stride a.oa.comp), the input is component 0 of a.in.  At least
+// two outputs, transforms of 512 on the register-pencil kernel (variant 1)
+bool zmfma_fan_ok(const PencilArgs<float>& a, int mask, int variant);
+hipError_t launch_zmfma_fan(const PencilArgs<float>& a, int mask, hipStream_t s);
+
 // three-component operators (div, curl) of the fused z pass on the matrix
 // cores: inputs are components 0..2 of `a.in`, outputs components of `a.out`
 bool zmfma3_supported(int L, int half, int op);

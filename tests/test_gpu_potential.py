@@ -138,3 +138,35 @@ def test_potential_long_z(prec):
     assert O.rel_l2(out["A"], lo_a[0]) <= 2e-6
     assert O.rel_l2([out["dx"], out["dy"], out["dz"]], lo_g) <= 2e-6
     assert O.rel_l2(out["lap"], lo_l[0]) <= 2e-6
+
+
+FAN_SUBSETS = [("A", "dz"), ("A", "lap"), ("dz", "lap"), ("dx", "dz"), ("dy", "lap"), ("A", "dx", "dy", "dz"),
+               NAMES]
+FAN_CASES = [((18, 10, 256), (0, 0, 0), (0.1, 0.12, 1 / 128), 2),
+             ((16, 12, 256), (1, 1, 0), (1 / 16, 1 / 12, 1 / 128), 6)]
+
+
+@pytest.mark.parametrize("case", FAN_CASES, ids=lambda c: "%dx%dx%d-%d%d%d" % (c[0] + c[1]))
+@pytest.mark.parametrize("zblk", [-1, 0, 3], ids=lambda z: "zblk%d" % z)
+def test_potential_fanout_512(case, zblk):
+    """z transforms of 512 (nz = 256 unbounded), mixed form: the z outputs
+    (A, d/dz A, Laplacian) from one forward transform of the scalar (tuning
+    zfan 1: every two- and three-output set, ragged column tiles, odd and
+    even slab counts) against the oracle (rel-L2 <= 1e-6) and against one z
+    launch per output (zfan 0; the same arithmetic, rel-L2 <= 1e-7)."""
+    n, bc, dx, reg = case
+    b = random_rhs(n, 81, 1)[0]
+    lo_a, _ = O.solve(n, bc, dx, reg, [b])
+    lo_g, _ = O.solve(n, bc, dx, reg, [b, b, b], op="grad")
+    lo_l, _ = O.solve(n, bc, dx, reg, [b], op="lap")
+    ref = {"A": lo_a[0], "dx": lo_g[0], "dy": lo_g[1], "dz": lo_g[2], "lap": lo_l[0]}
+    for outs in FAN_SUBSETS:
+        announce("potential fan-out 512", n, bc, zblk, outs)
+        f1, _ = _potential("mixed", n, bc, dx, reg, b, outputs=outs, tuning={"zblk": zblk, "zfan": 1})
+        f0, _ = _potential("mixed", n, bc, dx, reg, b, outputs=outs, tuning={"zblk": zblk, "zfan": 0})
+        for k in NAMES:
+            if k in outs:
+                assert O.rel_l2(f1[k], ref[k]) <= TOL["mixed"], (outs, k)
+                assert O.rel_l2(f1[k], f0[k]) <= 1e-7, (outs, k)
+            else:
+                assert np.isnan(f1[k]).all(), (outs, k)
