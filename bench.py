@@ -77,16 +77,23 @@ def alg_bytes_grid(ncell, bc, cin, cout, eb):
     return dict(x=P1, y=P2, z=P3, y_inv=P4, x_inv=P5, solve=P1 + P2 + P3 + P4 + P5)
 
 
-def alg_bytes_potential(ncell, bc, outputs, eb):
+def alg_bytes_potential(ncell, bc, outputs, eb, zlaunches=None):
     """Byte model of the potential form (ps_solve3d_potential): one scalar
     through the x and y passes; one z transform per distinct z multiplier (A,
     which d/dx and d/dy share, d/dz and the Laplacian), each reading the one
-    spectrum and Gk once and writing its own; the y^-1 and x^-1 passes once
-    per output (d/dx and d/dy read A's z output, times i k on load)."""
+    spectrum and Gk once and writing its own, or, when the z outputs run from
+    one forward transform (zlaunches 1), the spectrum and Gk read once and
+    every output written; the y^-1 and x^-1 passes once per output (d/dx and
+    d/dy read A's z output, times i k on load)."""
     one = alg_bytes_grid(ncell, bc, 1, 1, eb)
     back = alg_bytes_grid(ncell, bc, 1, len(outputs), eb)
     nzq = int(any(k in outputs for k in ("A", "dx", "dy"))) + int("dz" in outputs) + int("lap" in outputs)
-    d = dict(x=one["x"], y=one["y"], z=nzq * one["z"], y_inv=back["y_inv"], x_inv=back["x_inv"])
+    if zlaunches == 1 and nzq > 1:
+        two = alg_bytes_grid(ncell, bc, 1, 2, eb)
+        z = one["z"] + (nzq - 1) * (two["z"] - one["z"])
+    else:
+        z = nzq * one["z"]
+    d = dict(x=one["x"], y=one["y"], z=z, y_inv=back["y_inv"], x_inv=back["x_inv"])
     d["solve"] = sum(d.values())
     return d
 
@@ -410,7 +417,7 @@ def bench_config(a):
                     per["potential" if grp == pot else "potential_agrad"] = {
                         "ms": prof["solve"], "outputs": list(grp),
                         "pass_ms": {k: prof[k] for k in ("x", "y", "z", "y_inv", "x_inv")},
-                        "z_kernel": s.zpass_variant()}
+                        "z_kernel": s.zpass_variant(), "z_launches": s.potential_zlaunches(grp)}
                 else:
                     s.solve3d_potential(grp)
                 if s.last_status:
@@ -463,19 +470,21 @@ def bench_config(a):
         agrad = results[line_prec].get("potential_agrad")
         # the same four outputs (A, grad A) as the component-wise solves
         out["speedup_potential_vs_componentwise"] = comp_ms / (agrad["ms"] if agrad else step_ms)
-        B = alg_bytes_potential(ncell, bc, pot, eb)
+        zl = results[line_prec]["potential"].get("z_launches")
+        nzq_pot = int(any(k in pot for k in ("A", "dx", "dy"))) + int("dz" in pot) + int("lap" in pot)
+        B = alg_bytes_potential(ncell, bc, pot, eb, zl)
         Bc = sum(alg_bytes_grid(ncell, bc, ci, co, eb)["solve"] for ci, co in cin_out)
         pp = results[line_prec]["potential"]["pass_ms"]
         out["alg_bytes_per_step"] = B["solve"]
         if agrad:
-            out["alg_bytes_agrad"] = alg_bytes_potential(ncell, bc, agrad["outputs"], eb)["solve"]
+            out["alg_bytes_agrad"] = alg_bytes_potential(ncell, bc, agrad["outputs"], eb, agrad.get("z_launches"))["solve"]
         out["alg_bytes_componentwise"] = Bc
         out["step_hbm_gbs"] = B["solve"] / (step_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": B["z"] / (pp["z"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": B["z"] / (pp["z"] * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
                            "kernel": "z pass of the potential form (%s kernel: one spectrum in, %d quantities "
-                                     "out through %d z transforms)"
-                                     % (results[line_prec]["potential"]["z_kernel"], len(pot), B["z"] // alg_bytes_grid(ncell, bc, 1, 1, eb)["z"]),
+                                     "out through %d inverse z transforms in %d launch(es))"
+                                     % (results[line_prec]["potential"]["z_kernel"], len(pot), nzq_pot, zl),
                            "alg_bytes_per_launch": B["z"], "avg_ms": pp["z"]}
     if len(precs) == 2:
         f32 = sum(v["ms"] for k, v in results["fp32"].items() if pot is None or k == "potential")

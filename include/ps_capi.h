@@ -155,9 +155,11 @@ int ps_eval_special(const double* x, double* y, long n, int which);
      PS_POT_A    A = G * B                  (the plain solve of X)
      PS_POT_DX.. dA/dx, dA/dy, dA/dz         (grad of X, as solve3d's grad)
      PS_POT_LAP  Laplacian of A, -|k|^2 Ghat  (div of grad A)
-   with the context's regularisation.  The z pass transforms A, dA/dz and the
-   Laplacian (one z transform each); dA/dx and dA/dy are i kx and i ky times
-   A's z-pass output, applied as their y^-1 pass loads it.  Results stay in
+   with the context's regularisation.  The z pass runs one inverse z
+   transform each for A, dA/dz and the Laplacian (in the mixed form at
+   Mz = 512 and 2048, all of them in one launch from one forward transform;
+   tuning field zfan); dA/dx and dA/dy are i kx and i ky times A's z-pass
+   output, applied as their y^-1 pass loads it.  Results stay in
    the context until the next potential solve.  The held rhs is left as it
    was.  3D contexts. */
 enum { PS_POT_A = 1, PS_POT_DX = 2, PS_POT_DY = 4, PS_POT_DZ = 8, PS_POT_LAP = 16 };
@@ -166,6 +168,11 @@ int ps_solve3d_potential(ps_ctx* ctx, int outputs);
 int ps_pull_potential(ps_ctx* ctx, double* A, double* dAdx, double* dAdy, double* dAdz, double* lapA);
 /* device arrays of the solve precision's real type */
 int ps_pull_potential_device(ps_ctx* ctx, void* A, void* dAdx, void* dAdy, void* dAdz, void* lapA);
+/* number of z-pass launches a potential solve of `outputs` makes on this
+   context: 1 when its z outputs run from one forward transform, else one per
+   distinct z multiplier (A, dA/dz, Laplacian); -PS_ESTATE before setup3d,
+   -PS_EINVAL for a 2D context or a mask outside 1..31 */
+int ps_potential_zlaunches(const ps_ctx* ctx, int outputs);
 /* `iters` potential solves timed per pass as ps_profile_solve */
 int ps_profile_potential(ps_ctx* ctx, int outputs, int iters, double ms[6]);
 
@@ -255,8 +262,8 @@ typedef struct ps_tuning {
                          contiguous, and grad / curl transform their scalar once (PS_TUNE_T2D, 1) */
     int zfan;         /* PS_MIXED potential form (ps_solve3d_potential): 1 runs its z outputs (A,
                          d/dz A, Laplacian) from one forward transform of the scalar where the
-                         matrix-core kernel has that form (z transforms of 512, mfma 1); 0 one
-                         z launch per output (PS_TUNE_ZFAN, 1) */
+                         matrix-core kernel has that form (z transforms of 512 and 2048, mfma 1);
+                         0 one z launch per output (PS_TUNE_ZFAN, 1) */
 } ps_tuning;
 /* the defaults (with PS_TUNE_* overrides applied) */
 int ps_tuning_default(ps_tuning* t);

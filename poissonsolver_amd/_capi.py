@@ -79,6 +79,7 @@ SIGNATURES = {
     "ps_write_vti": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _c_int_p, _c_dbl_p, _c_dbl_p, ctypes.c_int,
                                     ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]),
     "ps_zpass_variant": (ctypes.c_int, [ctypes.c_void_p]),
+    "ps_potential_zlaunches": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ps_dft32_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
     "ps_dft16_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
     "ps_dft16_table": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int, ctypes.c_int]),

@@ -1448,6 +1448,24 @@ int ps_zpass_variant(const ps_ctx* c)
     return c->pz1.stat ? 1 : 0;
 }
 
+int ps_potential_zlaunches(const ps_ctx* c, int outputs)
+{
+    if (!c || !c->ready) return -PS_ESTATE;
+    if (c->dim2 || outputs < 1 || outputs > 31) return -PS_EINVAL;
+    const int zmask = ((outputs & (PS_POT_A | PS_POT_DX | PS_POT_DY)) ? 1 : 0) | ((outputs & PS_POT_DZ) ? 2 : 0) |
+                      ((outputs & PS_POT_LAP) ? 4 : 0);
+    const int nzq = __builtin_popcount((unsigned)zmask);
+    if (c->prec == PS_MIXED && c->zmfma && c->tune.zfan && nzq > 1) {
+        // the fan-out condition of the pipeline, on the context's z arguments
+        PencilArgs<float> za{};
+        fill_zargs<float>(const_cast<ps_ctx*>(c), za, c->pz1, nullptr, nullptr);
+        za.ia.comp = 0;
+        za.ncomp = 1;
+        if (zmfma_fan_ok(za, zmask, c->tune.mfma)) return 1;
+    }
+    return nzq;
+}
+
 int ps_solve3d_potential(ps_ctx* c, int outputs)
 {
     int r = check_ready(c);

@@ -176,24 +176,23 @@ __device__ __forceinline__ void mr2_combine(unsigned char* __restrict__ cur, uns
         }
 }
 
-// One tile T for wavefront w (pencil p = w & 7, residue rs = w >> 3), with the
-// slices of the previous and next tiles over its first half and the next
-// tile's multiplier inputs loaded after the last use of gm; the result of
-// pencil p is left in column p of cur.  Nothing here waits for vector memory.
-template <int OPK>
-__device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
-                                         unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
-                                         const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
-                                         const MrTile& T, const Mr2SliceOffs& so, char* obase, int ncv, bool hasprev,
-                                         const char* ibase, bool more, const MrTile& Tn, bool load_g,
-                                         float (&gm)[2][4][2], int w, int l)
+// Transform front of one tile for wavefront w (pencil p = w & 7, residue
+// rs = w >> 3): the pencil's input rows from cur (then a barrier: every
+// wavefront has read its pencil before any result is written), slice 0,
+// residue 1's W2048^n twist, the pencil's scale, stage 1, slice 1 and the
+// W1024^(b k1) twiddle; returns the rows' staging offsets sb, the input
+// scale's inverse and the twiddled stage-1 result Y.
+__device__ __forceinline__ void mr2_front(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                          unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
+                                          const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
+                                          const Mr2SliceOffs& so, char* obase, int ncv, bool hasprev,
+                                          const char* ibase, bool more, int w, int l, unsigned (&sb)[2], float& inv0,
+                                          mf_f4 (&yr)[2][2], mf_f4 (&yi)[2][2])
 {
-    constexpr bool mimag = mf_imag_multiplier<OPK>();
     const int g = l >> 4, j = l & 15;
-    const int p = w & 7, rs = w >> 3, kb = 4 * j + 256 * g + rs;
+    const int p = w & 7, rs = w >> 3;
     // staging offsets of rows z = 2 j + q + 32 (4 g + e + 16 h) (q = 0, 1):
     // block (j >> 3) + 8 g + 2 e + 32 h (rot = g & 1), row 2 (j & 7) + q
-    unsigned sb[2];
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
         const int r = 2 * (j & 7) + q;
@@ -222,13 +221,12 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
                 for (int e = 0; e < 4; ++e) x[tb][h][e] = cmul(x[tb][h][e], tsl[(8 * tb + 4 * h + e) * 64]);
     }
     // one scale for the pencil: |x[m] +- x[m + 16]| <= 2 max
-    float s0, inv0;
+    float s0;
     {
         const float m0 = mr_absmax8(x[0][0], x[0][1]), m1 = mr_absmax8(x[1][0], x[1][1]);
         mr_scale(2.0f * mr_wave_max(__builtin_fmaxf(m0, m1)), s0, inv0);
     }
     // ---- stage 1 (data x table): Y[b][k1] = sum_m W32^(m k1) x[b + 32 m] ----
-    mf_f4 yr[2][2], yi[2][2];
     {
         const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
 #pragma unroll
@@ -256,6 +254,29 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
                 yr[tb][tk][e] = v.x;
                 yi[tb][tk][e] = v.y;
             }
+}
+
+// One tile T for wavefront w (pencil p = w & 7, residue rs = w >> 3), with the
+// slices of the previous and next tiles over its first half and the next
+// tile's multiplier inputs loaded after the last use of gm; the result of
+// pencil p is left in column p of cur.  Nothing here waits for vector memory.
+template <int OPK>
+__device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                         unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
+                                         const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
+                                         const MrTile& T, const Mr2SliceOffs& so, char* obase, int ncv, bool hasprev,
+                                         const char* ibase, bool more, const MrTile& Tn, bool load_g,
+                                         float (&gm)[2][4][2], int w, int l)
+{
+    constexpr bool mimag = mf_imag_multiplier<OPK>();
+    const int g = l >> 4, j = l & 15;
+    const int p = w & 7, rs = w >> 3, kb = 4 * j + 256 * g + rs;
+    unsigned sb[2];
+    float inv0;
+    mf_f4 yr[2][2], yi[2][2];
+    mr2_front(a, cur, nxt, twm, tws, ft, so, obase, ncv, hasprev, ibase, more, w, l, sb, inv0, yr, yi);
+    const float2* twl = twm + l;
+    const float2* tsl = tws + l;
     mf_f4 ur[2][2], ui[2][2];
     float s2[2], inv2[2];
     MfKCoef kc{};
@@ -438,6 +459,294 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma2048r(PencilArgs<float> a)
     if (any) {
         unsigned char* last = stg0 + (it & 1) * M::STG;
         char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp));
+        const int ncv = a.ncol - Tp.col0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) mr2_slice(last, a, so, obase, ncv, true, nullptr, false, w, k);
+    }
+}
+
+
+// ---- Several outputs from one forward transform (the potential form) ----
+//
+// MASK as for transforms of 512: bit 0 A (Gk), bit 1 d/dz A (i kzh Gk), bit 2
+// the Laplacian (-|k|^2 Gk); output o (the o-th set bit) is component o of
+// the output addressing.  Each wavefront transforms its residue forward once
+// and holds the stage-2 results (E and O of both k1 parities) for every
+// output; each output runs the k-space multiply, the inverse stages and the
+// residue combine into cur.  An output other than the last is stored from cur
+// by the wavefronts' slices (after a barrier), and the next output's hand-over
+// rows are written after another one, when every wavefront has read its
+// blocks; the last stays in cur and is stored during the next tile.
+
+constexpr int mr2_popc(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1); }
+
+// multiplier of point (tt, h, e) for fan-out output OPK: A Gk, d/dz A
+// i kzh Gk (applied as kzh Gk times i), the Laplacian (c0 - kz^2) Gk with
+// c0 = -(kx^2 + ky^2) of the pencil
+template <int OPK>
+__device__ __forceinline__ float mr2_fan_kmul(float c0, float kst, const float (&gr)[2][4][2], int tt, int h, int e,
+                                              int kb, float kbf)
+{
+    const float v = gr[h][e][tt];
+    if constexpr (OPK == OP_GRAD || OPK == OP_LAPL) {
+        const float kzw = (kbf + (float)(2 * tt + 64 * e - 1024 * h)) * kst;
+        if constexpr (OPK == OP_GRAD) return ((h == 1 && tt == 0 && e == 0 && kb == 0) ? 0.0f : kzw) * v;
+        else return (c0 - kzw * kzw) * v;
+    }
+    return v;
+}
+
+// inverse stage 2 and the residue combine of wavefront residue RS: the
+// other residue's rows first (handed over through cur, after a barrier when
+// WBAR), then this wavefront's own, kept in registers over the barrier
+template <int RS, bool WBAR>
+__device__ __forceinline__ void mr2_fan_tail(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                             const mf_h8* __restrict__ ft, const float2* __restrict__ tsl,
+                                             const mf_f4 (&ur)[2][2], const mf_f4 (&ui)[2][2], float sa, float sb2,
+                                             float fo, const unsigned (&sb)[2], int l)
+{
+    float2 mine[2][4];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int tn = q ? RS : 1 - RS;
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = float2{ur[0][tn][e], ui[0][tn][e]};
+            v1[e] = float2{ur[1][tn][e], ui[1][tn][e]};
+        }
+        const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
+        mf_f4 dr, di, qr, qi;
+        mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(qr, qi, mr_table(ft, 2, l), B1);
+        float2 o[2][4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            o[0][e] = float2{(dr[e] + qr[e]) * fo, -(di[e] + qi[e]) * fo};
+            o[1][e] = float2{(dr[e] - qr[e]) * fo, -(di[e] - qi[e]) * fo};
+        }
+        if (q == 0) {
+            // the previous output's rows have been read from cur by every wavefront
+            if constexpr (WBAR) mr2_lds_barrier();
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) *reinterpret_cast<float2*>(cur + sb[tn] + 2048 * e + 32768 * h) = o[h][e];
+        } else {
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) mine[h][e] = o[h][e];
+        }
+    }
+    mr2_lds_barrier();
+    mr2_combine<RS>(cur, sb[RS], tsl, mine);
+}
+
+template <int OPK, bool WBAR, bool FIRST, bool LASTG>
+__device__ __forceinline__ void mr2_fan_out(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                            unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
+                                            const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
+                                            const MrTile& T, const Mr2SliceOffs& so, char* obase, int ncv,
+                                            bool hasprev, const char* ibase, bool more, const MrTile& Tn,
+                                            bool load_g, float (&gm)[2][4][2], const mf_f4 (&er)[2],
+                                            const mf_f4 (&ei)[2], const mf_f4 (&orr)[2], const mf_f4 (&oi)[2],
+                                            float inv0, const unsigned (&sb)[2], int w, int l)
+{
+    constexpr bool mimag = mf_imag_multiplier<OPK>();
+    const int g = l >> 4, j = l & 15;
+    const int p = w & 7, rs = w >> 3, kb = 4 * j + 256 * g + rs;
+    const float2* twl = twm + l;
+    const float2* tsl = tws + l;
+    float c0 = 0.0f;
+    if constexpr (OPK == OP_LAPL) c0 = mf_kcoef<OPK>(a, tile_k(a, T.col0 + p, T.slab, T.col0 + p < a.ncol), 0).c0;
+    const float kst = (float)(2.0 * kPi * a.dk[2]);
+    const float kbf = (float)kb;
+    mf_f4 ur[2][2], ui[2][2];
+    float s2[2], inv2[2];
+#pragma unroll
+    for (int tk = 0; tk < 2; ++tk) {
+        float2 za[4], zb[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float m0 = mr2_fan_kmul<OPK>(c0, kst, gm, tk, 0, e, kb, kbf);
+            const float m1 = mr2_fan_kmul<OPK>(c0, kst, gm, tk, 1, e, kb, kbf);
+            const float mp = m0 + m1, mm = m0 - m1;
+            const float ar = er[tk][e] * mp + orr[tk][e] * mm, ai = ei[tk][e] * mp + oi[tk][e] * mm;
+            const float br = er[tk][e] * mm + orr[tk][e] * mp, bi = ei[tk][e] * mm + oi[tk][e] * mp;
+            if constexpr (mimag) {
+                za[e] = float2{-ai, ar};
+                zb[e] = float2{-bi, br};
+            } else {
+                za[e] = float2{ar, ai};
+                zb[e] = float2{br, bi};
+            }
+        }
+        if (FIRST && tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 2);
+        if (LASTG && tk == 1 && load_g) mr2_kload<OPK>(gm, a, Tn, Tn.col0 + p, kb, mr_after(kb, za, zb));
+        mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
+        {
+            const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
+            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
+        }
+        if (FIRST && tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 3);
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float2 v = cmul(float2{ur[tk][tn][e], ui[tk][tn][e]}, twl[(8 * tn + 2 * e + tk) * 64]);
+                ur[tk][tn][e] = v.x;
+                ui[tk][tn][e] = v.y;
+            }
+    }
+    const float smin = __builtin_fminf(s2[0], s2[1]);
+    const float sa = 0.5f * smin * inv2[0], sb2 = 0.5f * smin * inv2[1];
+    const float fo = 2.0f * inv0 * __builtin_fmaxf(inv2[0], inv2[1]);
+    if (rs) mr2_fan_tail<1, WBAR>(a, cur, ft, tsl, ur, ui, sa, sb2, fo, sb, l);
+    else mr2_fan_tail<0, WBAR>(a, cur, ft, tsl, ur, ui, sa, sb2, fo, sb, l);
+}
+
+// after a non-last output: every row of it is in cur; wavefront w stores its
+// blocks w + 16 k to that output (ob: its tile's column 0, ncvt valid columns)
+__device__ __forceinline__ void mr2_fan_store(unsigned char* __restrict__ cur, const PencilArgs<float>& a,
+                                              const Mr2SliceOffs& so, char* ob, int ncvt, int w)
+{
+    mr2_lds_barrier();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) mr2_slice(cur, a, so, ob, ncvt, true, nullptr, false, w, k);
+}
+
+template <int MASK>
+__device__ __forceinline__ void mr2_fan_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                             unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
+                                             const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
+                                             const MrTile& T, const Mr2SliceOffs& so, char* obase, int ncv,
+                                             bool hasprev, const char* ibase, bool more, const MrTile& Tn,
+                                             bool load_g, float (&gm)[2][4][2], int w, int l, long ostride)
+{
+    constexpr int NO = mr2_popc(MASK);
+    constexpr int OD = MASK & 1, OL = mr2_popc(MASK & 3);
+    unsigned sb[2];
+    float inv0;
+    mf_f4 yr[2][2], yi[2][2];
+    mr2_front(a, cur, nxt, twm, tws, ft, so, obase, ncv, hasprev, ibase, more, w, l, sb, inv0, yr, yi);
+    // stage 2 (table x data) of both k1 parities, held for every output
+    mf_f4 er[2], ei[2], orr[2], oi[2];
+#pragma unroll
+    for (int tk = 0; tk < 2; ++tk) {
+        float2 v0[4], v1[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            v0[e] = float2{yr[0][tk][e], yi[0][tk][e]};
+            v1[e] = float2{yr[1][tk][e], yi[1][tk][e]};
+        }
+        mr_mfma_b<true>(er[tk], ei[tk], mr_table(ft, 0, l), mr_split4u(v0, a.fone));
+        mr_mfma_b<true>(orr[tk], oi[tk], mr_table(ft, 2, l), mr_split4u(v1, a.fone));
+    }
+    char* ob = const_cast<char*>(mr_tile_base(a.out, a.oa, T));
+    const int ncvt = a.ncol - T.col0;
+    if constexpr ((MASK & 1) != 0) {
+        mr2_fan_out<OP_PLAIN, false, true, NO == 1>(a, cur, nxt, twm, tws, ft, T, so, obase, ncv, hasprev, ibase,
+                                                    more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w, l);
+        if constexpr (NO > 1) mr2_fan_store(cur, a, so, ob, ncvt, w);
+    }
+    if constexpr ((MASK & 2) != 0) {
+        mr2_fan_out<OP_GRAD, (OD > 0), (OD == 0), (OD == NO - 1)>(a, cur, nxt, twm, tws, ft, T, so, obase, ncv,
+                                                                  hasprev, ibase, more, Tn, load_g, gm, er, ei, orr,
+                                                                  oi, inv0, sb, w, l);
+        if constexpr (OD < NO - 1) mr2_fan_store(cur, a, so, ob + OD * ostride, ncvt, w);
+    }
+    if constexpr ((MASK & 4) != 0)
+        mr2_fan_out<OP_LAPL, (OL > 0), (OL == 0), true>(a, cur, nxt, twm, tws, ft, T, so, obase, ncv, hasprev,
+                                                        ibase, more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w, l);
+}
+
+// the fan-out kernel, launched as the one-output kernel with a.kmap = KMAP_Z
+template <int MASK>
+__global__ void __launch_bounds__(1024, 4) k_zmfma2048f(PencilArgs<float> a)
+{
+    using M = MfmaZR2;
+    constexpr int NO = mr2_popc(MASK);
+    constexpr int OPG = (MASK & 1) ? OP_PLAIN : (MASK & 2) ? OP_GRAD : OP_LAPL;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    unsigned char* stg0 = smem_raw;
+    float2* twm = reinterpret_cast<float2*>(smem_raw + 2 * M::STG);
+    float2* tws = reinterpret_cast<float2*>(smem_raw + 2 * M::STG + M::TWB);
+    mf_h8* ftw = reinterpret_cast<mf_h8*>(smem_raw + 2 * M::STG + M::TWB + M::TSB);
+
+    const int tid = threadIdx.x;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6), l0 = tid & 63;
+    const int nunit = mr_nunit(a);
+    const int G = gridDim.x;
+    const long ostride = a.oa.comp * 8;
+    const long olast = (long)(NO - 1) * ostride;
+
+    {
+        const int slot = tid >> 6, j = tid & 15, g = (tid >> 4) & 3;
+        const int r = 2 * j + (slot >> 3), s = 8 * g + (slot & 7);
+        twm[tid] = a.plan.tw[(2 * r * s) & (M::L - 1)];
+        const int n = 2 * j + (slot >> 3) + 32 * (4 * g + (slot & 3) + 16 * ((slot >> 2) & 1));
+        tws[tid] = a.plan.tw[n];
+    }
+    if (tid < 3 * 4 * 64) {
+        const int ln = tid & 63, f = (tid >> 6) & 3, tb = tid >> 8;
+        const int fo = (f == 0) ? 0 : (f == 1) ? 2 : (f == 2) ? 1 : 3;
+        ftw[(tb * 4 + fo) * 64 + ln] = reinterpret_cast<const mf_h8*>(a.mfrag16)[(tb * 64 + ln) * 4 + f];
+    }
+    const Mr2SliceOffs so = mr2_slice_offs(a, w, l0);
+    int u = blockIdx.x, m = 0;
+    MrUnit U{};
+    MrTile T{};
+    const bool any = u < nunit && (U = mr_unit<M::P>(a, u, nunit), mr_tile(a, U, 0, T));
+    float gm[2][4][2];
+    if (any) {
+        const char* ib = mr_tile_base(a.in, a.ia, T);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int n = w + 16 * k;
+            __builtin_amdgcn_global_load_lds(ib + mr2_zoff_u(a.ia, 16 * n) + so.ilane, (mr_lds_ptr)(stg0 + n * 1024),
+                                             16, 0, 0);
+        }
+        const int kb0 = 4 * (l0 & 15) + 256 * (l0 >> 4) + (w >> 3);
+        mr2_kload<OPG>(gm, a, T, T.col0 + (w & 7), kb0, kb0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+
+    MrTile Tp = T;
+    int it = 0;
+    for (; any; ++it) {
+        int l = l0;
+        asm volatile("" : "+v"(l));
+        unsigned char* cur = stg0 + (it & 1) * M::STG;
+        unsigned char* nxt = stg0 + ((it & 1) ^ 1) * M::STG;
+        MrTile Tn{};
+        MrUnit Un = U;
+        int un = u, mn = 1;
+        if (m == 1 || !mr_tile(a, U, 1, Tn)) {
+            un = u + G;
+            mn = 0;
+        }
+        const bool more = un < nunit && (mn == 1 || (Un = mr_unit<M::P>(a, un, nunit), mr_tile(a, Un, 0, Tn)));
+        const bool load_g = more && !(mn == 1 && Tn.kyo == T.kyo);
+        char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp)) + olast;
+        const char* ibase = mr_tile_base(a.in, a.ia, Tn);
+        const int ncv = a.ncol - Tp.col0;
+        mr2_fan_step<MASK>(a, cur, nxt, twm, tws, ftw, T, so, obase, ncv, it > 0, ibase, more, Tn, load_g, gm, w, l,
+                           ostride);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        Tp = T;
+        T = Tn;
+        U = Un;
+        u = un;
+        m = mn;
+        if (!more) break;
+    }
+    if (any) {
+        unsigned char* last = stg0 + (it & 1) * M::STG;
+        char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp)) + olast;
         const int ncv = a.ncol - Tp.col0;
 #pragma unroll
         for (int k = 0; k < 4; ++k) mr2_slice(last, a, so, obase, ncv, true, nullptr, false, w, k);

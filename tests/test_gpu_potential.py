@@ -20,12 +20,14 @@ TOL = {"fp64": 1e-12, "fp32": 2e-6, "mixed": 1e-6}
 NAMES = ("A", "dx", "dy", "dz", "lap")
 
 
-def _potential(prec, n, bc, dx, reg, b, outputs=NAMES, tuning=None):
+def _potential(prec, n, bc, dx, reg, b, outputs=NAMES, tuning=None, zlaunches=None):
     s = P.poisson_solver(PREC[prec])
     if tuning:
         s.set_tuning(**tuning)
     s.setup3d(n, bc, dx, reg)
     assert s.last_status == 0
+    if zlaunches is not None:
+        assert s.potential_zlaunches(outputs) == zlaunches
     s.push(b)
     s.solve3d_potential(outputs)
     assert s.last_status == 0
@@ -76,7 +78,12 @@ def test_potential_subset_and_errors():
     lo_g, _ = O.solve(n, bc, dx, reg, [b, b, b], op="grad")
     assert O.rel_l2(out["dy"], lo_g[1]) <= 1e-12
     s = P.poisson_solver()
+    assert s.potential_zlaunches(("A",)) == -P.PS_ESTATE       # not set up
     s.setup3d(n, bc, dx, reg)
+    # FP64: one z launch per distinct z multiplier (A with d/dx and d/dy, d/dz, Laplacian)
+    assert s.potential_zlaunches(("A",)) == 1
+    assert s.potential_zlaunches(("dx", "dy", "dz")) == 2
+    assert s.potential_zlaunches(NAMES) == 3
     s.solve3d_potential(("A",))
     assert s.last_status == P.PS_ESTATE          # nothing pushed
     s.push(None, b, None)
@@ -143,13 +150,15 @@ def test_potential_long_z(prec):
 FAN_SUBSETS = [("A", "dz"), ("A", "lap"), ("dz", "lap"), ("dx", "dz"), ("dy", "lap"), ("A", "dx", "dy", "dz"),
                NAMES]
 FAN_CASES = [((18, 10, 256), (0, 0, 0), (0.1, 0.12, 1 / 128), 2),
-             ((16, 12, 256), (1, 1, 0), (1 / 16, 1 / 12, 1 / 128), 6)]
+             ((16, 12, 256), (1, 1, 0), (1 / 16, 1 / 12, 1 / 128), 6),
+             ((10, 6, 1024), (0, 0, 0), (0.1, 0.12, 1 / 512), 4),
+             ((12, 7, 1024), (1, 1, 0), (1 / 12, 1 / 7, 1 / 512), 10)]
 
 
 @pytest.mark.parametrize("case", FAN_CASES, ids=lambda c: "%dx%dx%d-%d%d%d" % (c[0] + c[1]))
 @pytest.mark.parametrize("zblk", [-1, 0, 3], ids=lambda z: "zblk%d" % z)
-def test_potential_fanout_512(case, zblk):
-    """z transforms of 512 (nz = 256 unbounded), mixed form: the z outputs
+def test_potential_fanout(case, zblk):
+    """z transforms of 512 and 2048 (nz = 256, 1024 unbounded), mixed form: the z outputs
     (A, d/dz A, Laplacian) from one forward transform of the scalar (tuning
     zfan 1: every two- and three-output set, ragged column tiles, odd and
     even slab counts) against the oracle (rel-L2 <= 1e-6) and against one z
@@ -161,9 +170,11 @@ def test_potential_fanout_512(case, zblk):
     lo_l, _ = O.solve(n, bc, dx, reg, [b], op="lap")
     ref = {"A": lo_a[0], "dx": lo_g[0], "dy": lo_g[1], "dz": lo_g[2], "lap": lo_l[0]}
     for outs in FAN_SUBSETS:
-        announce("potential fan-out 512", n, bc, zblk, outs)
-        f1, _ = _potential("mixed", n, bc, dx, reg, b, outputs=outs, tuning={"zblk": zblk, "zfan": 1})
-        f0, _ = _potential("mixed", n, bc, dx, reg, b, outputs=outs, tuning={"zblk": zblk, "zfan": 0})
+        announce("potential fan-out", n, bc, zblk, outs)
+        nzq = int(bool({"A", "dx", "dy"} & set(outs))) + int("dz" in outs) + int("lap" in outs)
+        f1, _ = _potential("mixed", n, bc, dx, reg, b, outputs=outs, tuning={"zblk": zblk, "zfan": 1}, zlaunches=1)
+        f0, _ = _potential("mixed", n, bc, dx, reg, b, outputs=outs, tuning={"zblk": zblk, "zfan": 0},
+                           zlaunches=nzq)
         for k in NAMES:
             if k in outs:
                 assert O.rel_l2(f1[k], ref[k]) <= TOL["mixed"], (outs, k)
