@@ -335,10 +335,11 @@ def bench_config(a):
     component-wise solves (A plain; grad of the scalar pushed three times),
     and the potential extension (one pipeline from the one scalar).  `value`
     is the rate of the potential form.  Each solve times its passes with HIP
-    events."""
+    events.  --pmc-child (for counter runs): the one precision --prec, no FP64
+    check, a progress line per solve and no JSON line."""
     import poissonsolver_amd as P
     codes = {"fp32": P.PS_FP32, "mixed": P.PS_MIXED, "fp64": P.PS_FP64}
-    precs = ["mixed", "fp32"] if a.prec == "mixed" else [a.prec]
+    precs = ["mixed", "fp32"] if a.prec == "mixed" and not a.pmc_child else [a.prec]
     pot = None          # outputs of the potential form
     if a.workload == "ppu256":
         n = a.n if a.n != 512 else 256
@@ -367,7 +368,7 @@ def bench_config(a):
     shape = (ncell[2], ncell[1], ncell[0])
     results = {}
     outs = {}
-    for prec in precs + ["fp64"]:
+    for prec in precs + ([] if a.pmc_child else ["fp64"]):
         per = {}
         for name, op, rhs in solves:
             s = P.poisson_solver(codes[prec])
@@ -384,6 +385,10 @@ def bench_config(a):
                 s.solve3d()
             if s.last_status:
                 raise RuntimeError("solve %s %s failed: status %d" % (prec, name, s.last_status))
+            if a.pmc_child:
+                print("solved", prec, name, flush=True)
+                s.close()
+                continue
             nout = 1 if op == "div" else len(rhs) if op != "plain" else len(rhs)
             lhs = [np.empty(shape) for _ in range(nout)]
             s.pull(*lhs)
@@ -422,6 +427,10 @@ def bench_config(a):
                     s.solve3d_potential(grp)
                 if s.last_status:
                     raise RuntimeError("potential solve %s %s failed: status %d" % (prec, grp, s.last_status))
+                if a.pmc_child:
+                    print("solved", prec, "potential", grp, flush=True)
+                    s.close()
+                    continue
                 arrs = {k: (np.empty(shape) if k in grp else None) for k in P.poisson_solver.POT_NAMES}
                 s.pull_potential(*[arrs[k] for k in P.poisson_solver.POT_NAMES])
                 for k in grp:
@@ -429,8 +438,11 @@ def bench_config(a):
                         got[k] = arrs[k][sub].copy()
                 del arrs
                 s.close()
-            outs[(prec, "potential")] = [got[k] for k in pot]
+            if not a.pmc_child:
+                outs[(prec, "potential")] = [got[k] for k in pot]
         results[prec] = per
+    if a.pmc_child:
+        return 0
     line_prec = precs[0]
     comp_ms = sum(results[line_prec][nm]["ms"] for nm, _, _ in solves)
     step_ms = results[line_prec]["potential"]["ms"] if pot is not None else comp_ms

@@ -205,6 +205,23 @@ template <int OPK, int P> static hipError_t launch_zmfma2048_p(PencilArgs<float>
     return hipGetLastError();
 }
 
+// the fan-out of 2048: workgroups of 8 wavefronts (one pencil each, both
+// residues; 2 per SIMD), one per CU at a time, four per CU over the launch
+template <int MASK> static hipError_t launch_zmfma2048_fanp_m(PencilArgs<float> a, hipStream_t s)
+{
+    const size_t shm = MfmaZR2::lds_bytes();
+    hipError_t e = allow_lds(k_zmfma2048p<MASK>, shm);
+    if (e != hipSuccess) return e;
+    a.kmap = KMAP_Z;
+    a.ncoltile = (a.ncol + MfmaZR2::P - 1) / MfmaZR2::P;
+    if ((long)a.ncoltile * a.nslab >= (1L << 31)) return hipErrorInvalidValue;
+    const long nunit = (long)a.ncoltile * ((a.nslab + 1) / 2);
+    const long ncu = std::max(device_cus(), 8);
+    const long nwg = std::min<long>(nunit, 4 * ncu);
+    hipLaunchKernelGGL((k_zmfma2048p<MASK>), dim3((unsigned)nwg), dim3(64 * MfmaZR2P::NW), shm, s, a);
+    return hipGetLastError();
+}
+
 // tuning mfma 1: the register-pencil kernel when the arguments allow it (as
 // for transforms of 1024), else 4-pencil workgroups that reload the pencil for
 // residue 1; 2: 8-pencil workgroups that hold it; 3: the 4-pencil ones
@@ -242,21 +259,6 @@ template <int MASK> static hipError_t launch_zmfma512_fan_m(PencilArgs<float> a,
     return hipGetLastError();
 }
 
-template <int MASK> static hipError_t launch_zmfma2048_fan_m(PencilArgs<float> a, hipStream_t s)
-{
-    const size_t shm = MfmaZR2::lds_bytes();
-    hipError_t e = allow_lds(k_zmfma2048f<MASK>, shm);
-    if (e != hipSuccess) return e;
-    a.kmap = KMAP_Z;
-    a.ncoltile = (a.ncol + MfmaZR2::P - 1) / MfmaZR2::P;
-    if ((long)a.ncoltile * a.nslab >= (1L << 31)) return hipErrorInvalidValue;
-    const long nunit = (long)a.ncoltile * ((a.nslab + 1) / 2);
-    const long ncu = std::max(device_cus(), 8);
-    const long nwg = std::min<long>(nunit, 4 * ncu);
-    hipLaunchKernelGGL((k_zmfma2048f<MASK>), dim3((unsigned)nwg), dim3(64 * MfmaZR2::NW), shm, s, a);
-    return hipGetLastError();
-}
-
 bool zmfma_fan_ok(const PencilArgs<float>& a, int mask, int variant)
 {
     const bool len = (a.plan.L == 512 || a.plan.L == 2048) && a.n_in == a.plan.L / 2 && a.n_out == a.plan.L / 2;
@@ -269,10 +271,10 @@ hipError_t launch_zmfma_fan(const PencilArgs<float>& a, int mask, hipStream_t s)
     if (!zmfma_fan_ok(a, mask, 1)) return hipErrorInvalidValue;
     if (a.plan.L == 2048) {
         switch (mask) {
-        case 3: return launch_zmfma2048_fan_m<3>(a, s);
-        case 5: return launch_zmfma2048_fan_m<5>(a, s);
-        case 6: return launch_zmfma2048_fan_m<6>(a, s);
-        default: return launch_zmfma2048_fan_m<7>(a, s);
+        case 3: return launch_zmfma2048_fanp_m<3>(a, s);
+        case 5: return launch_zmfma2048_fanp_m<5>(a, s);
+        case 6: return launch_zmfma2048_fanp_m<6>(a, s);
+        default: return launch_zmfma2048_fanp_m<7>(a, s);
         }
     }
     switch (mask) {

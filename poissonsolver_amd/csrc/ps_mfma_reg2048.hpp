@@ -470,13 +470,10 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma2048r(PencilArgs<float> a)
 //
 // MASK as for transforms of 512: bit 0 A (Gk), bit 1 d/dz A (i kzh Gk), bit 2
 // the Laplacian (-|k|^2 Gk); output o (the o-th set bit) is component o of
-// the output addressing.  Each wavefront transforms its residue forward once
-// and holds the stage-2 results (E and O of both k1 parities) for every
-// output; each output runs the k-space multiply, the inverse stages and the
-// residue combine into cur.  An output other than the last is stored from cur
-// by the wavefronts' slices (after a barrier), and the next output's hand-over
-// rows are written after another one, when every wavefront has read its
-// blocks; the last stays in cur and is stored during the next tile.
+// the output addressing.  An output other than the last is stored from the
+// tile buffer by the wavefronts' slices (after a barrier) before the next
+// one is written (after another); the last stays in the buffer and is stored
+// during the next tile.
 
 constexpr int mr2_popc(int m) { return (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1); }
 
@@ -496,71 +493,98 @@ __device__ __forceinline__ float mr2_fan_kmul(float c0, float kst, const float (
     return v;
 }
 
-// inverse stage 2 and the residue combine of wavefront residue RS: the
-// other residue's rows first (handed over through cur, after a barrier when
-// WBAR), then this wavefront's own, kept in registers over the barrier
-template <int RS, bool WBAR>
-__device__ __forceinline__ void mr2_fan_tail(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
-                                             const mf_h8* __restrict__ ft, const float2* __restrict__ tsl,
-                                             const mf_f4 (&ur)[2][2], const mf_f4 (&ui)[2][2], float sa, float sb2,
-                                             float fo, const unsigned (&sb)[2], int l)
+// ---- The fan-out with one wavefront per pencil (both residues) ----
+//
+// Workgroups of 8 wavefronts (2 per SIMD, up to 256 VGPRs each) on the same
+// 8-column tiles: wavefront w owns pencil w and its column of the tile, runs
+// both residues' forward transforms from its one read of the pencil, holds
+// both residues' stage-2 results and multiplier inputs for every output, and
+// combines the residues in its own column (residue 0's rows are written
+// there, then read back as residue 1's are added), so no residue hand-over
+// barrier is needed.  The slices move blocks w + 8 k (k < 8) of a tile.
+struct MfmaZR2P {
+    static constexpr int NW = 8;
+};
+
+// slice k of wavefront w (block w + 8 k, rot k & 1) with the offsets so[k & 1]
+__device__ __forceinline__ void mr2p_slice(unsigned char* __restrict__ buf, const PencilArgs<float>& a,
+                                           const Mr2SliceOffs (&so)[2], char* obase, int ncv, bool hasprev,
+                                           const char* ibase, bool more, int w, int k)
 {
-    float2 mine[2][4];
+    mr2_slice(buf, a, so[k & 1], obase, ncv, hasprev, ibase, more, w + 8 * (k & 1), k >> 1);
+}
+
+// residue RS's forward transform of the pencil x (the FP16 split scales it
+// by s0) to its stage-2 results E and O of both k1 parities; residue 1
+// multiplies x by W2048^n first
+template <int RS>
+__device__ __forceinline__ void mr2p_forward(const PencilArgs<float>& a, const float2 (&x)[2][2][4], float s0,
+                                             const float2* __restrict__ twl, const float2* __restrict__ tsl,
+                                             const mf_h8* __restrict__ ft, int l, mf_f4 (&er)[2], mf_f4 (&ei)[2],
+                                             mf_f4 (&orr)[2], mf_f4 (&oi)[2])
+{
+    mf_f4 yr[2][2], yi[2][2];
+    {
+        const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        const int tn = q ? RS : 1 - RS;
+        for (int tb = 0; tb < 2; ++tb) {
+            float2 xa[4], xd[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float2 x0 = x[tb][0][e], x1 = x[tb][1][e];
+                if (RS) {
+                    x0 = cmul(x0, tsl[(8 * tb + e) * 64]);
+                    x1 = cmul(x1, tsl[(8 * tb + 4 + e) * 64]);
+                }
+                xa[e] = float2{x0.x + x1.x, x0.y + x1.y};
+                xd[e] = float2{x0.x - x1.x, x0.y - x1.y};
+            }
+            const MfFrag XA = mr_split4s<false>(xa, s0, a.fzero), XD = mr_split4s<false>(xd, s0, a.fzero);
+            mr_mfma_a<true>(yr[tb][0], yi[tb][0], XA, Fe);
+            mr_mfma_a<true>(yr[tb][1], yi[tb][1], XD, Fo);
+        }
+    }
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int tk = 0; tk < 2; ++tk)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float2 v = cmul(float2{yr[tb][tk][e], yi[tb][tk][e]}, twl[(8 * tk + 2 * e + tb) * 64]);
+                yr[tb][tk][e] = v.x;
+                yi[tb][tk][e] = v.y;
+            }
+#pragma unroll
+    for (int tk = 0; tk < 2; ++tk) {
         float2 v0[4], v1[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            v0[e] = float2{ur[0][tn][e], ui[0][tn][e]};
-            v1[e] = float2{ur[1][tn][e], ui[1][tn][e]};
+            v0[e] = float2{yr[0][tk][e], yi[0][tk][e]};
+            v1[e] = float2{yr[1][tk][e], yi[1][tk][e]};
         }
-        const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
-        mf_f4 dr, di, qr, qi;
-        mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
-        mr_mfma_b<true>(qr, qi, mr_table(ft, 2, l), B1);
-        float2 o[2][4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            o[0][e] = float2{(dr[e] + qr[e]) * fo, -(di[e] + qi[e]) * fo};
-            o[1][e] = float2{(dr[e] - qr[e]) * fo, -(di[e] - qi[e]) * fo};
-        }
-        if (q == 0) {
-            // the previous output's rows have been read from cur by every wavefront
-            if constexpr (WBAR) mr2_lds_barrier();
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) *reinterpret_cast<float2*>(cur + sb[tn] + 2048 * e + 32768 * h) = o[h][e];
-        } else {
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) mine[h][e] = o[h][e];
-        }
+        mr_mfma_b<true>(er[tk], ei[tk], mr_table(ft, 0, l), mr_split4u(v0, a.fone));
+        mr_mfma_b<true>(orr[tk], oi[tk], mr_table(ft, 2, l), mr_split4u(v1, a.fone));
     }
-    mr2_lds_barrier();
-    mr2_combine<RS>(cur, sb[RS], tsl, mine);
 }
 
-template <int OPK, bool WBAR, bool FIRST, bool LASTG>
-__device__ __forceinline__ void mr2_fan_out(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
-                                            unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
-                                            const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
-                                            const MrTile& T, const Mr2SliceOffs& so, char* obase, int ncv,
-                                            bool hasprev, const char* ibase, bool more, const MrTile& Tn,
-                                            bool load_g, float (&gm)[2][4][2], const mf_f4 (&er)[2],
-                                            const mf_f4 (&ei)[2], const mf_f4 (&orr)[2], const mf_f4 (&oi)[2],
-                                            float inv0, const unsigned (&sb)[2], int w, int l)
+// one output of residue RS from its held stage-2 results: multiply, inverse
+// stages 1 and 2; residue 0 writes its rows to the pencil's column of cur,
+// residue 1 adds W2048^-n times its rows to them.  FIRST: slices 5 + RS
+// (and 7 after residue 1) of the tile's store and fetch; LASTG: the last use
+// of gr, after which the next tile's multiplier inputs are loaded into it.
+template <int OPK, int RS, bool FIRST, bool LASTG>
+__device__ __forceinline__ void mr2p_inverse(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                             unsigned char* __restrict__ nxt, const float2* __restrict__ twl,
+                                             const float2* __restrict__ tsl, const mf_h8* __restrict__ ft,
+                                             const MrTile& T, const Mr2SliceOffs (&so)[2], char* obase, int ncv,
+                                             bool hasprev, const char* ibase, bool more, const MrTile& Tn,
+                                             bool load_g, float (&gr)[2][4][2], const mf_f4 (&er)[2],
+                                             const mf_f4 (&ei)[2], const mf_f4 (&orr)[2], const mf_f4 (&oi)[2],
+                                             float inv0, float c0, float kst, const unsigned (&sb)[2], int w, int l)
 {
     constexpr bool mimag = mf_imag_multiplier<OPK>();
     const int g = l >> 4, j = l & 15;
-    const int p = w & 7, rs = w >> 3, kb = 4 * j + 256 * g + rs;
-    const float2* twl = twm + l;
-    const float2* tsl = tws + l;
-    float c0 = 0.0f;
-    if constexpr (OPK == OP_LAPL) c0 = mf_kcoef<OPK>(a, tile_k(a, T.col0 + p, T.slab, T.col0 + p < a.ncol), 0).c0;
-    const float kst = (float)(2.0 * kPi * a.dk[2]);
+    const int kb = 4 * j + 256 * g + RS;
     const float kbf = (float)kb;
     mf_f4 ur[2][2], ui[2][2];
     float s2[2], inv2[2];
@@ -569,8 +593,8 @@ __device__ __forceinline__ void mr2_fan_out(const PencilArgs<float>& a, unsigned
         float2 za[4], zb[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            const float m0 = mr2_fan_kmul<OPK>(c0, kst, gm, tk, 0, e, kb, kbf);
-            const float m1 = mr2_fan_kmul<OPK>(c0, kst, gm, tk, 1, e, kb, kbf);
+            const float m0 = mr2_fan_kmul<OPK>(c0, kst, gr, tk, 0, e, kb, kbf);
+            const float m1 = mr2_fan_kmul<OPK>(c0, kst, gr, tk, 1, e, kb, kbf);
             const float mp = m0 + m1, mm = m0 - m1;
             const float ar = er[tk][e] * mp + orr[tk][e] * mm, ai = ei[tk][e] * mp + oi[tk][e] * mm;
             const float br = er[tk][e] * mm + orr[tk][e] * mp, bi = ei[tk][e] * mm + oi[tk][e] * mp;
@@ -582,15 +606,14 @@ __device__ __forceinline__ void mr2_fan_out(const PencilArgs<float>& a, unsigned
                 zb[e] = float2{br, bi};
             }
         }
-        if (FIRST && tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 2);
-        if (LASTG && tk == 1 && load_g) mr2_kload<OPK>(gm, a, Tn, Tn.col0 + p, kb, mr_after(kb, za, zb));
+        if (LASTG && tk == 1 && load_g) mr2_kload<OPK>(gr, a, Tn, Tn.col0 + w, kb, mr_after(kb, za, zb));
         mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
             const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
             mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
-        if (FIRST && tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 3);
+        if (FIRST && tk == 0) mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 5 + RS);
 #pragma unroll
         for (int tn = 0; tn < 2; ++tn)
 #pragma unroll
@@ -603,70 +626,135 @@ __device__ __forceinline__ void mr2_fan_out(const PencilArgs<float>& a, unsigned
     const float smin = __builtin_fminf(s2[0], s2[1]);
     const float sa = 0.5f * smin * inv2[0], sb2 = 0.5f * smin * inv2[1];
     const float fo = 2.0f * inv0 * __builtin_fmaxf(inv2[0], inv2[1]);
-    if (rs) mr2_fan_tail<1, WBAR>(a, cur, ft, tsl, ur, ui, sa, sb2, fo, sb, l);
-    else mr2_fan_tail<0, WBAR>(a, cur, ft, tsl, ur, ui, sa, sb2, fo, sb, l);
-}
-
-// after a non-last output: every row of it is in cur; wavefront w stores its
-// blocks w + 16 k to that output (ob: its tile's column 0, ncvt valid columns)
-__device__ __forceinline__ void mr2_fan_store(unsigned char* __restrict__ cur, const PencilArgs<float>& a,
-                                              const Mr2SliceOffs& so, char* ob, int ncvt, int w)
-{
-    mr2_lds_barrier();
 #pragma unroll
-    for (int k = 0; k < 4; ++k) mr2_slice(cur, a, so, ob, ncvt, true, nullptr, false, w, k);
-}
-
-template <int MASK>
-__device__ __forceinline__ void mr2_fan_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
-                                             unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
-                                             const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
-                                             const MrTile& T, const Mr2SliceOffs& so, char* obase, int ncv,
-                                             bool hasprev, const char* ibase, bool more, const MrTile& Tn,
-                                             bool load_g, float (&gm)[2][4][2], int w, int l, long ostride)
-{
-    constexpr int NO = mr2_popc(MASK);
-    constexpr int OD = MASK & 1, OL = mr2_popc(MASK & 3);
-    unsigned sb[2];
-    float inv0;
-    mf_f4 yr[2][2], yi[2][2];
-    mr2_front(a, cur, nxt, twm, tws, ft, so, obase, ncv, hasprev, ibase, more, w, l, sb, inv0, yr, yi);
-    // stage 2 (table x data) of both k1 parities, held for every output
-    mf_f4 er[2], ei[2], orr[2], oi[2];
-#pragma unroll
-    for (int tk = 0; tk < 2; ++tk) {
+    for (int tn = 0; tn < 2; ++tn) {
         float2 v0[4], v1[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-            v0[e] = float2{yr[0][tk][e], yi[0][tk][e]};
-            v1[e] = float2{yr[1][tk][e], yi[1][tk][e]};
+            v0[e] = float2{ur[0][tn][e], ui[0][tn][e]};
+            v1[e] = float2{ur[1][tn][e], ui[1][tn][e]};
         }
-        mr_mfma_b<true>(er[tk], ei[tk], mr_table(ft, 0, l), mr_split4u(v0, a.fone));
-        mr_mfma_b<true>(orr[tk], oi[tk], mr_table(ft, 2, l), mr_split4u(v1, a.fone));
+        const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
+        mf_f4 dr, di, qr, qi;
+        mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(qr, qi, mr_table(ft, 2, l), B1);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                // row z = 2 j + tn + 32 (4 g + e + 16 h): E + O for h = 0, E - O for h = 1
+                const float yr = h ? (dr[e] - qr[e]) * fo : (dr[e] + qr[e]) * fo;
+                const float yi = h ? -(di[e] - qi[e]) * fo : -(di[e] + qi[e]) * fo;
+                float2* q = reinterpret_cast<float2*>(cur + sb[tn] + 2048 * e + 32768 * h);
+                if (RS == 0) {
+                    *q = float2{yr, yi};
+                } else {
+                    const float2 t = tsl[(8 * tn + 4 * h + e) * 64];
+                    const float2 y0 = *q;
+                    *q = float2{y0.x + yr * t.x + yi * t.y, y0.y + yi * t.x - yr * t.y};
+                }
+            }
     }
+}
+
+// after a non-last output: every column of cur holds it; wavefront w stores
+// its blocks w + 8 k to that output (ob: its tile's column 0)
+__device__ __forceinline__ void mr2p_fan_store(unsigned char* __restrict__ cur, const PencilArgs<float>& a,
+                                               const Mr2SliceOffs (&so)[2], char* ob, int ncvt, int w)
+{
+    mr2_lds_barrier();
+#pragma unroll
+    for (int k = 0; k < 8; ++k) mr2p_slice(cur, a, so, ob, ncvt, true, nullptr, false, w, k);
+}
+
+template <int OPK, bool WBAR, bool FIRST, bool LASTG>
+__device__ __forceinline__ void mr2p_fan_out(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                             unsigned char* __restrict__ nxt, const float2* __restrict__ twl,
+                                             const float2* __restrict__ tsl, const mf_h8* __restrict__ ft,
+                                             const MrTile& T, const Mr2SliceOffs (&so)[2], char* obase, int ncv,
+                                             bool hasprev, const char* ibase, bool more, const MrTile& Tn,
+                                             bool load_g, float (&gm)[2][2][4][2], const mf_f4 (&er)[2][2],
+                                             const mf_f4 (&ei)[2][2], const mf_f4 (&orr)[2][2],
+                                             const mf_f4 (&oi)[2][2], float inv0, const unsigned (&sb)[2], int w,
+                                             int l)
+{
+    float c0 = 0.0f;
+    if constexpr (OPK == OP_LAPL) c0 = mf_kcoef<OPK>(a, tile_k(a, T.col0 + w, T.slab, T.col0 + w < a.ncol), 0).c0;
+    const float kst = (float)(2.0 * kPi * a.dk[2]);
+    // the previous output's rows have been read from cur by every wavefront
+    if constexpr (WBAR) mr2_lds_barrier();
+    mr2p_inverse<OPK, 0, FIRST, LASTG>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv, hasprev, ibase, more, Tn,
+                                       load_g, gm[0], er[0], ei[0], orr[0], oi[0], inv0, c0, kst, sb, w, l);
+    mr2p_inverse<OPK, 1, FIRST, LASTG>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv, hasprev, ibase, more, Tn,
+                                       load_g, gm[1], er[1], ei[1], orr[1], oi[1], inv0, c0, kst, sb, w, l);
+    if (FIRST) mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 7);
+}
+
+template <int MASK>
+__device__ __forceinline__ void mr2p_fan_step(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
+                                              unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
+                                              const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
+                                              const MrTile& T, const Mr2SliceOffs (&so)[2], char* obase, int ncv,
+                                              bool hasprev, const char* ibase, bool more, const MrTile& Tn,
+                                              bool load_g, float (&gm)[2][2][4][2], int w, int l, long ostride)
+{
+    constexpr int NO = mr2_popc(MASK);
+    constexpr int OD = MASK & 1, OL = mr2_popc(MASK & 3);
+    const int g = l >> 4, j = l & 15;
+    const float2* twl = twm + l;
+    const float2* tsl = tws + l;
+    // staging offsets of rows z = 2 j + q + 32 (4 g + e + 16 h), column w
+    unsigned sb[2];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int r = 2 * (j & 7) + q;
+        sb[q] = (unsigned)(((j >> 3) + 8 * g) * 1024) + 16u * mr2_slot(r, w >> 1, g & 1) + (w & 1) * 8;
+    }
+    float2 x[2][2][4];
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                x[tb][h][e] = *reinterpret_cast<const float2*>(cur + sb[tb] + 2048 * e + 32768 * h);
+    mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 0);
+    float s0, inv0;
+    {
+        const float m0 = mr_absmax8(x[0][0], x[0][1]), m1 = mr_absmax8(x[1][0], x[1][1]);
+        mr_scale(2.0f * mr_wave_max(__builtin_fmaxf(m0, m1)), s0, inv0);
+    }
+    mf_f4 er[2][2], ei[2][2], orr[2][2], oi[2][2];
+    mr2p_forward<0>(a, x, s0, twl, tsl, ft, l, er[0], ei[0], orr[0], oi[0]);
+    mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 1);
+    mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 2);
+    mr2p_forward<1>(a, x, s0, twl, tsl, ft, l, er[1], ei[1], orr[1], oi[1]);
+    mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 3);
+    mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 4);
     char* ob = const_cast<char*>(mr_tile_base(a.out, a.oa, T));
     const int ncvt = a.ncol - T.col0;
     if constexpr ((MASK & 1) != 0) {
-        mr2_fan_out<OP_PLAIN, false, true, NO == 1>(a, cur, nxt, twm, tws, ft, T, so, obase, ncv, hasprev, ibase,
-                                                    more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w, l);
-        if constexpr (NO > 1) mr2_fan_store(cur, a, so, ob, ncvt, w);
+        mr2p_fan_out<OP_PLAIN, false, true, NO == 1>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv, hasprev, ibase,
+                                                     more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w, l);
+        if constexpr (NO > 1) mr2p_fan_store(cur, a, so, ob, ncvt, w);
     }
     if constexpr ((MASK & 2) != 0) {
-        mr2_fan_out<OP_GRAD, (OD > 0), (OD == 0), (OD == NO - 1)>(a, cur, nxt, twm, tws, ft, T, so, obase, ncv,
-                                                                  hasprev, ibase, more, Tn, load_g, gm, er, ei, orr,
-                                                                  oi, inv0, sb, w, l);
-        if constexpr (OD < NO - 1) mr2_fan_store(cur, a, so, ob + OD * ostride, ncvt, w);
+        mr2p_fan_out<OP_GRAD, (OD > 0), (OD == 0), (OD == NO - 1)>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv,
+                                                                   hasprev, ibase, more, Tn, load_g, gm, er, ei,
+                                                                   orr, oi, inv0, sb, w, l);
+        if constexpr (OD < NO - 1) mr2p_fan_store(cur, a, so, ob + OD * ostride, ncvt, w);
     }
     if constexpr ((MASK & 4) != 0)
-        mr2_fan_out<OP_LAPL, (OL > 0), (OL == 0), true>(a, cur, nxt, twm, tws, ft, T, so, obase, ncv, hasprev,
-                                                        ibase, more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w, l);
+        mr2p_fan_out<OP_LAPL, (OL > 0), (OL == 0), true>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv, hasprev,
+                                                         ibase, more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w,
+                                                         l);
 }
 
-// the fan-out kernel, launched as the one-output kernel with a.kmap = KMAP_Z
 template <int MASK>
-__global__ void __launch_bounds__(1024, 4) k_zmfma2048f(PencilArgs<float> a)
+__global__ void __launch_bounds__(512, 2) k_zmfma2048p(PencilArgs<float> a)
 {
     using M = MfmaZR2;
+    constexpr int NW = MfmaZR2P::NW;
     constexpr int NO = mr2_popc(MASK);
     constexpr int OPG = (MASK & 1) ? OP_PLAIN : (MASK & 2) ? OP_GRAD : OP_LAPL;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -682,34 +770,35 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma2048f(PencilArgs<float> a)
     const long ostride = a.oa.comp * 8;
     const long olast = (long)(NO - 1) * ostride;
 
-    {
-        const int slot = tid >> 6, j = tid & 15, g = (tid >> 4) & 3;
+    for (int i = tid; i < 1024; i += 64 * NW) {
+        const int slot = i >> 6, j = i & 15, g = (i >> 4) & 3;
         const int r = 2 * j + (slot >> 3), s = 8 * g + (slot & 7);
-        twm[tid] = a.plan.tw[(2 * r * s) & (M::L - 1)];
+        twm[i] = a.plan.tw[(2 * r * s) & (M::L - 1)];
         const int n = 2 * j + (slot >> 3) + 32 * (4 * g + (slot & 3) + 16 * ((slot >> 2) & 1));
-        tws[tid] = a.plan.tw[n];
+        tws[i] = a.plan.tw[n];
     }
-    if (tid < 3 * 4 * 64) {
-        const int ln = tid & 63, f = (tid >> 6) & 3, tb = tid >> 8;
+    for (int i = tid; i < 3 * 4 * 64; i += 64 * NW) {
+        const int ln = i & 63, f = (i >> 6) & 3, tb = i >> 8;
         const int fo = (f == 0) ? 0 : (f == 1) ? 2 : (f == 2) ? 1 : 3;
         ftw[(tb * 4 + fo) * 64 + ln] = reinterpret_cast<const mf_h8*>(a.mfrag16)[(tb * 64 + ln) * 4 + f];
     }
-    const Mr2SliceOffs so = mr2_slice_offs(a, w, l0);
+    const Mr2SliceOffs so[2] = {mr2_slice_offs(a, w, l0), mr2_slice_offs(a, w + 8, l0)};
     int u = blockIdx.x, m = 0;
     MrUnit U{};
     MrTile T{};
     const bool any = u < nunit && (U = mr_unit<M::P>(a, u, nunit), mr_tile(a, U, 0, T));
-    float gm[2][4][2];
+    float gm[2][2][4][2];
     if (any) {
         const char* ib = mr_tile_base(a.in, a.ia, T);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int n = w + 16 * k;
-            __builtin_amdgcn_global_load_lds(ib + mr2_zoff_u(a.ia, 16 * n) + so.ilane, (mr_lds_ptr)(stg0 + n * 1024),
-                                             16, 0, 0);
+        for (int k = 0; k < 8; ++k) {
+            const int n = w + 8 * k;
+            __builtin_amdgcn_global_load_lds(ib + mr2_zoff_u(a.ia, 16 * n) + so[k & 1].ilane,
+                                             (mr_lds_ptr)(stg0 + n * 1024), 16, 0, 0);
         }
-        const int kb0 = 4 * (l0 & 15) + 256 * (l0 >> 4) + (w >> 3);
-        mr2_kload<OPG>(gm, a, T, T.col0 + (w & 7), kb0, kb0);
+        const int kb0 = 4 * (l0 & 15) + 256 * (l0 >> 4);
+        mr2_kload<OPG>(gm[0], a, T, T.col0 + w, kb0, kb0);
+        mr2_kload<OPG>(gm[1], a, T, T.col0 + w, kb0 + 1, kb0 + 1);
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -733,8 +822,8 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma2048f(PencilArgs<float> a)
         char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp)) + olast;
         const char* ibase = mr_tile_base(a.in, a.ia, Tn);
         const int ncv = a.ncol - Tp.col0;
-        mr2_fan_step<MASK>(a, cur, nxt, twm, tws, ftw, T, so, obase, ncv, it > 0, ibase, more, Tn, load_g, gm, w, l,
-                           ostride);
+        mr2p_fan_step<MASK>(a, cur, nxt, twm, tws, ftw, T, so, obase, ncv, it > 0, ibase, more, Tn, load_g, gm, w, l,
+                            ostride);
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         Tp = T;
@@ -749,7 +838,7 @@ __global__ void __launch_bounds__(1024, 4) k_zmfma2048f(PencilArgs<float> a)
         char* obase = const_cast<char*>(mr_tile_base(a.out, a.oa, Tp)) + olast;
         const int ncv = a.ncol - Tp.col0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) mr2_slice(last, a, so, obase, ncv, true, nullptr, false, w, k);
+        for (int k = 0; k < 8; ++k) mr2p_slice(last, a, so, obase, ncv, true, nullptr, false, w, k);
     }
 }
 
