@@ -4,6 +4,7 @@
 #include "ps_mfma_reg.hpp"
 #include "ps_mfma_reg2048.hpp"
 #include "ps_mfma_reg512.hpp"
+#include "ps_mfma_reg3.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -179,9 +180,32 @@ bool zmfma3_supported(int L, int half, int op)
     return L == MfmaZ3::L && half && (op == OP_DIV || op == OP_CURL);
 }
 
-hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s)
+// register-pencil form: workgroups of 8 wavefronts (one pencil column and
+// its three components each, 2 per SIMD), one per CU at a time, four per CU
+// over the launch
+template <int OPK> static hipError_t launch_zmfma3r_op(PencilArgs<float> a, hipStream_t s)
+{
+    const size_t shm = MfmaZR3::lds_bytes();
+    hipError_t e = allow_lds(k_zmfma1024c<OPK>, shm);
+    if (e != hipSuccess) return e;
+    a.ncomp = 1;
+    a.ncoltile = (a.ncol + MfmaZR3::P - 1) / MfmaZR3::P;
+    if ((long)a.ncoltile * a.nslab >= (1L << 31)) return hipErrorInvalidValue;
+    const long nunit = (long)a.ncoltile * ((a.nslab + 1) / 2);
+    const long ncu = std::max(device_cus(), 8);
+    const long nwg = std::min<long>(nunit, 4 * ncu);
+    hipLaunchKernelGGL((k_zmfma1024c<OPK>), dim3((unsigned)nwg), dim3(64 * MfmaZR3::NW), shm, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_zmfma3(const PencilArgs<float>& a, int variant, hipStream_t s)
 {
     if (a.plan.L != MfmaZ3::L || a.n_in != MfmaZ3::L / 2 || a.n_out != MfmaZ3::L / 2) return hipErrorInvalidValue;
+    if (variant == 1 && zmfma_reg_ok(a)) {
+        if (a.op == OP_DIV) return launch_zmfma3r_op<OP_DIV>(a, s);
+        if (a.op == OP_CURL) return launch_zmfma3r_op<OP_CURL>(a, s);
+        return hipErrorInvalidValue;
+    }
     if (a.op == OP_DIV) return launch_zmfma3_op<OP_DIV>(a, s);
     if (a.op == OP_CURL) return launch_zmfma3_op<OP_CURL>(a, s);
     return hipErrorInvalidValue;

@@ -628,7 +628,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
                 CK(launch_zmfma(zz, c->tune.mfma, s));
                 mfma_done = true;
             } else if (c->zmfma && nc == 3 && zz.kmap == KMAP_3D && zmfma3_supported(zp.p.L, c->bc[2] == 0, zz.op)) {
-                CK(launch_zmfma3(zz, s));
+                CK(launch_zmfma3(zz, c->tune.mfma, s));
                 mfma_done = true;
             }
         }

@@ -33,9 +33,11 @@ bool zmfma_fan_ok(const PencilArgs<float>& a, int mask, int variant);
 hipError_t launch_zmfma_fan(const PencilArgs<float>& a, int mask, hipStream_t s);
 
 // three-component operators (div, curl) of the fused z pass on the matrix
-// cores: inputs are components 0..2 of `a.in`, outputs components of `a.out`
+// cores: inputs are components 0..2 of `a.in`, outputs components of `a.out`;
+// variant 1 the register-pencil kernel when the arguments allow it (as
+// above), else one tile of 4 pencils per workgroup
 bool zmfma3_supported(int L, int half, int op);
-hipError_t launch_zmfma3(const PencilArgs<float>& a, hipStream_t s);
+hipError_t launch_zmfma3(const PencilArgs<float>& a, int variant, hipStream_t s);
 
 // per-lane fp16 hi/lo fragments of the 32-point DFT matrix that the
 // matrix-core kernels multiply by (binary16 bit patterns, kZmfmaFragHalves

@@ -117,11 +117,29 @@ def test_mixed_other_lengths_use_fp32_kernels():
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "%s-%s" % (c[1], "x".join(map(str, c[0]))))
 @pytest.mark.parametrize("op", ["curl", "div"])
-def test_mixed_three_component_operators(case, op):
-    # curl and div: the fused three-component matrix-core z kernel
+@pytest.mark.parametrize("form", FORMS)
+def test_mixed_three_component_operators(case, op, form, monkeypatch):
+    # curl and div: the fused three-component matrix-core z kernels, the
+    # register-pencil one (1: one wavefront per pencil column and its three
+    # components) and the 4-pencil one-tile one (2)
+    monkeypatch.setenv("PS_TUNE_MFMA", form)
     n, bc, dx = case
-    announce("mixed", op, n, bc)
+    announce("mixed", op, n, bc, "form", form)
     b = random_rhs(n, 41)
+    _compare(n, bc, dx, 2, op, b)
+
+
+@pytest.mark.parametrize("zblk", ["-1", "0", "3", "5"], ids=lambda z: "zblk" + z)
+@pytest.mark.parametrize("op", ["curl", "div"])
+def test_mixed_three_component_every_layout(zblk, op, monkeypatch):
+    """The register-pencil three-component kernel in each z layout of the
+    spectrum buffer, on components of magnitudes 2^-30, 1 and 2^30 (each
+    component's input scale enters the k-space combination)."""
+    monkeypatch.setenv("PS_TUNE_ZBLK", zblk)
+    n, bc, dx = (10, 8, 512), (0, 0, 0), (0.1, 0.12, 2 / 512)
+    announce("mixed three-component layout", op, n, zblk)
+    rng = np.random.default_rng(53)
+    b = [rng.standard_normal((n[2], n[1], n[0])) * 2.0 ** sc for sc in (-30, 0, 30)]
     _compare(n, bc, dx, 2, op, b)
 
 
