@@ -449,6 +449,11 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
     const int g = l >> 4, j = l & 15;
     const int p = w, kb = 2 * j + 128 * g;
     mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 0);
+    // the 16-point DFT table (8 of the 16 products) held for the tile where
+    // the registers allow (the filtered rhs's multipliers take them)
+    constexpr bool hold16 = OPK != OP_RHSF;
+    MfT16 F16{};
+    if constexpr (hold16) F16 = mr_table(ft, 0, l);
     // staging offsets of rows z = 2 j + q + 32 (4 g + e) (q = 0, 1; 128 g + 32 e
     // is a whole number of blocks): block (j >> 2) + 16 g + 4 e, row
     // 2 (j & 3) + q, n & 1 = (j >> 2) & 1, (n >> 1) & 1 = (j >> 3) & 1
@@ -471,7 +476,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
     // yr/yi[tb][tk]: rows b = 2 (4 g + e) + tb, column k1 = 2 j + tk
     mf_f4 yr[2][2], yi[2][2];
     {
-        const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
+        const MfT16 Fe = hold16 ? F16 : mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) {
             const MfFrag X = mr_split4s<false>(x[tb], s0, a.fzero);
@@ -512,7 +517,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         }
         const MfFrag B0 = mr_split4u(v0, a.fone), B1 = mr_split4u(v1, a.fone);
         mf_f4 er, ei, orr, oi;
-        mr_mfma_b<true>(er, ei, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(er, ei, hold16 ? F16 : mr_table(ft, 0, l), B0);
         mr_mfma_b<true>(orr, oi, mr_table(ft, 2, l), B1);
         // X[kz] (kz = k1 + 32 k2, k2 = 4 g + e and + 16) and the k-space operator
         // the k-space operator and inverse stage 1's butterflies in one step:
@@ -542,7 +547,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
             const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
-            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, hold16 ? F16 : mr_table(ft, 0, l));
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
         if (tk == 0) mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 3);
@@ -575,7 +580,7 @@ __device__ __forceinline__ void mr_step(const PencilArgs<float>& a, unsigned cha
         }
         const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
         mf_f4 dr, di;
-        mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(dr, di, hold16 ? F16 : mr_table(ft, 0, l), B0);
         mr_mfma_b<false>(dr, di, mr_table(ft, 2, l), B1);
         // rows z = 32 (4 g + e) + 2 j + tn
 #pragma unroll

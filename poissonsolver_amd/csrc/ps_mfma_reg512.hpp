@@ -100,8 +100,8 @@ __device__ __forceinline__ float mr5_kmul(const MfKCoef& kc, const float (&gr)[2
 __device__ __forceinline__ void mr5_front(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
                                           unsigned char* __restrict__ nxt, const float2* __restrict__ tw1,
                                           const mf_h8* __restrict__ ft, const MrSliceOffs& so, char* obase, int nv,
-                                          bool hasprev, const char* ibase, bool more, int w, int l, unsigned (&sb)[4],
-                                          float& inv0, mf_f4 (&yr)[2], mf_f4 (&yi)[2])
+                                          bool hasprev, const char* ibase, bool more, int w, int l, const MfT16& F16,
+                                          unsigned (&sb)[4], float& inv0, mf_f4 (&yr)[2], mf_f4 (&yi)[2])
 {
     const int g = l >> 4, j = l & 15;
     const int p = w;
@@ -121,7 +121,7 @@ __device__ __forceinline__ void mr5_front(const PencilArgs<float>& a, unsigned c
     // ---- stage 1 (data x table): Y[b][k1], rows b = 4 g + e, column k1 = 2 j + tk ----
     {
         const MfFrag X = mr_split4s<false>(x, s0, a.fzero);
-        mr_mfma_a<true>(yr[0], yi[0], X, mr_table(ft, 0, l));
+        mr_mfma_a<true>(yr[0], yi[0], X, F16);
         mr_mfma_a<true>(yr[1], yi[1], X, mr_table(ft, 1, l));
     }
     mr_slice(nxt, so, obase, nv, hasprev, ibase, more, w, 1);
@@ -156,7 +156,9 @@ __device__ __forceinline__ void mr5_step(const PencilArgs<float>& a, unsigned ch
     unsigned sb[4];
     float inv0;
     mf_f4 yr[2], yi[2];
-    mr5_front(a, cur, nxt, tw1, ft, so, obase, nv, hasprev, ibase, more, w, l, sb, inv0, yr, yi);
+    // the 16-point DFT table (6 of the 8 products) held for the tile
+    const MfT16 F16 = mr_table(ft, 0, l);
+    mr5_front(a, cur, nxt, tw1, ft, so, obase, nv, hasprev, ibase, more, w, l, F16, sb, inv0, yr, yi);
     const float2* t2 = tw2 + l;
     mf_f4 ur[2], ui[2];
     float s2[2], inv2[2];
@@ -172,7 +174,7 @@ __device__ __forceinline__ void mr5_step(const PencilArgs<float>& a, unsigned ch
         for (int e = 0; e < 4; ++e) v[e] = float2{yr[tk][e], yi[tk][e]};
         const MfFrag B = mr_split4u(v, a.fone);
         mf_f4 xr, xi;
-        mr_mfma_b<true>(xr, xi, mr_table(ft, 0, l), B);
+        mr_mfma_b<true>(xr, xi, F16, B);
         float2 z[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -187,7 +189,7 @@ __device__ __forceinline__ void mr5_step(const PencilArgs<float>& a, unsigned ch
         mr_scale(mr_wave_max(mr5_absmax4(z)), s2[tk], inv2[tk]);
         {
             const MfFrag Z = mr_split4s<true>(z, s2[tk], a.fzero);
-            mr_mfma_a<true>(ur[tk], ui[tk], Z, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk], ui[tk], Z, F16);
         }
         // T = W512^(n2 k1) V: slot 4 tk + e of lane (j, g) holds W512^(j (2 (4 g + e) + tk))
 #pragma unroll
@@ -212,7 +214,7 @@ __device__ __forceinline__ void mr5_step(const PencilArgs<float>& a, unsigned ch
         }
         const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
         mf_f4 dr, di;
-        mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(dr, di, F16, B0);
         mr_mfma_b<false>(dr, di, mr_table(ft, 2, l), B1);
         // rows z = j + 16 (4 g + e)
 #pragma unroll
@@ -329,7 +331,7 @@ __device__ __forceinline__ void mr5_fan_out(const PencilArgs<float>& a, unsigned
                                             const float2* __restrict__ tw2, const mf_h8* __restrict__ ft,
                                             const MrTile& T, const MrTile& Tn, bool load_g, float (&gm)[2][4],
                                             const mf_f4 (&xr)[2], const mf_f4 (&xi)[2], float inv0,
-                                            const unsigned (&sb)[4], int p, int l)
+                                            const unsigned (&sb)[4], int p, int l, const MfT16& F16)
 {
     constexpr bool mimag = mf_imag_multiplier<OPK>();
     const int g = l >> 4, j = l & 15, kb = 2 * j + 128 * g;
@@ -355,7 +357,7 @@ __device__ __forceinline__ void mr5_fan_out(const PencilArgs<float>& a, unsigned
         mr_scale(mr_wave_max(mr5_absmax4(z)), s2[tk], inv2[tk]);
         {
             const MfFrag Z = mr_split4s<true>(z, s2[tk], a.fzero);
-            mr_mfma_a<true>(ur[tk], ui[tk], Z, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk], ui[tk], Z, F16);
         }
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
@@ -375,7 +377,7 @@ __device__ __forceinline__ void mr5_fan_out(const PencilArgs<float>& a, unsigned
     }
     const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
     mf_f4 dr, di;
-    mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
+    mr_mfma_b<true>(dr, di, F16, B0);
     mr_mfma_b<false>(dr, di, mr_table(ft, 2, l), B1);
     if constexpr (WBAR) mr5_lds_barrier();
 #pragma unroll
@@ -405,7 +407,8 @@ __device__ __forceinline__ void mr5_fan_step(const PencilArgs<float>& a, unsigne
     unsigned sb[4];
     float inv0;
     mf_f4 yr[2], yi[2];
-    mr5_front(a, cur, nxt, tw1, ft, so, obase, nv, hasprev, ibase, more, w, l, sb, inv0, yr, yi);
+    const MfT16 F16 = mr_table(ft, 0, l);
+    mr5_front(a, cur, nxt, tw1, ft, so, obase, nv, hasprev, ibase, more, w, l, F16, sb, inv0, yr, yi);
     // stage 2 of both k1 parities, held for every output
     mf_f4 xr[2], xi[2];
 #pragma unroll
@@ -413,20 +416,21 @@ __device__ __forceinline__ void mr5_fan_step(const PencilArgs<float>& a, unsigne
         float2 v[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) v[e] = float2{yr[tk][e], yi[tk][e]};
-        mr_mfma_b<true>(xr[tk], xi[tk], mr_table(ft, 0, l), mr_split4u(v, a.fone));
+        mr_mfma_b<true>(xr[tk], xi[tk], F16, mr_split4u(v, a.fone));
     }
     char* ob = const_cast<char*>(mr_tile_base(a.out, a.oa, T));
     const int nvt = a.ncol - T.col0 - 2 * (l & 7);
     if constexpr ((MASK & 1) != 0) {
-        mr5_fan_out<OP_PLAIN, false, NO == 1>(a, cur, tw2, ft, T, Tn, load_g, gm, xr, xi, inv0, sb, w, l);
+        mr5_fan_out<OP_PLAIN, false, NO == 1>(a, cur, tw2, ft, T, Tn, load_g, gm, xr, xi, inv0, sb, w, l, F16);
         if constexpr (NO > 1) mr5_fan_store(cur, so, ob, nvt, w);
     }
     if constexpr ((MASK & 2) != 0) {
-        mr5_fan_out<OP_GRAD, (OD > 0), (OD == NO - 1)>(a, cur, tw2, ft, T, Tn, load_g, gm, xr, xi, inv0, sb, w, l);
+        mr5_fan_out<OP_GRAD, (OD > 0), (OD == NO - 1)>(a, cur, tw2, ft, T, Tn, load_g, gm, xr, xi, inv0, sb, w, l,
+                                                       F16);
         if constexpr (OD < NO - 1) mr5_fan_store(cur, so, ob + OD * ostride, nvt, w);
     }
     if constexpr ((MASK & 4) != 0)
-        mr5_fan_out<OP_LAPL, (OL > 0), true>(a, cur, tw2, ft, T, Tn, load_g, gm, xr, xi, inv0, sb, w, l);
+        mr5_fan_out<OP_LAPL, (OL > 0), true>(a, cur, tw2, ft, T, Tn, load_g, gm, xr, xi, inv0, sb, w, l, F16);
 }
 
 // the fan-out kernel: launched as the one-output kernel (persistent, slab
