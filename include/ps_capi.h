@@ -173,6 +173,11 @@ int ps_pull_potential_device(ps_ctx* ctx, void* A, void* dAdx, void* dAdy, void*
    distinct z multiplier (A, dA/dz, Laplacian); -PS_ESTATE before setup3d,
    -PS_EINVAL for a 2D context or a mask outside 1..31 */
 int ps_potential_zlaunches(const ps_ctx* ctx, int outputs);
+/* number of y^-1 launches a potential solve of `outputs` makes: one per
+   output, except that A, dA/dx and dA/dy (two or more of them) run as one
+   from one load of A's z-pass output in the FP32 and mixed forms where the
+   y length has a specialised kernel (tuning field zfan); negative as above */
+int ps_potential_ylaunches(const ps_ctx* ctx, int outputs);
 /* `iters` potential solves timed per pass as ps_profile_solve */
 int ps_profile_potential(ps_ctx* ctx, int outputs, int iters, double ms[6]);
 
@@ -260,10 +265,12 @@ typedef struct ps_tuning {
     int t2d;          /* 2D contexts whose fused y pass holds one or two columns per workgroup
                          (long pencils): the x pass output is transposed so that y pencils are
                          contiguous, and grad / curl transform their scalar once (PS_TUNE_T2D, 1) */
-    int zfan;         /* PS_MIXED potential form (ps_solve3d_potential): 1 runs its z outputs (A,
-                         d/dz A, Laplacian) from one forward transform of the scalar where the
-                         matrix-core kernel has that form (z transforms of 512 and 2048, mfma 1);
-                         0 one z launch per output (PS_TUNE_ZFAN, 1) */
+    int zfan;         /* potential form (ps_solve3d_potential): 1 runs its z outputs (A, d/dz A,
+                         Laplacian) from one forward transform of the scalar where the
+                         matrix-core kernel has that form (PS_MIXED; z transforms of 512 and
+                         2048, mfma 1), and the y^-1 passes of A, d/dx A and d/dy A from one
+                         load of A's z output (FP32 and PS_MIXED, specialised y lengths); 0 one
+                         launch per output (PS_TUNE_ZFAN, 1) */
 } ps_tuning;
 /* the defaults (with PS_TUNE_* overrides applied) */
 int ps_tuning_default(ps_tuning* t);

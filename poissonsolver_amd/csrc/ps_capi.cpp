@@ -400,6 +400,17 @@ int select_op(ps_ctx* c, OpSel* s)
     return PS_OK;
 }
 
+// the outputs among A, d/dx A, d/dy A (bits 0..2 of the potential mask)
+// whose y^-1 passes run as one launch from one load of A's z output: at
+// least two, FP32 storage (FP32 and mixed forms), the specialised y kernel,
+// tuning zfan; 0 when they run one launch each
+int pot_yfan_mask(const ps_ctx* c, int outputs)
+{
+    const int m = outputs & 7;
+    if (c->prec == PS_FP64 || !c->py.stat || !c->tune.zfan || c->dim2) return 0;
+    return __builtin_popcount((unsigned)m) >= 2 ? m : 0;
+}
+
 template <typename T>
 void fill_zargs(ps_ctx* c, PencilArgs<T>& a, const DevPlan& d, void* A1, void* A2)
 {
@@ -703,8 +714,20 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         yi.Mx = c->M[0];
         yi.My = c->M[1];
         for (int d = 0; d < 2; ++d) yi.dk[d] = 1.0 / ((double)c->M[d] * c->dx[d]);
+        // A, d/dx A and d/dy A from one load of A's z output where the FP32
+        // specialised kernel has that form (their slots are consecutive)
+        const int yfan = pot_yfan_mask(c, kmap);
+        if (yfan) {
+            PencilArgs<T> y1 = yi;
+            y1.ncomp = 1;
+            y1.ik_in = 0;
+            y1.ik_fan = yfan;
+            y1.in = A2o + (size_t)zslot[0] * c->A2c * cbytes;
+            y1.out = static_cast<char*>(yi.out) + (size_t)slot[__builtin_ctz((unsigned)yfan)] * yi.oa.comp * cbytes;
+            CK(static_launch_pencil<T>(c->py.p.L, y1, 1, SM_INVFAN, c->bc[1] == 0, s));
+        }
         for (int qn = 0; qn < 5; ++qn) {
-            if (slot[qn] < 0) continue;
+            if (slot[qn] < 0 || ((yfan >> qn) & 1)) continue;
             PencilArgs<T> y1 = yi;
             y1.ncomp = 1;
             y1.ik_in = (qn == 1) ? 1 : (qn == 2) ? 2 : 0;
@@ -1464,6 +1487,14 @@ int ps_potential_zlaunches(const ps_ctx* c, int outputs)
         if (zmfma_fan_ok(za, zmask, c->tune.mfma)) return 1;
     }
     return nzq;
+}
+
+int ps_potential_ylaunches(const ps_ctx* c, int outputs)
+{
+    if (!c || !c->ready) return -PS_ESTATE;
+    if (c->dim2 || outputs < 1 || outputs > 31) return -PS_EINVAL;
+    const int yfan = pot_yfan_mask(c, outputs);
+    return __builtin_popcount((unsigned)outputs) - (yfan ? __builtin_popcount((unsigned)yfan) - 1 : 0);
 }
 
 int ps_solve3d_potential(ps_ctx* c, int outputs)

@@ -86,6 +86,10 @@ template <typename T> struct PencilArgs {
     // Hermitian part, 0 at the Nyquist index) before the transform; 1: k = kx
     // of the column (Mx, dk[0]), 2: k = ky of the position (My, dk[1]); 0: none
     int ik_in;
+    // y^-1 pass of one spectrum into several outputs (SM_INVFAN): bit m set
+    // for an output with ik_in mode m (0 none, 1 i kx, 2 i ky), written to
+    // consecutive components of the output addressing in m order
+    int ik_fan;
     // matrix-core z pass: per-lane fp16 hi/lo fragments of the 32-point DFT
     // matrix (64 lanes x 8 fragments x 8 halves), built once at setup
     const void* mfrag;

@@ -73,12 +73,14 @@ __device__ __forceinline__ TileK<T> tile_k(const PencilArgs<T>& a, int col, int 
     return k;
 }
 
-// PencilArgs::ik_in on one thread's loaded values v[u] (positions t + TP u)
+// ik_in mode `mode` (PencilArgs::ik_in) on one thread's loaded values v[u]
+// (positions t + TP u)
 template <typename T, int E>
-__device__ __forceinline__ void apply_ik_in(cplx<T> (&v)[E], const PencilArgs<T>& a, int col, int t, int TP)
+__device__ __forceinline__ void apply_ik_mode(cplx<T> (&v)[E], const PencilArgs<T>& a, int mode, int col, int t,
+                                              int TP)
 {
-    if (a.ik_in == 0) return;
-    if (a.ik_in == 1) {
+    if (mode == 0) return;
+    if (mode == 1) {
         const int kx = col + a.col0;
         const T k = (2 * kx == a.Mx) ? T(0) : (T)kappa_full(kx, a.Mx, a.dk[0]);
 #pragma unroll
@@ -91,6 +93,12 @@ __device__ __forceinline__ void apply_ik_in(cplx<T> (&v)[E], const PencilArgs<T>
             v[u] = cplx<T>{-k * v[u].y, k * v[u].x};
         }
     }
+}
+
+template <typename T, int E>
+__device__ __forceinline__ void apply_ik_in(cplx<T> (&v)[E], const PencilArgs<T>& a, int col, int t, int TP)
+{
+    apply_ik_mode<T, E>(v, a, a.ik_in, col, t, TP);
 }
 
 // Signed wavenumber of vector component c under map `kmap` (KMap), from the

@@ -205,14 +205,41 @@ __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
             }
         }
     }
-    if constexpr (MODE != SM_INV) {
+    if constexpr (MODE == SM_FWD || MODE == SM_FUSED) {
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             if constexpr (TWREG) sfft<L, E, B, -1, T>(v[c], W, my, t);
             else sfft_g<L, E, B, -1, T>(v[c], a.plan.tw, my, t);
         }
-    } else if constexpr (NC == 1) {
+    } else if constexpr (NC == 1 && MODE == SM_INV) {
         apply_ik_in<T, E>(v[0], a, col, t, TP);
+    }
+
+    if constexpr (MODE == SM_INVFAN) {
+        // one loaded spectrum, outputs for the set bits m of a.ik_fan (ik
+        // mode m applied to a copy, then the inverse transform), written to
+        // consecutive components of the output addressing
+        int slot = 0;
+#pragma unroll
+        for (int m = 0; m < 3; ++m) {
+            if (!((a.ik_fan >> m) & 1)) continue;
+            C w[E];
+#pragma unroll
+            for (int u = 0; u < E; ++u) w[u] = v[0][u];
+            apply_ik_mode<T, E>(w, a, m, col, t, TP);
+            if constexpr (TWREG) sfft<L, E, B, +1, T>(w, W, my, t);
+            else sfft_g<L, E, B, +1, T>(w, a.plan.tw, my, t);
+            C* op_ = reinterpret_cast<C*>(a.out) + slot * a.oa.comp + col * a.oa.col +
+                     boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi) +
+                     boff(t, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
+            if (cv) {
+#pragma unroll
+                for (int u = 0; u < (HOUT ? E / 2 : E); ++u)
+                    if (HOUT || t + TP * u < a.n_out) op_[u * ou] = w[u];
+            }
+            ++slot;
+        }
+        return;
     }
 
     const int nout = (NC == 1 || a.op == OP_DIV) ? 1 : 3;
@@ -620,6 +647,16 @@ hipError_t s_launch_pencil(const PencilArgs<T>& a, int nc, int mode, int half, h
     if (mode == SM_INV) {
         if (nc != 1) return hipErrorInvalidValue;
         return half ? s_launch_pencil_one<T, L, 1, SM_INV, 0, 1>(a, s) : s_launch_pencil_one<T, L, 1, SM_INV, 0, 0>(a, s);
+    }
+    if (mode == SM_INVFAN) {
+        // FP32 only; stage twiddles from the table (the loaded spectrum and a
+        // working copy take the registers)
+        if constexpr (sizeof(T) == 4) {
+            if (nc != 1 || a.ik_fan < 1 || a.ik_fan > 7) return hipErrorInvalidValue;
+            return half ? s_launch_pencil_tw<T, L, 1, SM_INVFAN, 0, 1, 0, 2>(a, s)
+                        : s_launch_pencil_tw<T, L, 1, SM_INVFAN, 0, 0, 0, 2>(a, s);
+        }
+        return hipErrorInvalidValue;
     }
     if (nc == 1 && a.zwave) return s_launch_zwave<T, L>(a, half, s);
     if (nc == 1)

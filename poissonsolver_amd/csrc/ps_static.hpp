@@ -11,7 +11,9 @@
 
 namespace psamd {
 
-enum StaticMode : int { SM_FWD = 0, SM_INV = 1, SM_FUSED = 2 };
+// SM_INVFAN: the y^-1 pass of one loaded spectrum into up to three outputs
+// (PencilArgs::ik_fan; FP32)
+enum StaticMode : int { SM_FWD = 0, SM_INV = 1, SM_FUSED = 2, SM_INVFAN = 3 };
 
 constexpr int s_lds_len(int L) { return L + (L >> 5) + 1; }
 

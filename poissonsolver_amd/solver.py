@@ -219,6 +219,14 @@ class poisson_solver:
             mask |= 1 << self.POT_NAMES.index(name)
         return self._lib.ps_potential_zlaunches(self._ctx, mask)
 
+    def potential_ylaunches(self, outputs=("A", "dx", "dy", "dz")) -> int:
+        """y^-1 launches a potential solve of these outputs makes (A, dA/dx
+        and dA/dy may share one); negative before setup3d."""
+        mask = 0
+        for name in outputs:
+            mask |= 1 << self.POT_NAMES.index(name)
+        return self._lib.ps_potential_ylaunches(self._ctx, mask)
+
     def pull_potential(self, A=None, dx=None, dy=None, dz=None, lap=None):
         arrs = [self._host(a, True) for a in (A, dx, dy, dz, lap)]
         self._report(self._lib.ps_pull_potential(self._ctx, *[_ptr(a) for a in arrs]), "pull_potential")
