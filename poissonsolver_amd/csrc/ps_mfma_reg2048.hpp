@@ -520,12 +520,13 @@ __device__ __forceinline__ void mr2p_slice(unsigned char* __restrict__ buf, cons
 template <int RS>
 __device__ __forceinline__ void mr2p_forward(const PencilArgs<float>& a, const float2 (&x)[2][2][4], float s0,
                                              const float2* __restrict__ twl, const float2* __restrict__ tsl,
-                                             const mf_h8* __restrict__ ft, int l, mf_f4 (&er)[2], mf_f4 (&ei)[2],
-                                             mf_f4 (&orr)[2], mf_f4 (&oi)[2])
+                                             const mf_h8* __restrict__ ft, const MfT16& F16, int l, mf_f4 (&er)[2],
+                                             mf_f4 (&ei)[2], mf_f4 (&orr)[2], mf_f4 (&oi)[2])
 {
     mf_f4 yr[2][2], yi[2][2];
     {
-        const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
+        const MfT16& Fe = F16;
+        const MfT16 Fo = mr_table(ft, 1, l);
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) {
             float2 xa[4], xd[4];
@@ -562,7 +563,7 @@ __device__ __forceinline__ void mr2p_forward(const PencilArgs<float>& a, const f
             v0[e] = float2{yr[0][tk][e], yi[0][tk][e]};
             v1[e] = float2{yr[1][tk][e], yi[1][tk][e]};
         }
-        mr_mfma_b<true>(er[tk], ei[tk], mr_table(ft, 0, l), mr_split4u(v0, a.fone));
+        mr_mfma_b<true>(er[tk], ei[tk], F16, mr_split4u(v0, a.fone));
         mr_mfma_b<true>(orr[tk], oi[tk], mr_table(ft, 2, l), mr_split4u(v1, a.fone));
     }
 }
@@ -580,7 +581,8 @@ __device__ __forceinline__ void mr2p_inverse(const PencilArgs<float>& a, unsigne
                                              bool hasprev, const char* ibase, bool more, const MrTile& Tn,
                                              bool load_g, float (&gr)[2][4][2], const mf_f4 (&er)[2],
                                              const mf_f4 (&ei)[2], const mf_f4 (&orr)[2], const mf_f4 (&oi)[2],
-                                             float inv0, float c0, float kst, const unsigned (&sb)[2], int w, int l)
+                                             float inv0, float c0, float kst, const unsigned (&sb)[2], int w, int l,
+                                             const MfT16& F16)
 {
     constexpr bool mimag = mf_imag_multiplier<OPK>();
     const int g = l >> 4, j = l & 15;
@@ -610,7 +612,7 @@ __device__ __forceinline__ void mr2p_inverse(const PencilArgs<float>& a, unsigne
         mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
             const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
-            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, F16);
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
         if (FIRST && tk == 0) mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 5 + RS);
@@ -636,7 +638,7 @@ __device__ __forceinline__ void mr2p_inverse(const PencilArgs<float>& a, unsigne
         }
         const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
         mf_f4 dr, di, qr, qi;
-        mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(dr, di, F16, B0);
         mr_mfma_b<true>(qr, qi, mr_table(ft, 2, l), B1);
 #pragma unroll
         for (int h = 0; h < 2; ++h)
@@ -676,7 +678,7 @@ __device__ __forceinline__ void mr2p_fan_out(const PencilArgs<float>& a, unsigne
                                              bool load_g, float (&gm)[2][2][4][2], const mf_f4 (&er)[2][2],
                                              const mf_f4 (&ei)[2][2], const mf_f4 (&orr)[2][2],
                                              const mf_f4 (&oi)[2][2], float inv0, const unsigned (&sb)[2], int w,
-                                             int l)
+                                             int l, const MfT16& F16)
 {
     float c0 = 0.0f;
     if constexpr (OPK == OP_LAPL) c0 = mf_kcoef<OPK>(a, tile_k(a, T.col0 + w, T.slab, T.col0 + w < a.ncol), 0).c0;
@@ -684,9 +686,9 @@ __device__ __forceinline__ void mr2p_fan_out(const PencilArgs<float>& a, unsigne
     // the previous output's rows have been read from cur by every wavefront
     if constexpr (WBAR) mr2_lds_barrier();
     mr2p_inverse<OPK, 0, FIRST, LASTG>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv, hasprev, ibase, more, Tn,
-                                       load_g, gm[0], er[0], ei[0], orr[0], oi[0], inv0, c0, kst, sb, w, l);
+                                       load_g, gm[0], er[0], ei[0], orr[0], oi[0], inv0, c0, kst, sb, w, l, F16);
     mr2p_inverse<OPK, 1, FIRST, LASTG>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv, hasprev, ibase, more, Tn,
-                                       load_g, gm[1], er[1], ei[1], orr[1], oi[1], inv0, c0, kst, sb, w, l);
+                                       load_g, gm[1], er[1], ei[1], orr[1], oi[1], inv0, c0, kst, sb, w, l, F16);
     if (FIRST) mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 7);
 }
 
@@ -725,29 +727,31 @@ __device__ __forceinline__ void mr2p_fan_step(const PencilArgs<float>& a, unsign
         mr_scale(2.0f * mr_wave_max(__builtin_fmaxf(m0, m1)), s0, inv0);
     }
     mf_f4 er[2][2], ei[2][2], orr[2][2], oi[2][2];
-    mr2p_forward<0>(a, x, s0, twl, tsl, ft, l, er[0], ei[0], orr[0], oi[0]);
+    // the 16-point DFT table (8 of each residue's 16 products) held for the tile
+    const MfT16 F16 = mr_table(ft, 0, l);
+    mr2p_forward<0>(a, x, s0, twl, tsl, ft, F16, l, er[0], ei[0], orr[0], oi[0]);
     mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 1);
     mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 2);
-    mr2p_forward<1>(a, x, s0, twl, tsl, ft, l, er[1], ei[1], orr[1], oi[1]);
+    mr2p_forward<1>(a, x, s0, twl, tsl, ft, F16, l, er[1], ei[1], orr[1], oi[1]);
     mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 3);
     mr2p_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 4);
     char* ob = const_cast<char*>(mr_tile_base(a.out, a.oa, T));
     const int ncvt = a.ncol - T.col0;
     if constexpr ((MASK & 1) != 0) {
         mr2p_fan_out<OP_PLAIN, false, true, NO == 1>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv, hasprev, ibase,
-                                                     more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w, l);
+                                                     more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w, l, F16);
         if constexpr (NO > 1) mr2p_fan_store(cur, a, so, ob, ncvt, w);
     }
     if constexpr ((MASK & 2) != 0) {
         mr2p_fan_out<OP_GRAD, (OD > 0), (OD == 0), (OD == NO - 1)>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv,
                                                                    hasprev, ibase, more, Tn, load_g, gm, er, ei,
-                                                                   orr, oi, inv0, sb, w, l);
+                                                                   orr, oi, inv0, sb, w, l, F16);
         if constexpr (OD < NO - 1) mr2p_fan_store(cur, a, so, ob + OD * ostride, ncvt, w);
     }
     if constexpr ((MASK & 4) != 0)
         mr2p_fan_out<OP_LAPL, (OL > 0), (OL == 0), true>(a, cur, nxt, twl, tsl, ft, T, so, obase, ncv, hasprev,
                                                          ibase, more, Tn, load_g, gm, er, ei, orr, oi, inv0, sb, w,
-                                                         l);
+                                                         l, F16);
 }
 
 template <int MASK>
