@@ -182,12 +182,13 @@ __device__ __forceinline__ void mr2_combine(unsigned char* __restrict__ cur, uns
 // residue 1's W2048^n twist, the pencil's scale, stage 1, slice 1 and the
 // W1024^(b k1) twiddle; returns the rows' staging offsets sb, the input
 // scale's inverse and the twiddled stage-1 result Y.
+template <bool H16>
 __device__ __forceinline__ void mr2_front(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
                                           unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
                                           const float2* __restrict__ tws, const mf_h8* __restrict__ ft,
                                           const Mr2SliceOffs& so, char* obase, int ncv, bool hasprev,
-                                          const char* ibase, bool more, int w, int l, unsigned (&sb)[2], float& inv0,
-                                          mf_f4 (&yr)[2][2], mf_f4 (&yi)[2][2])
+                                          const char* ibase, bool more, int w, int l, const MfT16& F16,
+                                          unsigned (&sb)[2], float& inv0, mf_f4 (&yr)[2][2], mf_f4 (&yi)[2][2])
 {
     const int g = l >> 4, j = l & 15;
     const int p = w & 7, rs = w >> 3;
@@ -228,7 +229,7 @@ __device__ __forceinline__ void mr2_front(const PencilArgs<float>& a, unsigned c
     }
     // ---- stage 1 (data x table): Y[b][k1] = sum_m W32^(m k1) x[b + 32 m] ----
     {
-        const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
+        const MfT16 Fe = H16 ? F16 : mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) {
             float2 xa[4], xd[4];
@@ -274,7 +275,12 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
     unsigned sb[2];
     float inv0;
     mf_f4 yr[2][2], yi[2][2];
-    mr2_front(a, cur, nxt, twm, tws, ft, so, obase, ncv, hasprev, ibase, more, w, l, sb, inv0, yr, yi);
+    // the 16-point DFT table (8 of the 16 products) held for the tile where
+    // the registers allow
+    constexpr bool hold16 = OPK == OP_PLAIN || OPK == OP_NONE;
+    MfT16 F16{};
+    if constexpr (hold16) F16 = mr_table(ft, 0, l);
+    mr2_front<hold16>(a, cur, nxt, twm, tws, ft, so, obase, ncv, hasprev, ibase, more, w, l, F16, sb, inv0, yr, yi);
     const float2* twl = twm + l;
     const float2* tsl = tws + l;
     mf_f4 ur[2][2], ui[2][2];
@@ -294,7 +300,7 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
         }
         const MfFrag B0 = mr_split4u(v0, a.fone), B1 = mr_split4u(v1, a.fone);
         mf_f4 er, ei, orr, oi;
-        mr_mfma_b<true>(er, ei, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(er, ei, hold16 ? F16 : mr_table(ft, 0, l), B0);
         mr_mfma_b<true>(orr, oi, mr_table(ft, 2, l), B1);
         // the k-space operator and inverse stage 1's butterflies in one step:
         // with X = E +- O at rows k2 and k2 + 16 and multipliers m0, m1 there,
@@ -322,7 +328,7 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
         mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
             const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
-            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, hold16 ? F16 : mr_table(ft, 0, l));
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
         if (tk == 0) mr2_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 3);
@@ -352,7 +358,7 @@ __device__ __forceinline__ void mr2_step(const PencilArgs<float>& a, unsigned ch
         }
         const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
         mf_f4 er, ei, orr, oi;
-        mr_mfma_b<true>(er, ei, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(er, ei, hold16 ? F16 : mr_table(ft, 0, l), B0);
         mr_mfma_b<true>(orr, oi, mr_table(ft, 2, l), B1);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {

@@ -64,12 +64,13 @@ __device__ __forceinline__ void mr3_rows(unsigned (&sb)[2], int w, int l)
 // + O at k2 = 4 g + e, E - O at k2 + 16) and the input scale's inverse.  Its
 // four slices (the previous output's blocks of nxt stored, the next tile's
 // fetched into them) are spread over it.
+template <bool H16>
 __device__ __forceinline__ void mr3_forward(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
                                             unsigned char* __restrict__ nxt, const float2* __restrict__ twm,
                                             const mf_h8* __restrict__ ft, const Mr2SliceOffs (&so)[2],
                                             char* obase, int ncv, bool hasprev, const char* ibase, bool more, int w,
-                                            int l, mf_f4 (&er)[2], mf_f4 (&ei)[2], mf_f4 (&orr)[2], mf_f4 (&oi)[2],
-                                            float& inv0)
+                                            int l, const MfT16& F16, mf_f4 (&er)[2], mf_f4 (&ei)[2], mf_f4 (&orr)[2],
+                                            mf_f4 (&oi)[2], float& inv0)
 {
     mr3_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 0);
     unsigned sb[2];
@@ -84,7 +85,7 @@ __device__ __forceinline__ void mr3_forward(const PencilArgs<float>& a, unsigned
     mr3_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 1);
     mf_f4 yr[2][2], yi[2][2];
     {
-        const MfT16 Fe = mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
+        const MfT16 Fe = H16 ? F16 : mr_table(ft, 0, l), Fo = mr_table(ft, 1, l);
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) {
             const MfFrag X = mr_split4s<false>(x[tb], s0, a.fzero);
@@ -112,7 +113,7 @@ __device__ __forceinline__ void mr3_forward(const PencilArgs<float>& a, unsigned
             v0[e] = float2{yr[0][tk][e], yi[0][tk][e]};
             v1[e] = float2{yr[1][tk][e], yi[1][tk][e]};
         }
-        mr_mfma_b<true>(er[tk], ei[tk], mr_table(ft, 0, l), mr_split4u(v0, a.fone));
+        mr_mfma_b<true>(er[tk], ei[tk], H16 ? F16 : mr_table(ft, 0, l), mr_split4u(v0, a.fone));
         mr_mfma_b<true>(orr[tk], oi[tk], mr_table(ft, 2, l), mr_split4u(v1, a.fone));
     }
     mr3_slice(nxt, a, so, obase, ncv, hasprev, ibase, more, w, 3);
@@ -129,11 +130,11 @@ struct Mr3Spec {
 // the pencil's column of cur (after a barrier when WBAR: the previous output
 // has been read from cur by every wavefront).  LASTG: the last use of gm,
 // after which the next slab pair's multiplier inputs are loaded into it.
-template <int OPK, int O, bool WBAR, bool LASTG>
+template <int OPK, int O, bool WBAR, bool LASTG, bool H16>
 __device__ __forceinline__ void mr3_output(const PencilArgs<float>& a, unsigned char* __restrict__ cur,
                                            const float2* __restrict__ twm, const mf_h8* __restrict__ ft,
                                            const MrTile& T, const MrTile& Tn, bool load_g, mr_f2a (&gm)[2][4],
-                                           Mr3Spec& S, int w, int l)
+                                           Mr3Spec& S, int w, int l, const MfT16& F16)
 {
     const int g = l >> 4, j = l & 15, kb = 2 * j + 128 * g;
     const float kbf = (float)kb;
@@ -198,7 +199,7 @@ __device__ __forceinline__ void mr3_output(const PencilArgs<float>& a, unsigned 
         mr_scale(mr_wave_max(mr_absmax8(za, zb)), s2[tk], inv2[tk]);
         {
             const MfFrag XA = mr_split4s<true>(za, s2[tk], a.fzero), XB = mr_split4s<true>(zb, s2[tk], a.fzero);
-            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, mr_table(ft, 0, l));
+            mr_mfma_a<true>(ur[tk][0], ui[tk][0], XA, H16 ? F16 : mr_table(ft, 0, l));
             mr_mfma_a<true>(ur[tk][1], ui[tk][1], XB, mr_table(ft, 1, l));
         }
 #pragma unroll
@@ -228,7 +229,7 @@ __device__ __forceinline__ void mr3_output(const PencilArgs<float>& a, unsigned 
         }
         const MfFrag B0 = mr_split4s<false>(v0, sa, a.fzero), B1 = mr_split4s<false>(v1, sb2, a.fzero);
         mf_f4 dr, di;
-        mr_mfma_b<true>(dr, di, mr_table(ft, 0, l), B0);
+        mr_mfma_b<true>(dr, di, H16 ? F16 : mr_table(ft, 0, l), B0);
         mr_mfma_b<false>(dr, di, mr_table(ft, 2, l), B1);
         // rows z = 32 (4 g + e) + 2 j + tn
 #pragma unroll
@@ -330,27 +331,31 @@ __global__ void __launch_bounds__(512, 2) k_zmfma1024c(PencilArgs<float> a)
         const char* ibase = mr_tile_base(a.in, a.ia, Tn);
         const int ncv = a.ncol - Tp.col0;
         const bool hasprev = hasout && c == 0;
+        // the 16-point DFT table held for the iteration (div: the registers allow)
+        constexpr bool H16 = OPK == OP_DIV;
+        MfT16 F16{};
+        if constexpr (H16) F16 = mr_table(ftw, 0, l);
         if (c == 0)
-            mr3_forward(a, cur, nxt, twm, ftw, so, obase, ncv, hasprev, ibase, more, w, l, S.er[0], S.ei[0],
-                        S.orr[0], S.oi[0], S.inv0[0]);
+            mr3_forward<H16>(a, cur, nxt, twm, ftw, so, obase, ncv, hasprev, ibase, more, w, l, F16, S.er[0],
+                             S.ei[0], S.orr[0], S.oi[0], S.inv0[0]);
         else if (c == 1)
-            mr3_forward(a, cur, nxt, twm, ftw, so, obase, ncv, false, ibase, more, w, l, S.er[1], S.ei[1], S.orr[1],
-                        S.oi[1], S.inv0[1]);
+            mr3_forward<H16>(a, cur, nxt, twm, ftw, so, obase, ncv, false, ibase, more, w, l, F16, S.er[1], S.ei[1],
+                             S.orr[1], S.oi[1], S.inv0[1]);
         else {
-            mr3_forward(a, cur, nxt, twm, ftw, so, obase, ncv, false, ibase, more, w, l, S.er[2], S.ei[2], S.orr[2],
-                        S.oi[2], S.inv0[2]);
+            mr3_forward<H16>(a, cur, nxt, twm, ftw, so, obase, ncv, false, ibase, more, w, l, F16, S.er[2], S.ei[2],
+                             S.orr[2], S.oi[2], S.inv0[2]);
             MrTile T0 = T;
             T0.comp = 0;
             char* ob = const_cast<char*>(mr_tile_base(a.out, a.oa, T0));
             const int ncvt = a.ncol - T.col0;
             if constexpr (OPK == OP_DIV) {
-                mr3_output<OPK, 0, false, true>(a, cur, twm, ftw, T, Tn, load_g, gm, S, w, l);
+                mr3_output<OPK, 0, false, true, H16>(a, cur, twm, ftw, T, Tn, load_g, gm, S, w, l, F16);
             } else {
-                mr3_output<OPK, 0, false, false>(a, cur, twm, ftw, T, Tn, load_g, gm, S, w, l);
+                mr3_output<OPK, 0, false, false, H16>(a, cur, twm, ftw, T, Tn, load_g, gm, S, w, l, F16);
                 mr3_store(cur, a, so, ob, ncvt, w);
-                mr3_output<OPK, 1, true, false>(a, cur, twm, ftw, T, Tn, load_g, gm, S, w, l);
+                mr3_output<OPK, 1, true, false, H16>(a, cur, twm, ftw, T, Tn, load_g, gm, S, w, l, F16);
                 mr3_store(cur, a, so, ob + ostride, ncvt, w);
-                mr3_output<OPK, 2, true, true>(a, cur, twm, ftw, T, Tn, load_g, gm, S, w, l);
+                mr3_output<OPK, 2, true, true, H16>(a, cur, twm, ftw, T, Tn, load_g, gm, S, w, l, F16);
             }
             Tp = T;
             hasout = true;
