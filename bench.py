@@ -77,7 +77,7 @@ def alg_bytes_grid(ncell, bc, cin, cout, eb):
     return dict(x=P1, y=P2, z=P3, y_inv=P4, x_inv=P5, solve=P1 + P2 + P3 + P4 + P5)
 
 
-def alg_bytes_potential(ncell, bc, outputs, eb, zlaunches=None, ylaunches=None):
+def alg_bytes_potential(ncell, bc, outputs, eb, zlaunches=None, ylaunches=None, youtputs=None):
     """Byte model of the potential form (ps_solve3d_potential): one scalar
     through the x and y passes; one z transform per distinct z multiplier (A,
     which d/dx and d/dy share, d/dz and the Laplacian), each reading the one
@@ -85,7 +85,8 @@ def alg_bytes_potential(ncell, bc, outputs, eb, zlaunches=None, ylaunches=None):
     one forward transform (zlaunches 1), the spectrum and Gk read once and
     every output written; the y^-1 and x^-1 passes once per output (d/dx and
     d/dy read A's z output, times i k on load), the y^-1 reads once per
-    launch when outputs share one (ylaunches)."""
+    launch when outputs share one (ylaunches) and writes youtputs spectra
+    (d/dx A may be taken from A's by the x^-1 pass)."""
     one = alg_bytes_grid(ncell, bc, 1, 1, eb)
     back = alg_bytes_grid(ncell, bc, 1, len(outputs), eb)
     nzq = int(any(k in outputs for k in ("A", "dx", "dy"))) + int("dz" in outputs) + int("lap" in outputs)
@@ -95,9 +96,12 @@ def alg_bytes_potential(ncell, bc, outputs, eb, zlaunches=None, ylaunches=None):
     else:
         z = nzq * one["z"]
     y_inv = back["y_inv"]
+    M = [2 * n if b == 0 else n for n, b in zip(ncell, bc)]
+    H = M[0] // 2 + 1
     if ylaunches is not None and 0 < ylaunches < len(outputs):
-        M = [2 * n if b == 0 else n for n, b in zip(ncell, bc)]
-        y_inv -= (len(outputs) - ylaunches) * (M[0] // 2 + 1) * M[1] * ncell[2] * 2 * eb
+        y_inv -= (len(outputs) - ylaunches) * H * M[1] * ncell[2] * 2 * eb
+    if youtputs is not None and 0 < youtputs < len(outputs):
+        y_inv -= (len(outputs) - youtputs) * H * ncell[1] * ncell[2] * 2 * eb
     d = dict(x=one["x"], y=one["y"], z=z, y_inv=y_inv, x_inv=back["x_inv"])
     d["solve"] = sum(d.values())
     return d
@@ -428,7 +432,8 @@ def bench_config(a):
                         "ms": prof["solve"], "outputs": list(grp),
                         "pass_ms": {k: prof[k] for k in ("x", "y", "z", "y_inv", "x_inv")},
                         "z_kernel": s.zpass_variant(), "z_launches": s.potential_zlaunches(grp),
-                        "y_inv_launches": s.potential_ylaunches(grp)}
+                        "y_inv_launches": s.potential_ylaunches(grp),
+                        "y_inv_outputs": s.potential_youtputs(grp)}
                 else:
                     s.solve3d_potential(grp)
                 if s.last_status:
@@ -490,13 +495,15 @@ def bench_config(a):
         out["speedup_potential_vs_componentwise"] = comp_ms / (agrad["ms"] if agrad else step_ms)
         zl = results[line_prec]["potential"].get("z_launches")
         nzq_pot = int(any(k in pot for k in ("A", "dx", "dy"))) + int("dz" in pot) + int("lap" in pot)
-        B = alg_bytes_potential(ncell, bc, pot, eb, zl, results[line_prec]["potential"].get("y_inv_launches"))
+        B = alg_bytes_potential(ncell, bc, pot, eb, zl, results[line_prec]["potential"].get("y_inv_launches"),
+                                results[line_prec]["potential"].get("y_inv_outputs"))
         Bc = sum(alg_bytes_grid(ncell, bc, ci, co, eb)["solve"] for ci, co in cin_out)
         pp = results[line_prec]["potential"]["pass_ms"]
         out["alg_bytes_per_step"] = B["solve"]
         if agrad:
             out["alg_bytes_agrad"] = alg_bytes_potential(ncell, bc, agrad["outputs"], eb, agrad.get("z_launches"),
-                                                         agrad.get("y_inv_launches"))["solve"]
+                                                         agrad.get("y_inv_launches"),
+                                                         agrad.get("y_inv_outputs"))["solve"]
         out["alg_bytes_componentwise"] = Bc
         out["step_hbm_gbs"] = B["solve"] / (step_ms * 1e-3) / 1e9
         out["roofline"] = {"bound": "hbm", "achieved": B["z"] / (pp["z"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,

@@ -174,10 +174,15 @@ int ps_pull_potential_device(ps_ctx* ctx, void* A, void* dAdx, void* dAdy, void*
    -PS_EINVAL for a 2D context or a mask outside 1..31 */
 int ps_potential_zlaunches(const ps_ctx* ctx, int outputs);
 /* number of y^-1 launches a potential solve of `outputs` makes: one per
-   output, except that A, dA/dx and dA/dy (two or more of them) run as one
-   from one load of A's z-pass output in the FP32 and mixed forms where the
-   y length has a specialised kernel (tuning field zfan); negative as above */
+   y^-1 output, except that those from A's z-pass output (A, dA/dx, dA/dy)
+   run as one launch from one load of it in the FP32 and mixed forms where
+   the y length has a specialised kernel (tuning field zfan); negative as
+   above */
 int ps_potential_ylaunches(const ps_ctx* ctx, int outputs);
+/* number of y^-1 outputs written: one per output, except that dA/dx is taken
+   by the x^-1 pass from A's y^-1 output (times i kx) where the x length has
+   a specialised kernel (tuning field zfan); negative as above */
+int ps_potential_youtputs(const ps_ctx* ctx, int outputs);
 /* `iters` potential solves timed per pass as ps_profile_solve */
 int ps_profile_potential(ps_ctx* ctx, int outputs, int iters, double ms[6]);
 

@@ -81,6 +81,7 @@ SIGNATURES = {
     "ps_zpass_variant": (ctypes.c_int, [ctypes.c_void_p]),
     "ps_potential_zlaunches": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ps_potential_ylaunches": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "ps_potential_youtputs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "ps_dft32_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
     "ps_dft16_fragments": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int]),
     "ps_dft16_table": (ctypes.c_int, [ctypes.POINTER(ctypes.c_ushort), ctypes.c_int, ctypes.c_int]),

@@ -400,15 +400,35 @@ int select_op(ps_ctx* c, OpSel* s)
     return PS_OK;
 }
 
-// the outputs among A, d/dx A, d/dy A (bits 0..2 of the potential mask)
-// whose y^-1 passes run as one launch from one load of A's z output: at
-// least two, FP32 storage (FP32 and mixed forms), the specialised y kernel,
-// tuning zfan; 0 when they run one launch each
-int pot_yfan_mask(const ps_ctx* c, int outputs)
+// The y^-1 outputs of the potential form that come from A's z output: A
+// (ik mode 0), d/dx A (mode 1) and d/dy A (mode 2).  With xdx the x^-1
+// pass derives d/dx A from A's y^-1 output (times i kx; specialised x
+// kernels, tuning zfan), so d/dx A has no y^-1 output of its own when A is
+// wanted, and takes A's (mode 0) when it is not.  fan: these y^-1 outputs
+// run as one launch from one load of A's z output (two or more of them, FP32
+// storage, the specialised y kernel, tuning zfan).
+struct PotYPlan {
+    int n, qn[3], mode[3];
+    bool xdx, fan;
+};
+
+PotYPlan pot_yplan(const ps_ctx* c, int outputs)
 {
-    const int m = outputs & 7;
-    if (c->prec == PS_FP64 || !c->py.stat || !c->tune.zfan || c->dim2) return 0;
-    return __builtin_popcount((unsigned)m) >= 2 ? m : 0;
+    PotYPlan p{};
+    p.xdx = (outputs & PS_POT_DX) && c->px.stat && c->tune.zfan && !c->dim2;
+    for (int qn = 0; qn < 3; ++qn) {
+        if (!((outputs >> qn) & 1)) continue;
+        int mode = qn;
+        if (qn == 1 && p.xdx) {
+            if (outputs & PS_POT_A) continue;
+            mode = 0;
+        }
+        p.qn[p.n] = qn;
+        p.mode[p.n] = mode;
+        ++p.n;
+    }
+    p.fan = p.n >= 2 && c->prec != PS_FP64 && c->py.stat && c->tune.zfan && !c->dim2;
+    return p;
 }
 
 template <typename T>
@@ -540,6 +560,7 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
             if (qn == 0 ? (kmap & 7) != 0 : ((kmap >> qn) & 1) != 0) zslot[qn] = nzq++;
         zslot[1] = zslot[2] = zslot[0];
     }
+    const PotYPlan yp = potfan ? pot_yplan(c, kmap) : PotYPlan{};
     if (!wb) {
         const int n = std::max(std::max(nin, nc), nout);
         int r = ensure_comp_buffers(c, potfan ? nout : n, potfan ? 1 + nzq : n);
@@ -714,23 +735,34 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
         yi.Mx = c->M[0];
         yi.My = c->M[1];
         for (int d = 0; d < 2; ++d) yi.dk[d] = 1.0 / ((double)c->M[d] * c->dx[d]);
-        // A, d/dx A and d/dy A from one load of A's z output where the FP32
-        // specialised kernel has that form (their slots are consecutive)
-        const int yfan = pot_yfan_mask(c, kmap);
-        if (yfan) {
+        // the outputs from A's z output: one launch from one load of it
+        // where the FP32 specialised kernel has that form, else one each
+        if (yp.fan) {
             PencilArgs<T> y1 = yi;
             y1.ncomp = 1;
             y1.ik_in = 0;
-            y1.ik_fan = yfan;
+            y1.ik_fan = 0;
+            for (int i = 0; i < yp.n; ++i) {
+                y1.ik_fan |= 1 << yp.mode[i];
+                y1.fan_slot[yp.mode[i]] = slot[yp.qn[i]];
+            }
             y1.in = A2o + (size_t)zslot[0] * c->A2c * cbytes;
-            y1.out = static_cast<char*>(yi.out) + (size_t)slot[__builtin_ctz((unsigned)yfan)] * yi.oa.comp * cbytes;
             CK(static_launch_pencil<T>(c->py.p.L, y1, 1, SM_INVFAN, c->bc[1] == 0, s));
+        } else {
+            for (int i = 0; i < yp.n; ++i) {
+                PencilArgs<T> y1 = yi;
+                y1.ncomp = 1;
+                y1.ik_in = yp.mode[i];
+                y1.in = A2o + (size_t)zslot[0] * c->A2c * cbytes;
+                y1.out = static_cast<char*>(yi.out) + (size_t)slot[yp.qn[i]] * yi.oa.comp * cbytes;
+                if (const int yr = ylaunch(y1)) return yr;
+            }
         }
-        for (int qn = 0; qn < 5; ++qn) {
-            if (slot[qn] < 0 || ((yfan >> qn) & 1)) continue;
+        for (int qn = 3; qn < 5; ++qn) {
+            if (slot[qn] < 0) continue;
             PencilArgs<T> y1 = yi;
             y1.ncomp = 1;
-            y1.ik_in = (qn == 1) ? 1 : (qn == 2) ? 2 : 0;
+            y1.ik_in = 0;
             y1.in = A2o + (size_t)zslot[qn] * c->A2c * cbytes;
             y1.out = static_cast<char*>(yi.out) + (size_t)slot[qn] * yi.oa.comp * cbytes;
             if (const int yr = ylaunch(y1)) return yr;
@@ -759,6 +791,15 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     xi.out = reinterpret_cast<T*>(dst) + (long)out_first * c->n3;
     xi.r_comp = (long)out_step * c->n3;
     xi.ncomp = nout;
+    if (yp.xdx) {
+        // d/dx A from A's y^-1 output (its own slot holds A's when A is not wanted)
+        xi.remap = 1;
+        for (int k = 0; k < 5; ++k) xi.src[k] = k;
+        if (slot[0] >= 0) xi.src[slot[1]] = slot[0];
+        xi.ikx = 1 << slot[1];
+        xi.Mx = c->M[0];
+        xi.dk0 = 1.0 / ((double)c->M[0] * c->dx[0]);
+    }
     if (c->px.stat) CK(static_launch_x<T>(c->px.p.L, xi, 1, c->bc[0] == 0, s));
     else CK(launch_x_c2r<T>(c->px.p.E, xi, s));
     if (ev) CK(hipEventRecord(ev[5], s));
@@ -1493,8 +1534,16 @@ int ps_potential_ylaunches(const ps_ctx* c, int outputs)
 {
     if (!c || !c->ready) return -PS_ESTATE;
     if (c->dim2 || outputs < 1 || outputs > 31) return -PS_EINVAL;
-    const int yfan = pot_yfan_mask(c, outputs);
-    return __builtin_popcount((unsigned)outputs) - (yfan ? __builtin_popcount((unsigned)yfan) - 1 : 0);
+    const PotYPlan yp = pot_yplan(c, outputs);
+    return (yp.fan ? 1 : yp.n) + ((outputs & PS_POT_DZ) ? 1 : 0) + ((outputs & PS_POT_LAP) ? 1 : 0);
+}
+
+int ps_potential_youtputs(const ps_ctx* c, int outputs)
+{
+    if (!c || !c->ready) return -PS_ESTATE;
+    if (c->dim2 || outputs < 1 || outputs > 31) return -PS_EINVAL;
+    const PotYPlan yp = pot_yplan(c, outputs);
+    return yp.n + ((outputs & PS_POT_DZ) ? 1 : 0) + ((outputs & PS_POT_LAP) ? 1 : 0);
 }
 
 int ps_solve3d_potential(ps_ctx* c, int outputs)

@@ -88,8 +88,8 @@ template <typename T> struct PencilArgs {
     int ik_in;
     // y^-1 pass of one spectrum into several outputs (SM_INVFAN): bit m set
     // for an output with ik_in mode m (0 none, 1 i kx, 2 i ky), written to
-    // consecutive components of the output addressing in m order
-    int ik_fan;
+    // component fan_slot[m] of the output addressing
+    int ik_fan, fan_slot[3];
     // matrix-core z pass: per-lane fp16 hi/lo fragments of the 32-point DFT
     // matrix (64 lanes x 8 fragments x 8 halves), built once at setup
     const void* mfrag;
@@ -112,6 +112,11 @@ template <typename T> struct XArgs {
     int P;                // pairs per workgroup
     int npairs;
     int ncomp;
+    // x^-1 pass (specialised kernel) of the potential form: with remap,
+    // output component c reads input component src[c], times i kx (the
+    // Hermitian part; Mx, dk0 = 1/(Mx dx)) where bit c of ikx is set
+    int remap, src[5], ikx, Mx;
+    double dk0;
 };
 
 }  // namespace psamd

@@ -103,7 +103,8 @@ __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
     const int comp = blockIdx.y;
     const long ra = 2 * pair, rb = ra + 1;
     const bool va = ra < a.nrows, vb = rb < a.nrows;
-    const C* in = reinterpret_cast<const C*>(a.in) + comp * a.c_comp;
+    const C* in = reinterpret_cast<const C*>(a.in) + (a.remap ? a.src[comp] : comp) * a.c_comp;
+    const bool dx = (a.ikx >> comp) & 1;
 
     STw<L, E, T> W;
     if constexpr (s_xi_twreg<T, L>()) load_stwiddles<L, E, T>(W, t, a.plan.tw);
@@ -127,6 +128,18 @@ __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
             zb.y = -zb.y;
         }
         v[u] = C{za.x - zb.y, za.y + zb.x};
+    }
+    if (dx) {
+        // the derivative's spectrum i kh(n) X[n] (kh 0 at the Nyquist index)
+        // and its Hermitian extension -i kh(m) conj(X[m]) for both rows: kh
+        // is real and the same for the two rows packed in v, so v times i kk
+        const T kst = (T)(2.0 * kPi * a.dk0);
+#pragma unroll
+        for (int u = 0; u < E; ++u) {
+            const int pos = t + TP * u;
+            const T kk = (pos < H) ? ((2 * pos == L) ? T(0) : kst * (T)pos) : -kst * (T)(L - pos);
+            v[u] = C{-kk * v[u].y, kk * v[u].x};
+        }
     }
     C* my = lds + pp * lds_len(L);
     constexpr int WS = (64 % TP == 0) ? 1 : 0;
@@ -218,8 +231,7 @@ __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
     if constexpr (MODE == SM_INVFAN) {
         // one loaded spectrum, outputs for the set bits m of a.ik_fan (ik
         // mode m applied to a copy, then the inverse transform), written to
-        // consecutive components of the output addressing
-        int slot = 0;
+        // components fan_slot[m] of the output addressing
 #pragma unroll
         for (int m = 0; m < 3; ++m) {
             if (!((a.ik_fan >> m) & 1)) continue;
@@ -229,7 +241,7 @@ __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
             apply_ik_mode<T, E>(w, a, m, col, t, TP);
             if constexpr (TWREG) sfft<L, E, B, +1, T>(w, W, my, t);
             else sfft_g<L, E, B, +1, T>(w, a.plan.tw, my, t);
-            C* op_ = reinterpret_cast<C*>(a.out) + slot * a.oa.comp + col * a.oa.col +
+            C* op_ = reinterpret_cast<C*>(a.out) + a.fan_slot[m] * a.oa.comp + col * a.oa.col +
                      boff(slab, a.oa.slab_sh, a.oa.slab_lo, a.oa.slab_hi) +
                      boff(t, a.oa.pos_sh, a.oa.pos_lo, a.oa.pos_hi);
             if (cv) {
@@ -237,7 +249,6 @@ __global__ void __launch_bounds__(kMaxThreads, MINW) k_pencil_s(PencilArgs<T> a)
                 for (int u = 0; u < (HOUT ? E / 2 : E); ++u)
                     if (HOUT || t + TP * u < a.n_out) op_[u * ou] = w[u];
             }
-            ++slot;
         }
         return;
     }

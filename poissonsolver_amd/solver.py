@@ -227,6 +227,14 @@ class poisson_solver:
             mask |= 1 << self.POT_NAMES.index(name)
         return self._lib.ps_potential_ylaunches(self._ctx, mask)
 
+    def potential_youtputs(self, outputs=("A", "dx", "dy", "dz")) -> int:
+        """y^-1 outputs a potential solve of these outputs writes (dA/dx may be
+        taken from A's in the x^-1 pass); negative before setup3d."""
+        mask = 0
+        for name in outputs:
+            mask |= 1 << self.POT_NAMES.index(name)
+        return self._lib.ps_potential_youtputs(self._ctx, mask)
+
     def pull_potential(self, A=None, dx=None, dy=None, dz=None, lap=None):
         arrs = [self._host(a, True) for a in (A, dx, dy, dz, lap)]
         self._report(self._lib.ps_pull_potential(self._ctx, *[_ptr(a) for a in arrs]), "pull_potential")
