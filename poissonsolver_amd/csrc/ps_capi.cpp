@@ -792,11 +792,25 @@ int run_pipeline(ps_ctx* c, const void* src, int in_first, int in_step, int nin,
     xi.r_comp = (long)out_step * c->n3;
     xi.ncomp = nout;
     if (yp.xdx) {
-        // d/dx A from A's y^-1 output (its own slot holds A's when A is not wanted)
+        // d/dx A from A's y^-1 output: a second pass of A's workgroups, or,
+        // when A is not wanted, its own grid component (its slot holds A's)
         xi.remap = 1;
-        for (int k = 0; k < 5; ++k) xi.src[k] = k;
-        if (slot[0] >= 0) xi.src[slot[1]] = slot[0];
-        xi.ikx = 1 << slot[1];
+        xi.dx_also = -1;
+        int gc = 0, a_gc = -1;
+        for (int qn = 0; qn < 5; ++qn) {
+            if (slot[qn] < 0) continue;
+            if (qn == 1 && a_gc >= 0) {
+                xi.dx_also = a_gc;
+                xi.dx_slot = slot[1];
+                continue;
+            }
+            xi.src[gc] = slot[qn];
+            xi.oslot[gc] = slot[qn];
+            if (qn == 0) a_gc = gc;
+            if (qn == 1) xi.ikx |= 1 << gc;
+            ++gc;
+        }
+        xi.ncomp = gc;
         xi.Mx = c->M[0];
         xi.dk0 = 1.0 / ((double)c->M[0] * c->dx[0]);
     }

@@ -113,9 +113,11 @@ template <typename T> struct XArgs {
     int npairs;
     int ncomp;
     // x^-1 pass (specialised kernel) of the potential form: with remap,
-    // output component c reads input component src[c], times i kx (the
-    // Hermitian part; Mx, dk0 = 1/(Mx dx)) where bit c of ikx is set
-    int remap, src[5], ikx, Mx;
+    // grid component c reads input component src[c] and writes output
+    // component oslot[c], times i kx (the Hermitian part; Mx, dk0 = 1/(Mx
+    // dx)) where bit c of ikx is set; the workgroups of grid component
+    // dx_also (-1: none) also write i kx times their rows to dx_slot
+    int remap, src[5], oslot[5], ikx, dx_also, dx_slot, Mx;
     double dk0;
 };
 

@@ -104,10 +104,16 @@ __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
     const long ra = 2 * pair, rb = ra + 1;
     const bool va = ra < a.nrows, vb = rb < a.nrows;
     const C* in = reinterpret_cast<const C*>(a.in) + (a.remap ? a.src[comp] : comp) * a.c_comp;
-    const bool dx = (a.ikx >> comp) & 1;
+    // the workgroups of grid component dx_also make a second pass over their
+    // rows (re-read from the cache) for d/dx of them
+    const int npass = (a.remap && comp == a.dx_also) ? 2 : 1;
 
     STw<L, E, T> W;
     if constexpr (s_xi_twreg<T, L>()) load_stwiddles<L, E, T>(W, t, a.plan.tw);
+    C* my = lds + pp * lds_len(L);
+    for (int pass = 0; pass < npass; ++pass) {
+    const bool dx = pass == 1 || ((a.ikx >> comp) & 1);
+    const int oc = !a.remap ? comp : (pass ? a.dx_slot : a.oslot[comp]);
     C v[E];
 #pragma unroll
     for (int u = 0; u < E; ++u) {
@@ -141,11 +147,10 @@ __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
             v[u] = C{-kk * v[u].y, kk * v[u].x};
         }
     }
-    C* my = lds + pp * lds_len(L);
     constexpr int WS = (64 % TP == 0) ? 1 : 0;
     if constexpr (s_xi_twreg<T, L>()) sfft<L, E, 1, +1, T, WS>(v, W, my, t);
     else sfft_g<L, E, 1, +1, T, WS>(v, a.plan.tw, my, t);
-    T* out = reinterpret_cast<T*>(a.out) + comp * a.r_comp;
+    T* out = reinterpret_cast<T*>(a.out) + oc * a.r_comp;
 #pragma unroll
     for (int u = 0; u < (HALF ? E / 2 : E); ++u) {
         const int pos = t + TP * u;
@@ -153,6 +158,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
             if (va) out[ra * a.r_row + pos] = v[u].x;
             if (vb) out[rb * a.r_row + pos] = v[u].y;
         }
+    }
     }
 }
 
