@@ -86,7 +86,8 @@ def alg_bytes_potential(ncell, bc, outputs, eb, zlaunches=None, ylaunches=None, 
     every output written; the y^-1 and x^-1 passes once per output (d/dx and
     d/dy read A's z output, times i k on load), the y^-1 reads once per
     launch when outputs share one (ylaunches) and writes youtputs spectra
-    (d/dx A may be taken from A's by the x^-1 pass)."""
+    (d/dx A may be taken from A's by the x^-1 pass, which then reads A's
+    rows once for both when A is wanted too)."""
     one = alg_bytes_grid(ncell, bc, 1, 1, eb)
     back = alg_bytes_grid(ncell, bc, 1, len(outputs), eb)
     nzq = int(any(k in outputs for k in ("A", "dx", "dy"))) + int("dz" in outputs) + int("lap" in outputs)
@@ -100,9 +101,12 @@ def alg_bytes_potential(ncell, bc, outputs, eb, zlaunches=None, ylaunches=None, 
     H = M[0] // 2 + 1
     if ylaunches is not None and 0 < ylaunches < len(outputs):
         y_inv -= (len(outputs) - ylaunches) * H * M[1] * ncell[2] * 2 * eb
+    x_inv = back["x_inv"]
     if youtputs is not None and 0 < youtputs < len(outputs):
         y_inv -= (len(outputs) - youtputs) * H * ncell[1] * ncell[2] * 2 * eb
-    d = dict(x=one["x"], y=one["y"], z=z, y_inv=y_inv, x_inv=back["x_inv"])
+        if "A" in outputs and "dx" in outputs:
+            x_inv -= H * ncell[1] * ncell[2] * 2 * eb
+    d = dict(x=one["x"], y=one["y"], z=z, y_inv=y_inv, x_inv=x_inv)
     d["solve"] = sum(d.values())
     return d
 
