@@ -174,7 +174,7 @@ int ps_pull_potential_device(ps_ctx* ctx, void* A, void* dAdx, void* dAdy, void*
    -PS_EINVAL for a 2D context or a mask outside 1..31 */
 int ps_potential_zlaunches(const ps_ctx* ctx, int outputs);
 /* number of y^-1 launches a potential solve of `outputs` makes: one per
-   y^-1 output, except that those from A's z-pass output (A, dA/dx, dA/dy)
+   y^-1 output, except that three from A's z-pass output (A, dA/dx, dA/dy)
    run as one launch from one load of it in the FP32 and mixed forms where
    the y length has a specialised kernel (tuning field zfan); negative as
    above */

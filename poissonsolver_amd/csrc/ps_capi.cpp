@@ -405,8 +405,9 @@ int select_op(ps_ctx* c, OpSel* s)
 // pass derives d/dx A from A's y^-1 output (times i kx; specialised x
 // kernels, tuning zfan), so d/dx A has no y^-1 output of its own when A is
 // wanted, and takes A's (mode 0) when it is not.  fan: these y^-1 outputs
-// run as one launch from one load of A's z output (two or more of them, FP32
-// storage, the specialised y kernel, tuning zfan).
+// run as one launch from one load of A's z output (all three of them, FP32
+// storage, the specialised y kernel, tuning zfan; for two, two launches are
+// faster: 2 x 7.1 against 15.5 ms at 1024^3).
 struct PotYPlan {
     int n, qn[3], mode[3];
     bool xdx, fan;
@@ -427,7 +428,7 @@ PotYPlan pot_yplan(const ps_ctx* c, int outputs)
         p.mode[p.n] = mode;
         ++p.n;
     }
-    p.fan = p.n >= 2 && c->prec != PS_FP64 && c->py.stat && c->tune.zfan && !c->dim2;
+    p.fan = p.n >= 3 && c->prec != PS_FP64 && c->py.stat && c->tune.zfan && !c->dim2;
     return p;
 }
 

@@ -195,7 +195,7 @@ def _ylaunch_plan(outs, zfan, xstat, ystat, prec):
     """(y^-1 launches, y^-1 outputs) of the potential form, as the library plans them."""
     xdx = bool(zfan and xstat and "dx" in outs)
     ents = [q for q in ("A", "dx", "dy") if q in outs and not (q == "dx" and xdx and "A" in outs)]
-    fan = len(ents) >= 2 and prec != "fp64" and ystat and zfan
+    fan = len(ents) >= 3 and prec != "fp64" and ystat and zfan
     rest = int("dz" in outs) + int("lap" in outs)
     return (1 if fan else len(ents)) + rest, len(ents) + rest
 
