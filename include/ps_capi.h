@@ -180,8 +180,9 @@ int ps_potential_zlaunches(const ps_ctx* ctx, int outputs);
    above */
 int ps_potential_ylaunches(const ps_ctx* ctx, int outputs);
 /* number of y^-1 outputs written: one per output, except that dA/dx is taken
-   by the x^-1 pass from A's y^-1 output (times i kx) where the x length has
-   a specialised kernel (tuning field zfan); negative as above */
+   by the x^-1 pass from A's y^-1 output (times i kx) in the FP32 and mixed
+   forms where the x length has a specialised kernel (tuning field zfan);
+   negative as above */
 int ps_potential_youtputs(const ps_ctx* ctx, int outputs);
 /* `iters` potential solves timed per pass as ps_profile_solve */
 int ps_profile_potential(ps_ctx* ctx, int outputs, int iters, double ms[6]);

@@ -402,8 +402,8 @@ int select_op(ps_ctx* c, OpSel* s)
 
 // The y^-1 outputs of the potential form that come from A's z output: A
 // (ik mode 0), d/dx A (mode 1) and d/dy A (mode 2).  With xdx the x^-1
-// pass derives d/dx A from A's y^-1 output (times i kx; specialised x
-// kernels, tuning zfan), so d/dx A has no y^-1 output of its own when A is
+// pass derives d/dx A from A's y^-1 output (times i kx; FP32 storage,
+// specialised x kernels, tuning zfan), so d/dx A has no y^-1 output of its own when A is
 // wanted, and takes A's (mode 0) when it is not.  fan: these y^-1 outputs
 // run as one launch from one load of A's z output (all three of them, FP32
 // storage, the specialised y kernel, tuning zfan; for two, two launches are
@@ -416,7 +416,7 @@ struct PotYPlan {
 PotYPlan pot_yplan(const ps_ctx* c, int outputs)
 {
     PotYPlan p{};
-    p.xdx = (outputs & PS_POT_DX) && c->px.stat && c->tune.zfan && !c->dim2;
+    p.xdx = (outputs & PS_POT_DX) && c->prec != PS_FP64 && c->px.stat && c->tune.zfan && !c->dim2;
     for (int qn = 0; qn < 3; ++qn) {
         if (!((outputs >> qn) & 1)) continue;
         int mode = qn;

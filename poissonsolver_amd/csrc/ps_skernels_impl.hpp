@@ -88,8 +88,10 @@ __global__ void __launch_bounds__(kMaxThreads, (s_x_minw<T, L>())) k_x_r2c_s(XAr
 
 // ---------------------------------------------------------------------------
 // x pass, inverse.  HALF: only the first L/2 outputs are stored (truncation).
+// FAN: the workgroups of grid component a.dx_also make a second pass (a
+// separate instantiation: the loop costs registers in the one-pass kernel).
 // ---------------------------------------------------------------------------
-template <typename T, int L, int HALF>
+template <typename T, int L, int HALF, int FAN = 0>
 __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
 {
     using C = cplx<T>;
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(kMaxThreads) k_x_c2r_s(XArgs<T> a)
     const C* in = reinterpret_cast<const C*>(a.in) + (a.remap ? a.src[comp] : comp) * a.c_comp;
     // the workgroups of grid component dx_also make a second pass over their
     // rows (re-read from the cache) for d/dx of them
-    const int npass = (a.remap && comp == a.dx_also) ? 2 : 1;
+    const int npass = (FAN && a.remap && comp == a.dx_also) ? 2 : 1;
 
     STw<L, E, T> W;
     if constexpr (s_xi_twreg<T, L>()) load_stwiddles<L, E, T>(W, t, a.plan.tw);
@@ -594,6 +596,19 @@ template <typename T, int L> hipError_t s_launch_x(const XArgs<T>& a, int invers
         } else {
             if ((e = allow_lds(k_x_r2c_s<T, L, 0>, shm)) != hipSuccess) return e;
             hipLaunchKernelGGL((k_x_r2c_s<T, L, 0>), grid, block, shm, s, a);
+        }
+    } else if (a.remap && a.dx_also >= 0) {
+        // the second pass for d/dx of one component's rows (FP32)
+        if constexpr (sizeof(T) == 4) {
+            if (half) {
+                if ((e = allow_lds(k_x_c2r_s<T, L, 1, 1>, shm)) != hipSuccess) return e;
+                hipLaunchKernelGGL((k_x_c2r_s<T, L, 1, 1>), grid, block, shm, s, a);
+            } else {
+                if ((e = allow_lds(k_x_c2r_s<T, L, 0, 1>, shm)) != hipSuccess) return e;
+                hipLaunchKernelGGL((k_x_c2r_s<T, L, 0, 1>), grid, block, shm, s, a);
+            }
+        } else {
+            return hipErrorInvalidValue;
         }
     } else {
         if (half) {
