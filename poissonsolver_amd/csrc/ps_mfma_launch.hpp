@@ -28,7 +28,8 @@ inline bool zmfma_reg_layout(int L, int variant) { return (L == 512 || L == 1024
 // potential form): mask bit 0 A (Gk), bit 1 d/dz A (i kzh Gk), bit 2 the
 // Laplacian (-|k|^2 Gk); output o (the o-th set bit) is written to component
 // o of a.out (stride a.oa.comp), the input is component 0 of a.in.  At least
-// two outputs, transforms of 512 on the register-pencil kernel (variant 1)
+// two outputs, variant 1, transforms of 512 (one pencil per wavefront) and
+// 2048 (one pencil, both residues, per wavefront)
 bool zmfma_fan_ok(const PencilArgs<float>& a, int mask, int variant);
 hipError_t launch_zmfma_fan(const PencilArgs<float>& a, int mask, hipStream_t s);
 
